@@ -1,0 +1,143 @@
+/*
+ * tbj.h — C ABI of the MI355X equi-join library (libtbj.so).
+ *
+ * This is the boundary the reference's join operator API forwards to.  The
+ * reference's operator is
+ *     result_t* NAME(relation_t* R, relation_t* S, int nthreads)
+ * (Include/data-types.h:68-71, Joins/radix_join.h:17-24) selected by name
+ * through algorithm_t tables (native.cpp:34-46); teebench/joins.h keeps
+ * those C++ entry points and forwards to the functions below.  Plain
+ * pointers and sizes only: no torch, no HIP types in the signatures.
+ *
+ * Status codes replace the reference's process exits (util.c:11-18,
+ * radix_join.c:1331-1334, CHT.hpp:203-204).
+ */
+#ifndef TBJ_H
+#define TBJ_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* 8-byte tuple, layout-identical to the reference's struct row_t
+ * {type_key key; type_value payload;} (Include/data-types.h:44-47). */
+typedef struct tbj_tuple { uint32_t key; uint32_t payload; } tbj_tuple;
+/* One materialised match (R.payload, S.payload): the two payload fields of
+ * the reference's output_list_t (Include/data-types.h:49-54). */
+typedef struct tbj_pair { uint32_t r_payload; uint32_t s_payload; } tbj_pair;
+
+/* 16-byte tuples for 64-bit keys (no reference counterpart: the reference
+ * hard-wires 8-byte tuples, Include/data-types.h:32-47). */
+typedef struct tbj_tuple64 { uint64_t key; uint64_t payload; } tbj_tuple64;
+typedef struct tbj_pair64 { uint64_t r_payload; uint64_t s_payload; } tbj_pair64;
+
+/* Join algorithms, named like the reference's -a selector
+ * (native.cpp:34-46; RHO_atomic from Enclave/TrustedJoins/secure_joins.cpp:122). */
+enum tbj_algo {
+    TBJ_RHO = 0,        /* radix partition + per-partition bucket join (radix_join.c:1557) */
+    TBJ_RHT = 1,        /* radix partition + histogram join (radix_join.c:1563) */
+    TBJ_RJ = 2,         /* single-stream radix join (radix_join.c:1398) */
+    TBJ_PHT = 3,        /* no-partitioning shared hash table (no_partitioning_join.h:543) */
+    TBJ_CHT = 4,        /* concise hash table (cht/CHTJoinWrapper.cpp:56) */
+    TBJ_RSM = 5,        /* radix partition + sort-merge per partition (radix_sortmerge_join.c:102) */
+    TBJ_PSM = 6,        /* sort both relations + merge (psm/parallel_sortmerge_join.h:61) */
+    TBJ_RHO_ATOMIC = 7, /* RHO with lock-free task indices (rho_atomic/radix_join_atomic.cpp:1278) */
+    TBJ_NPO_ST = 8,     /* single-table variant of PHT (no_partitioning_join.h:474) */
+    TBJ_ALGO_COUNT = 9
+};
+
+enum tbj_partitioner {
+    TBJ_PART_AUTO = 0,    /* radix */
+    TBJ_PART_RADIX = 1,   /* low key bits, as prj_params.h:18-23 */
+    TBJ_PART_LEARNED = 2  /* piecewise-linear key-CDF range partitioner, radix fallback */
+};
+
+enum tbj_status {
+    TBJ_OK = 0,
+    TBJ_ERR_INVALID = -1,      /* bad argument */
+    TBJ_ERR_NO_DEVICE = -2,    /* no HIP device / library not initialised */
+    TBJ_ERR_OOM = -3,          /* device allocation failed or workspace cap hit */
+    TBJ_ERR_HIP = -4,          /* a HIP runtime call failed */
+    TBJ_ERR_UNSUPPORTED = -5,  /* algorithm/option combination not available */
+    TBJ_ERR_TOO_LARGE = -6     /* relation exceeds 2^32 - 2^20 tuples */
+};
+
+typedef struct tbj_options {
+    int32_t partitioner;        /* enum tbj_partitioner */
+    int32_t radix_bits;         /* total partition bits; 0 = chosen from |R| */
+    int32_t passes;             /* 1 or 2 global partition passes; 0 = auto */
+    int32_t materialize;        /* 1: write (R.payload, S.payload) pairs */
+    uint64_t workspace_cap;     /* bytes; 0 = no cap (a cap forces TBJ_ERR_OOM) */
+    uint32_t learned_bound;     /* max R tuples per learned partition before the
+                                   radix fallback; 0 = the LDS capacity */
+    int32_t learned_samples;    /* keys sampled to fit the CDF model; 0 = auto */
+    int32_t s_chunk;            /* S tuples per join task; 0 = auto */
+    int32_t reserved[5];
+} tbj_options;
+
+typedef struct tbj_result {
+    uint64_t matches;           /* total matches (the reference's totalresults) */
+    uint64_t checksum;          /* sum of (R.payload + S.payload) over matches, mod 2^64
+                                   (CHT's join_result_t.checksum, data-types.h:86-92) */
+    uint64_t pairs_written;     /* min(matches, pairs_cap) when materialising */
+    int32_t partitioner_used;   /* enum tbj_partitioner actually used */
+    int32_t fallback;           /* 1 if the learned partitioner fell back to radix */
+    int32_t radix_bits;
+    int32_t passes;
+    uint32_t num_partitions;
+    uint32_t max_r_partition;   /* largest R partition (tuples) */
+    uint64_t s_dropped;         /* S tuples dropped as outside R's key range */
+    float ms_partition;         /* device time of the partition phase */
+    float ms_join;              /* device time of build/probe (or sort/merge) */
+    float ms_total;             /* device time of the whole call */
+    float reserved_f;
+} tbj_result;
+
+/* Initialise on a HIP device (0-based).  Safe to call more than once. */
+int tbj_init(int device);
+/* Release cached device workspace. */
+void tbj_shutdown(void);
+void tbj_default_options(tbj_options* opt);
+const char* tbj_status_str(int status);
+/* Number of compiled join algorithms and their names (the -a strings). */
+const char* tbj_algo_name(int algo);
+int tbj_algo_from_name(const char* name);
+
+/*
+ * Equi-join of two device-resident relations.  R and S point to HBM; they
+ * are not modified (unlike the reference RHO/PSM, which clobber their inputs,
+ * radix_join.c:596-606, parallel_sortmerge_join.h:78-86).  pairs (device,
+ * may be NULL) receives up to pairs_cap matches when opt->materialize is set,
+ * in no particular order.  stream is a hipStream_t (NULL = default stream).
+ * Synchronous: returns after the result is complete.
+ */
+int tbj_join(int algo,
+             const tbj_tuple* R, uint64_t nR,
+             const tbj_tuple* S, uint64_t nS,
+             tbj_pair* pairs, uint64_t pairs_cap,
+             const tbj_options* opt, tbj_result* res, void* stream);
+
+/* Same with host pointers: copies R and S to HBM, joins, copies pairs back. */
+int tbj_join_host(int algo,
+                  const tbj_tuple* R, uint64_t nR,
+                  const tbj_tuple* S, uint64_t nS,
+                  tbj_pair* pairs, uint64_t pairs_cap,
+                  const tbj_options* opt, tbj_result* res);
+
+/*
+ * 64-bit keys, 16-byte tuples, sort-merge join (algo TBJ_PSM or TBJ_RSM).
+ * pairs come out ordered by (key, R.payload, S.payload) when the learned
+ * range partitioner is used or algo is TBJ_PSM.
+ */
+int tbj_join64(int algo,
+               const tbj_tuple64* R, uint64_t nR,
+               const tbj_tuple64* S, uint64_t nS,
+               tbj_pair64* pairs, uint64_t pairs_cap,
+               const tbj_options* opt, tbj_result* res, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

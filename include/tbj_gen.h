@@ -1,0 +1,61 @@
+/*
+ * tbj_gen.h — relation generators of libtbj.so (host side).
+ *
+ * Bit-exact with the reference's generators on the same seeds: the keys come
+ * from the same glibc TYPE_3 rand() stream and the same shuffle and Zipf
+ * arithmetic (App/Lib/generator.cpp:20-23,76-111,144-171,353-379,475-555,
+ * 639-662; App/Lib/genzipf.c:28-158), but independent shuffled blocks and the
+ * Zipf lookups run on several host threads.
+ *
+ * The reference never writes payloads (they are whatever the allocation
+ * held, zero in practice).  payload_mode selects TBJ_PAYLOAD_ZERO (the same
+ * zero payloads) or TBJ_PAYLOAD_ROWID (payload = row index, so that
+ * materialised pairs identify both tuples).
+ */
+#ifndef TBJ_GEN_H
+#define TBJ_GEN_H
+#include <stdint.h>
+#include "tbj.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum tbj_payload_mode { TBJ_PAYLOAD_ZERO = 0, TBJ_PAYLOAD_ROWID = 1 };
+
+/* glibc rand() restated: the first n outputs after srand(seed). */
+void tbj_rand_stream(uint32_t seed, uint64_t n, int32_t* out);
+
+/* create_relation_pk: keys 1..n shuffled (generator.cpp:353-379). */
+int tbj_gen_pk(uint32_t seed, uint64_t n, tbj_tuple* out, int payload_mode, int nthreads);
+/* create_relation_fk: n/maxid shuffled blocks of 1..maxid (+ remainder)
+ * (generator.cpp:475-514). */
+int tbj_gen_fk(uint32_t seed, uint64_t n, int64_t maxid, tbj_tuple* out, int payload_mode,
+               int nthreads);
+/* create_relation_fk_sel (generator.cpp:516-555); maxid = 100*|R|/selectivity
+ * as native.cpp:113-114 computes it. */
+int tbj_gen_fk_sel(uint32_t seed, uint64_t n, int64_t maxid, tbj_tuple* out, int payload_mode,
+                   int nthreads);
+/* create_relation_zipf / gen_zipf (generator.cpp:639-662, genzipf.c:97-158). */
+int tbj_gen_zipf(uint32_t seed, uint64_t n, uint32_t alphabet, double z, tbj_tuple* out,
+                 int payload_mode, int nthreads);
+
+/* Remaps every odd-indexed tuple's key to key + offset (keys outside R's
+ * domain when offset >= max R key): the out-of-domain half of the skewed
+ * 50%-selectivity workload.  No reference counterpart (the reference CLI
+ * cannot combine -z with -l, native.cpp:107-118). */
+void tbj_remap_odd(tbj_tuple* rel, uint64_t n, uint32_t offset);
+
+/* 16-byte tuples: n unique nonzero random 64-bit keys (splitmix64 stream
+ * with rejection of duplicates), payload = row index. */
+int tbj_gen_unique64(uint64_t seed, uint64_t n, tbj_tuple64* out, int nthreads);
+/* n probe tuples whose keys are Zipf(z) ranks over R's keys (rank 1 = R[0]),
+ * payload = row index.  Ranks are drawn by inverse-CDF lookup over a
+ * double-precision Zipf table with a splitmix64 uniform stream. */
+int tbj_gen_zipf_fk64(uint64_t seed, uint64_t n, const tbj_tuple64* R, uint64_t nR, double z,
+                      tbj_tuple64* out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,80 @@
+// Device-side helpers shared by the join kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define TBJ_WAVE 64
+#define TBJ_DROP 0xFFFFFFFFu
+
+namespace tbj {
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (TBJ_WAVE - 1); }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / TBJ_WAVE; }
+
+// Inclusive prefix sum across the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < TBJ_WAVE; d <<= 1) {
+        uint32_t y = __shfl_up(x, d, TBJ_WAVE);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_reduce_u64(uint64_t x) {
+#pragma unroll
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, TBJ_WAVE);
+    return x;
+}
+
+// Exclusive prefix sum across a block of BLOCK threads.  `scratch` must hold
+// BLOCK/64 + 1 words of LDS.  Returns the thread's exclusive prefix; *total
+// receives the block-wide sum.  Contains two barriers; every thread of the
+// block must call it.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* scratch,
+                                                         uint32_t* total) {
+    constexpr int NW = BLOCK / TBJ_WAVE;
+    const uint32_t lane = lane_id(), wid = wave_id();
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == TBJ_WAVE - 1) scratch[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        uint32_t w = lane < NW ? scratch[lane] : 0;
+        uint32_t wi = wave_inclusive_scan(w);
+        if (lane < NW) scratch[lane] = wi - w;  // exclusive wave offsets
+        if (lane == NW - 1) scratch[NW] = wi;
+    }
+    __syncthreads();
+    uint32_t r = scratch[wid] + inc - v;
+    *total = scratch[NW];
+    return r;
+}
+
+// In-place exclusive scan of `n` LDS words (n <= BLOCK * 64) by a block.
+// Returns the total.  Ends with a barrier.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_scan_lds(uint32_t* a, uint32_t n, uint32_t* scratch) {
+    const uint32_t per = (n + BLOCK - 1) / BLOCK;
+    const uint32_t beg = threadIdx.x * per;
+    const uint32_t end = beg + per < n ? beg + per : n;
+    uint32_t s = 0;
+    for (uint32_t i = beg; i < end; i++) s += a[i];
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<BLOCK>(s, scratch, &total);
+    for (uint32_t i = beg; i < end; i++) {
+        uint32_t c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ __forceinline__ uint32_t next_pow2_u32(uint32_t v) {
+    if (v <= 1) return 1;
+    return 1u << (32 - __clz(v - 1));
+}
+
+}  // namespace tbj

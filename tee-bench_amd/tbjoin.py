@@ -1,0 +1,245 @@
+"""ctypes binding of libtbj.so, the MI355X equi-join library (include/tbj.h).
+
+Mirrors the reference's join operator API: ``join("RHO", R, S)`` is the
+device-side counterpart of ``result_t* RHO(relation_t*, relation_t*, int)``
+(Joins/radix_join.c:1557-1561), selected by the same names as the reference's
+``-a`` flag (native.cpp:34-46).  Relations are device tensors of 8-byte tuples
+viewed as int64/uint32 pairs; PyTorch only provides the HBM buffers.
+
+The library is required: every entry point raises when libtbj.so is missing
+or no HIP device is visible.  There is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtbj.so")
+
+ALGOS = {"RHO": 0, "RHT": 1, "RJ": 2, "PHT": 3, "CHT": 4, "RSM": 5, "PSM": 6,
+         "RHO_atomic": 7, "NPO_st": 8}
+PART_AUTO, PART_RADIX, PART_LEARNED = 0, 1, 2
+PARTITIONERS = {"auto": PART_AUTO, "radix": PART_RADIX, "learned": PART_LEARNED}
+
+TBJ_OK = 0
+ERR_INVALID, ERR_NO_DEVICE, ERR_OOM, ERR_HIP, ERR_UNSUPPORTED, ERR_TOO_LARGE = -1, -2, -3, -4, -5, -6
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("partitioner", ctypes.c_int32), ("radix_bits", ctypes.c_int32),
+                ("passes", ctypes.c_int32), ("materialize", ctypes.c_int32),
+                ("workspace_cap", ctypes.c_uint64), ("learned_bound", ctypes.c_uint32),
+                ("learned_samples", ctypes.c_int32), ("s_chunk", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("matches", ctypes.c_uint64), ("checksum", ctypes.c_uint64),
+                ("pairs_written", ctypes.c_uint64), ("partitioner_used", ctypes.c_int32),
+                ("fallback", ctypes.c_int32), ("radix_bits", ctypes.c_int32),
+                ("passes", ctypes.c_int32), ("num_partitions", ctypes.c_uint32),
+                ("max_r_partition", ctypes.c_uint32), ("s_dropped", ctypes.c_uint64),
+                ("ms_partition", ctypes.c_float), ("ms_join", ctypes.c_float),
+                ("ms_total", ctypes.c_float), ("reserved_f", ctypes.c_float)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved_f"}
+
+
+class TbjError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"tbj status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads libtbj.so (built by __graft_entry__.build()); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u64 = ctypes.c_void_p, ctypes.c_uint64
+        L.tbj_init.argtypes = [ctypes.c_int]
+        L.tbj_init.restype = ctypes.c_int
+        L.tbj_status_str.argtypes = [ctypes.c_int]
+        L.tbj_status_str.restype = ctypes.c_char_p
+        L.tbj_algo_name.argtypes = [ctypes.c_int]
+        L.tbj_algo_name.restype = ctypes.c_char_p
+        L.tbj_algo_from_name.argtypes = [ctypes.c_char_p]
+        L.tbj_algo_from_name.restype = ctypes.c_int
+        L.tbj_default_options.argtypes = [ctypes.POINTER(Options)]
+        L.tbj_join.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64,
+                               ctypes.POINTER(Options), ctypes.POINTER(Result), vp]
+        L.tbj_join.restype = ctypes.c_int
+        L.tbj_join_host.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64,
+                                    ctypes.POINTER(Options), ctypes.POINTER(Result)]
+        L.tbj_join_host.restype = ctypes.c_int
+        L.tbj_join64.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64,
+                                 ctypes.POINTER(Options), ctypes.POINTER(Result), vp]
+        L.tbj_join64.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != TBJ_OK:
+        raise TbjError(status, lib().tbj_status_str(status).decode())
+
+
+def init(device=0):
+    check(lib().tbj_init(device))
+
+
+def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
+                 workspace_cap=0, learned_bound=0, learned_samples=0, s_chunk=0):
+    o = Options()
+    lib().tbj_default_options(ctypes.byref(o))
+    o.partitioner = PARTITIONERS[partitioner] if isinstance(partitioner, str) else int(partitioner)
+    o.radix_bits, o.passes = radix_bits, passes
+    o.materialize = 1 if materialize else 0
+    o.workspace_cap, o.learned_bound = workspace_cap, learned_bound
+    o.learned_samples, o.s_chunk = learned_samples, s_chunk
+    return o
+
+
+def _algo(name):
+    if isinstance(name, int):
+        return name
+    if name not in ALGOS:
+        raise ValueError(f"unknown join algorithm {name!r}")
+    return ALGOS[name]
+
+
+def join_ptr(algo, r_ptr, nR, s_ptr, nS, pairs_ptr=0, pairs_cap=0, options=None, stream=0):
+    """Joins device relations given as raw HBM pointers; returns a Result."""
+    o = options if options is not None else make_options()
+    res = Result()
+    check(lib().tbj_join(_algo(algo), r_ptr, nR, s_ptr, nS, pairs_ptr or None, pairs_cap,
+                         ctypes.byref(o), ctypes.byref(res), stream or None))
+    return res
+
+
+def join(algo, R, S, pairs=None, **opts):
+    """Joins device tensors R, S (int64 [n] or uint32/int32 [n, 2], 8-byte tuples).
+
+    pairs: optional device tensor with room for the (R.payload, S.payload)
+    matches (int64 [cap] or int32 [cap, 2]); materialises when given.
+    """
+    for t in (R, S):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("relations must be contiguous device tensors")
+    nR = R.numel() * R.element_size() // 8
+    nS = S.numel() * S.element_size() // 8
+    cap = 0
+    if pairs is not None:
+        cap = pairs.numel() * pairs.element_size() // 8
+        opts.setdefault("materialize", True)
+    o = make_options(**opts)
+    import torch
+    stream = torch.cuda.current_stream(R.device).cuda_stream
+    return join_ptr(algo, R.data_ptr(), nR, S.data_ptr(), nS,
+                    pairs.data_ptr() if pairs is not None else 0, cap, o, stream)
+
+
+def join64_ptr(algo, r_ptr, nR, s_ptr, nS, pairs_ptr=0, pairs_cap=0, options=None, stream=0):
+    o = options if options is not None else make_options()
+    res = Result()
+    check(lib().tbj_join64(_algo(algo), r_ptr, nR, s_ptr, nS, pairs_ptr or None, pairs_cap,
+                           ctypes.byref(o), ctypes.byref(res), stream or None))
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Relation generators (host, include/tbj_gen.h): numpy arrays of tuples,
+# shape (n, 2), columns (key, payload).
+
+PAYLOAD_ZERO, PAYLOAD_ROWID = 0, 1
+_gen_ready = False
+
+
+def _gen_lib():
+    global _gen_ready
+    L = lib()
+    if not _gen_ready:
+        vp, u64 = ctypes.c_void_p, ctypes.c_uint64
+        L.tbj_rand_stream.argtypes = [ctypes.c_uint32, u64, vp]
+        L.tbj_gen_pk.argtypes = [ctypes.c_uint32, u64, vp, ctypes.c_int, ctypes.c_int]
+        L.tbj_gen_fk.argtypes = [ctypes.c_uint32, u64, ctypes.c_int64, vp, ctypes.c_int,
+                                 ctypes.c_int]
+        L.tbj_gen_fk_sel.argtypes = L.tbj_gen_fk.argtypes
+        L.tbj_gen_zipf.argtypes = [ctypes.c_uint32, u64, ctypes.c_uint32, ctypes.c_double, vp,
+                                   ctypes.c_int, ctypes.c_int]
+        L.tbj_remap_odd.argtypes = [vp, u64, ctypes.c_uint32]
+        L.tbj_gen_unique64.argtypes = [u64, u64, vp, ctypes.c_int]
+        L.tbj_gen_zipf_fk64.argtypes = [u64, u64, vp, u64, ctypes.c_double, vp, ctypes.c_int]
+        for f in ("tbj_gen_pk", "tbj_gen_fk", "tbj_gen_fk_sel", "tbj_gen_zipf",
+                  "tbj_gen_unique64", "tbj_gen_zipf_fk64"):
+            getattr(L, f).restype = ctypes.c_int
+        _gen_ready = True
+    return L
+
+
+def _np():
+    import numpy
+    return numpy
+
+
+def _alloc(n, out, dtype_bits=32):
+    np = _np()
+    dt = np.uint32 if dtype_bits == 32 else np.uint64
+    if out is None:
+        out = np.empty((n, 2), dt)
+    assert out.dtype == dt and out.shape == (n, 2) and out.flags.c_contiguous
+    return out
+
+
+def rand_stream(seed, n):
+    np = _np()
+    out = np.empty(n, np.int32)
+    _gen_lib().tbj_rand_stream(seed, n, out.ctypes.data)
+    return out
+
+
+def gen_pk(seed, n, payload=PAYLOAD_ROWID, nthreads=0, out=None):
+    out = _alloc(n, out)
+    check(_gen_lib().tbj_gen_pk(seed, n, out.ctypes.data, payload, nthreads))
+    return out
+
+
+def gen_fk(seed, n, maxid, payload=PAYLOAD_ROWID, nthreads=0, out=None):
+    out = _alloc(n, out)
+    check(_gen_lib().tbj_gen_fk(seed, n, maxid, out.ctypes.data, payload, nthreads))
+    return out
+
+
+def gen_fk_sel(seed, n, maxid, payload=PAYLOAD_ROWID, nthreads=0, out=None):
+    out = _alloc(n, out)
+    check(_gen_lib().tbj_gen_fk_sel(seed, n, maxid, out.ctypes.data, payload, nthreads))
+    return out
+
+
+def gen_zipf(seed, n, alphabet, z, payload=PAYLOAD_ROWID, nthreads=0, out=None):
+    out = _alloc(n, out)
+    check(_gen_lib().tbj_gen_zipf(seed, n, alphabet, z, out.ctypes.data, payload, nthreads))
+    return out
+
+
+def remap_odd(rel, offset):
+    _gen_lib().tbj_remap_odd(rel.ctypes.data, rel.shape[0], offset)
+    return rel
+
+
+def gen_unique64(seed, n, nthreads=0):
+    out = _alloc(n, None, 64)
+    check(_gen_lib().tbj_gen_unique64(seed, n, out.ctypes.data, nthreads))
+    return out
+
+
+def gen_zipf_fk64(seed, n, R, z, nthreads=0):
+    out = _alloc(n, None, 64)
+    check(_gen_lib().tbj_gen_zipf_fk64(seed, n, R.ctypes.data, R.shape[0], z, out.ctypes.data,
+                                       nthreads))
+    return out

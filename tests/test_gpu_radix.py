@@ -1,0 +1,125 @@
+"""GPU parity of the partitioned joins (RHO, RHT, RJ, RHO_atomic) against the
+oracle: exact counts, CHT checksum and sorted (R.payload, S.payload) pairs."""
+import numpy as np
+import pytest
+
+import gpu_util as gu
+
+pytestmark = pytest.mark.gpu
+
+RADIX_ALGOS = ["RHO", "RHT", "RJ", "RHO_atomic"]
+
+
+def rel(keys, payload=None):
+    keys = np.asarray(keys, np.uint32)
+    out = np.empty((len(keys), 2), np.uint32)
+    out[:, 0] = keys
+    out[:, 1] = np.arange(len(keys), dtype=np.uint32) if payload is None else payload
+    return out
+
+
+def assert_join(tbj, oracle, algo, R, S, **opts):
+    res, got = gu.run(tbj, algo, R, S, cap=None, **opts)
+    n, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    assert res.matches == n
+    assert res.pairs_written == min(n, S.shape[0])
+    if n <= S.shape[0]:
+        ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+        assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+    assert res.checksum == gu.checksum(ra, sa)
+    return res
+
+
+@pytest.mark.parametrize("algo", RADIX_ALGOS)
+def test_reference_generators_small(tbj, oracle, algo):
+    nR, nS = 1 << 16, 1 << 18
+    R = tbj.gen_pk(11111, nR)
+    for S in (tbj.gen_fk(22222, nS, nR), tbj.gen_fk_sel(22222, nS, 2 * nR),
+              tbj.gen_zipf(22222, nS, nR, 1.0)):
+        assert_join(tbj, oracle, algo, R, S)
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+@pytest.mark.parametrize("bits", [4, 9, 13, 16])
+def test_pass_shapes(tbj, oracle, passes, bits):
+    nR, nS = 100003, 300007
+    R = tbj.gen_pk(7, nR)
+    S = tbj.gen_zipf(8, nS, nR, 0.8)
+    if passes == 1 and bits > 11:
+        pytest.skip("one pass holds at most 11 bits")
+    res = assert_join(tbj, oracle, "RHO", R, S, radix_bits=bits, passes=passes)
+    assert res.passes == passes and res.radix_bits == bits
+
+
+def test_duplicates_and_extreme_keys(tbj, oracle):
+    rng = np.random.default_rng(1)
+    rk = np.concatenate([rng.integers(0, 5000, 20000), [0, 0, 0xFFFFFFFF, 0xFFFFFFFE]])
+    sk = np.concatenate([rng.integers(0, 6000, 30000), [0, 0xFFFFFFFF, 0xFFFFFFFF]])
+    R, S = rel(rk), rel(sk)
+    n = oracle.count(R[:, 0], S[:, 0])
+    for algo in ("RHO", "RHT"):
+        res, got = gu.run(tbj, algo, R, S, cap=n)
+        _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+        ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+        assert res.matches == n and np.array_equal(ga, ra) and np.array_equal(gb, sa)
+
+
+def test_oversized_partition_is_chunked(tbj, oracle):
+    # one key repeated far beyond the LDS table capacity on the build side
+    rk = np.concatenate([np.full(10000, 42), np.arange(1, 3000)])
+    sk = np.concatenate([np.full(50, 42), np.arange(1, 9000)])
+    R, S = rel(rk), rel(sk)
+    n = oracle.count(R[:, 0], S[:, 0])
+    res, got = gu.run(tbj, "RHO", R, S, cap=n)
+    _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert res.matches == n and res.max_r_partition > 4096
+    assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+
+
+def test_empty_and_tiny(tbj, oracle):
+    R, S = rel([5]), rel([5, 5, 6])
+    res, got = gu.run(tbj, "RHO", R, S)
+    assert res.matches == 2 and sorted(got[:, 1].tolist()) == [0, 1]
+    res, _ = gu.run(tbj, "RHO", rel([]), S, materialize=False)
+    assert res.matches == 0
+    res, _ = gu.run(tbj, "RHO", R, rel([]), materialize=False)
+    assert res.matches == 0
+
+
+def test_output_capacity_truncates_but_counts(tbj, oracle):
+    nR, nS = 5000, 20000
+    R = tbj.gen_pk(3, nR)
+    S = tbj.gen_fk(4, nS, nR)
+    res, got = gu.run(tbj, "RHO", R, S, cap=1234)
+    assert res.matches == nS and res.pairs_written == 1234
+    assert np.all(R[got[:, 0], 0] == S[got[:, 1], 0])
+
+
+@pytest.mark.parametrize("z", [0.0, 1.0, 1.5])
+def test_learned_partitioner_matches_radix(tbj, oracle, z):
+    nR, nS = 1 << 18, 1 << 20
+    R = tbj.gen_pk(11111, nR)
+    S = tbj.gen_zipf(22222, nS, nR, z) if z else tbj.gen_fk(22222, nS, nR)
+    S = tbj.remap_odd(S, nR)  # half of S outside R's key range
+    a = assert_join(tbj, oracle, "RHO", R, S, partitioner="learned")
+    b = assert_join(tbj, oracle, "RHO", R, S, partitioner="radix")
+    assert a.partitioner_used == tbj.PART_LEARNED and a.fallback == 0
+    assert a.s_dropped == nS // 2 and b.s_dropped == 0
+    assert (a.matches, a.checksum) == (b.matches, b.checksum)
+
+
+def test_learned_fallback_to_radix(tbj, oracle):
+    nR, nS = 1 << 16, 1 << 18
+    R = tbj.gen_pk(5, nR)
+    S = tbj.gen_fk(6, nS, nR)
+    res = assert_join(tbj, oracle, "RHO", R, S, partitioner="learned", learned_bound=16)
+    assert res.fallback == 1 and res.partitioner_used == tbj.PART_RADIX
+
+
+def test_reference_config1_count(tbj):
+    # |R| = |S| = 2^21, default generators: the reference prints Matches = 2097152
+    R = tbj.gen_pk(11111, 1 << 21, payload=tbj.PAYLOAD_ZERO)
+    S = tbj.gen_fk(22222, 1 << 21, 1 << 21, payload=tbj.PAYLOAD_ZERO)
+    res, _ = gu.run(tbj, "RHO", R, S, materialize=False)
+    assert res.matches == 2097152 and res.checksum == 0

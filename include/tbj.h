@@ -74,7 +74,12 @@ typedef struct tbj_options {
                                    radix fallback; 0 = the LDS capacity */
     int32_t learned_samples;    /* keys sampled to fit the CDF model; 0 = auto */
     int32_t s_chunk;            /* S tuples per join task; 0 = auto */
-    int32_t reserved[5];
+    int32_t sampled_regions;    /* 1: size second-pass regions from a 1/16 sample
+                                   taken by the first histogram instead of exact
+                                   per-tile histograms (falls back to exact on
+                                   overflow) */
+    int32_t profile;            /* 1: record per-phase device times in phase_ms */
+    int32_t reserved[3];
 } tbj_options;
 
 typedef struct tbj_result {
@@ -92,7 +97,11 @@ typedef struct tbj_result {
     float ms_partition;         /* device time of the partition phase */
     float ms_join;              /* device time of build/probe (or sort/merge) */
     float ms_total;             /* device time of the whole call */
-    float reserved_f;
+    int32_t redone_exact;       /* 1 if a sample-sized region overflowed and the
+                                   call was redone with exact histograms */
+    /* with opt->profile: device ms of hist1, scan1, scatter1, pass2 sizing,
+       scatter2, tasks, join, other */
+    float phase_ms[8];
 } tbj_result;
 
 /* Initialise on a HIP device (0-based).  Safe to call more than once. */

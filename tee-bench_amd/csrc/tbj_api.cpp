@@ -51,6 +51,33 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
     return TBJ_OK;
 }
 
+int PhaseTimer::begin() {
+    if (!on_) return TBJ_OK;
+    for (auto& e : c_.pev)
+        if (!e) TBJ_HIP(hipEventCreate(&e));
+    n_ = 0;
+    TBJ_HIP(hipEventRecord(c_.pev[n_++], st_));
+    return TBJ_OK;
+}
+
+int PhaseTimer::mark(int phase) {
+    if (!on_ || n_ >= 24) return TBJ_OK;
+    phase_[n_] = phase;
+    TBJ_HIP(hipEventRecord(c_.pev[n_++], st_));
+    return TBJ_OK;
+}
+
+int PhaseTimer::finish(tbj_result* res) {
+    if (!on_) return TBJ_OK;
+    TBJ_HIP(hipEventSynchronize(c_.pev[n_ - 1]));
+    for (int i = 1; i < n_; i++) {
+        float ms = 0;
+        TBJ_HIP(hipEventElapsedTime(&ms, c_.pev[i - 1], c_.pev[i]));
+        res->phase_ms[phase_[i]] += ms;
+    }
+    return TBJ_OK;
+}
+
 void Workspace::release_all() {
     for (auto& kv : bufs_)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
@@ -151,7 +178,17 @@ int tbj_join(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint
         case TBJ_RHT:
         case TBJ_RJ:
         case TBJ_RHO_ATOMIC:
+        case TBJ_RSM:
             return run_radix_join(algo, R, nR, S, nS, pairs, pairs_cap, o, res, st);
+        case TBJ_PSM:
+            // global sort order: range-partition by the learned key CDF
+            if (o.partitioner == TBJ_PART_AUTO) o.partitioner = TBJ_PART_LEARNED;
+            return run_radix_join(algo, R, nR, S, nS, pairs, pairs_cap, o, res, st);
+        case TBJ_PHT:
+        case TBJ_NPO_ST:
+            return run_pht(R, nR, S, nS, pairs, pairs_cap, o, res, st);
+        case TBJ_CHT:
+            return run_cht(R, nR, S, nS, pairs, pairs_cap, o, res, st);
         default:
             return TBJ_ERR_UNSUPPORTED;
     }

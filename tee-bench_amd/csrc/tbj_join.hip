@@ -52,42 +52,29 @@ __device__ __forceinline__ uint32_t tasks_of(uint2 r, uint2 s, uint32_t cap, uin
     return ((nr + cap - 1) / cap) * ((ns + chunk - 1) / chunk);
 }
 
-// Per-partition task counts -> exclusive offsets (one workgroup).
-// stats[0] = largest R partition, stats[1] = largest S partition.
-__global__ __launch_bounds__(kScanBlock) void k_count_tasks(const uint2* __restrict__ bounds,
-                                                            uint32_t nparts, uint32_t cap,
-                                                            uint32_t chunk,
-                                                            uint32_t* __restrict__ task_off,
-                                                            uint32_t* __restrict__ ntasks_dev,
-                                                            uint32_t* __restrict__ stats) {
-    __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t carry_s, rmax_s, smax_s;
-    if (threadIdx.x == 0) carry_s = rmax_s = smax_s = 0;
-    __syncthreads();
-    uint32_t rmax = 0, smax = 0;
-    for (uint32_t base = 0; base < nparts; base += kScanBlock) {
-        const uint32_t p = base + threadIdx.x;
-        uint32_t c = 0;
-        if (p < nparts) {
-            const uint2 r = bounds[p], s = bounds[nparts + p];
-            c = tasks_of(r, s, cap, chunk);
-            rmax = max(rmax, r.y - r.x);
-            smax = max(smax, s.y - s.x);
-        }
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kScanBlock>(c, scratch, &total);
-        if (p < nparts) task_off[p] = carry_s + ex;
-        __syncthreads();
-        if (threadIdx.x == 0) carry_s += total;
-        __syncthreads();
+// Per-partition task counts (task_cnt[nparts] is a zero sentinel so that the
+// exclusive scan leaves the total there).  stats[0] / stats[1] = largest R /
+// S partition.
+__global__ void k_task_counts(const uint2* __restrict__ bounds, uint32_t nparts, uint32_t cap,
+                              uint32_t chunk, uint32_t* __restrict__ task_cnt,
+                              uint32_t* __restrict__ stats) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t rn = 0, sn = 0;
+    if (p < nparts) {
+        const uint2 r = bounds[p], s = bounds[nparts + p];
+        task_cnt[p] = tasks_of(r, s, cap, chunk);
+        rn = r.y - r.x;
+        sn = s.y - s.x;
+    } else if (p == nparts) {
+        task_cnt[p] = 0;
     }
-    atomicMax(&rmax_s, rmax);
-    atomicMax(&smax_s, smax);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        *ntasks_dev = carry_s;
-        stats[0] = rmax_s;
-        stats[1] = smax_s;
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+        rn = max(rn, (uint32_t)__shfl_xor(rn, d, TBJ_WAVE));
+        sn = max(sn, (uint32_t)__shfl_xor(sn, d, TBJ_WAVE));
+    }
+    if (lane_id() == 0 && (rn | sn)) {
+        atomicMax(&stats[0], rn);
+        atomicMax(&stats[1], sn);
     }
 }
 
@@ -117,6 +104,48 @@ __global__ void k_fill_tasks(const uint2* __restrict__ bounds, uint32_t nparts, 
 // LDS build/probe join.  counters[0] = matches, [1] = checksum of
 // (R.payload + S.payload), [2] = output cursor.
 
+// Looks one S tuple up in the LDS table; EMIT=false counts matches and sums
+// the checksum, EMIT=true writes them from output slot o on.
+template <int MODE, bool EMIT>
+__device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hmask,
+                                          const uint32_t* rkey, const uint32_t* rpay,
+                                          const uint32_t* head, const uint16_t* nxt,
+                                          uint32_t& c, uint64_t& cks, uint2* __restrict__ pairs,
+                                          uint64_t cap, uint64_t& o) {
+    const uint32_t key = sv.x;
+    const uint32_t h = (key >> hshift) & hmask;
+    if (MODE == kModeChain) {
+        for (uint32_t e = head[h]; e; e = nxt[e - 1]) {
+            if (rkey[e - 1] == key) {
+                if (EMIT) {
+                    if (o < cap) pairs[o] = make_uint2(rpay[e - 1], sv.y);
+                    o++;
+                } else {
+                    c++;
+                    cks += (uint64_t)rpay[e - 1] + sv.y;
+                }
+            }
+        }
+    } else {
+        const uint32_t en = head[h + 1];
+        for (uint32_t e = head[h]; e < en; e++) {
+            if (rkey[e] == key) {
+                if (EMIT) {
+                    if (o < cap) pairs[o] = make_uint2(rpay[e], sv.y);
+                    o++;
+                } else {
+                    c++;
+                    cks += (uint64_t)rpay[e] + sv.y;
+                }
+            }
+        }
+    }
+}
+
+// One task per loop trip: every R and S tuple of the task is loaded into
+// registers first (one memory phase), the R range is built into LDS, the S
+// tuples are probed, and with MAT the block claims one output range for
+// the whole task.  S ranges are at most kJoinBlock * kJoinSItems tuples.
 template <int MODE, bool MAT>
 __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ Rb,
                                                      const uint2* __restrict__ Sb,
@@ -139,13 +168,26 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
 
     const uint32_t ntasks = *ntasks_dev;
     uint64_t matches = 0, cks = 0;
+    if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+    for (uint32_t i = threadIdx.x; i < NHEAD; i += kJoinBlock) head[i] = 0;
+    __syncthreads();
     for (;;) {
-        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
-        __syncthreads();
         const uint32_t t = task_s;
         if (t >= ntasks) break;
         const JoinTask tk = tasks[t];
         const uint32_t n = tk.r_len;
+        // ---- loads: R and S of the task in flight together
+        uint2 rv[RITEMS], sv[kJoinSItems];
+#pragma unroll
+        for (int k = 0; k < RITEMS; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < n) rv[k] = Rb[tk.r_off + j];
+        }
+#pragma unroll
+        for (int k = 0; k < kJoinSItems; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
+        }
         uint32_t nb;
         if (MODE == kModeChain) {
             nb = next_pow2_u32(n);
@@ -155,34 +197,24 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
         }
         const uint32_t hmask = nb - 1;
 
-        // ---- build
+        // ---- build (heads were cleared at the end of the previous task)
         if (MODE == kModeChain) {
-            for (uint32_t i = threadIdx.x; i < nb; i += kJoinBlock) head[i] = 0;
-            __syncthreads();
 #pragma unroll
             for (int k = 0; k < RITEMS; k++) {
                 const uint32_t j = k * kJoinBlock + threadIdx.x;
                 if (j < n) {
-                    const uint2 v = Rb[tk.r_off + j];
-                    rkey[j] = v.x;
-                    rpay[j] = v.y;
-                    const uint32_t h = (v.x >> hshift) & hmask;
-                    nxt[j] = (uint16_t)atomicExch(&head[h], j + 1);
+                    rkey[j] = rv[k].x;
+                    rpay[j] = rv[k].y;
+                    nxt[j] = (uint16_t)atomicExch(&head[(rv[k].x >> hshift) & hmask], j + 1);
                 }
             }
             __syncthreads();
         } else {
-            for (uint32_t i = threadIdx.x; i <= nb; i += kJoinBlock) head[i] = 0;
-            __syncthreads();
-            uint2 v[RITEMS];
             uint32_t rk[RITEMS];
 #pragma unroll
             for (int k = 0; k < RITEMS; k++) {
                 const uint32_t j = k * kJoinBlock + threadIdx.x;
-                if (j < n) {
-                    v[k] = Rb[tk.r_off + j];
-                    rk[k] = atomicAdd(&head[(v[k].x >> hshift) & hmask], 1u);
-                }
+                if (j < n) rk[k] = atomicAdd(&head[(rv[k].x >> hshift) & hmask], 1u);
             }
             __syncthreads();
             block_scan_lds<kJoinBlock>(head, nb + 1, scratch);
@@ -190,81 +222,48 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
             for (int k = 0; k < RITEMS; k++) {
                 const uint32_t j = k * kJoinBlock + threadIdx.x;
                 if (j < n) {
-                    const uint32_t pos = head[(v[k].x >> hshift) & hmask] + rk[k];
-                    rkey[pos] = v[k].x;
-                    rpay[pos] = v[k].y;
+                    const uint32_t pos = head[(rv[k].x >> hshift) & hmask] + rk[k];
+                    rkey[pos] = rv[k].x;
+                    rpay[pos] = rv[k].y;
                 }
             }
             __syncthreads();
         }
 
         // ---- probe
-        for (uint32_t base = 0; base < tk.s_len; base += kJoinBlock * kJoinProbeItems) {
-            uint2 sv[kJoinProbeItems];
-            uint32_t c = 0;
+        uint32_t c = 0;
+        uint64_t o = 0;
 #pragma unroll
-            for (int k = 0; k < kJoinProbeItems; k++) {
-                const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                sv[k] = j < tk.s_len ? Sb[tk.s_off + j] : make_uint2(0, 0);
-            }
+        for (int k = 0; k < kJoinSItems; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < tk.s_len)
+                lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c, cks, pairs,
+                                       cap, o);
+        }
+        matches += c;
+        if (MAT) {
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<kJoinBlock>(c, scratch, &total);
+            if (threadIdx.x == 0)
+                obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
+            __syncthreads();
+            o = obase_s + ex;
+            if (c) {
+                uint32_t c2 = 0;
+                uint64_t k2 = 0;
 #pragma unroll
-            for (int k = 0; k < kJoinProbeItems; k++) {
-                const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                if (j >= tk.s_len) continue;
-                const uint32_t key = sv[k].x;
-                const uint32_t h = (key >> hshift) & hmask;
-                if (MODE == kModeChain) {
-                    for (uint32_t e = head[h]; e; e = nxt[e - 1]) {
-                        if (rkey[e - 1] == key) {
-                            c++;
-                            cks += (uint64_t)rpay[e - 1] + sv[k].y;
-                        }
-                    }
-                } else {
-                    const uint32_t en = head[h + 1];
-                    for (uint32_t e = head[h]; e < en; e++) {
-                        if (rkey[e] == key) {
-                            c++;
-                            cks += (uint64_t)rpay[e] + sv[k].y;
-                        }
-                    }
+                for (int k = 0; k < kJoinSItems; k++) {
+                    const uint32_t j = k * kJoinBlock + threadIdx.x;
+                    if (j < tk.s_len)
+                        lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
+                                              pairs, cap, o);
                 }
-            }
-            matches += c;
-            if (MAT) {
-                uint32_t total;
-                const uint32_t ex = block_exclusive_scan<kJoinBlock>(c, scratch, &total);
-                if (threadIdx.x == 0) obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
-                __syncthreads();
-                uint64_t o = obase_s + ex;
-                if (c) {
-#pragma unroll
-                    for (int k = 0; k < kJoinProbeItems; k++) {
-                        const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                        if (j >= tk.s_len) continue;
-                        const uint32_t key = sv[k].x;
-                        const uint32_t h = (key >> hshift) & hmask;
-                        if (MODE == kModeChain) {
-                            for (uint32_t e = head[h]; e; e = nxt[e - 1]) {
-                                if (rkey[e - 1] == key) {
-                                    if (o < cap) pairs[o] = make_uint2(rpay[e - 1], sv[k].y);
-                                    o++;
-                                }
-                            }
-                        } else {
-                            const uint32_t en = head[h + 1];
-                            for (uint32_t e = head[h]; e < en; e++) {
-                                if (rkey[e] == key) {
-                                    if (o < cap) pairs[o] = make_uint2(rpay[e], sv[k].y);
-                                    o++;
-                                }
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
             }
         }
+        __syncthreads();
+        // clear the heads for the next task and fetch its index
+        for (uint32_t i = threadIdx.x; i <= nb && i < (uint32_t)NHEAD; i += kJoinBlock) head[i] = 0;
+        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
         __syncthreads();
     }
     // block reduction of the counters
@@ -287,6 +286,129 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
 }
 
 // ---------------------------------------------------------------------------
+// Sort-merge task (RSM: radix_sortmerge_join.c:74-100; PSM's merge:
+// parallel_sortmerge_join.h:18-51).  The R range is sorted in LDS by
+// (key, payload) with a bitonic network; each S tuple then finds its
+// equal-key run with a lower-bound search and walks it.  The merge is
+// bounded on both sides (the reference's unbounded inner loops overrun the
+// arrays and miscount, parallel_sortmerge_join.h:26-46).
+
+__global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restrict__ Rb,
+                                                          const uint2* __restrict__ Sb,
+                                                          const JoinTask* __restrict__ tasks,
+                                                          const uint32_t* __restrict__ ntasks_dev,
+                                                          uint32_t* __restrict__ task_ctr,
+                                                          uint2* __restrict__ pairs, uint64_t cap,
+                                                          bool mat,
+                                                          unsigned long long* __restrict__ counters) {
+    __shared__ unsigned long long srt[kJoinCap];
+    __shared__ uint32_t scratch[kJoinBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t task_s;
+    __shared__ unsigned long long obase_s;
+    __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
+    const uint32_t ntasks = *ntasks_dev;
+    uint64_t matches = 0, cks = 0;
+    for (;;) {
+        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+        __syncthreads();
+        const uint32_t t = task_s;
+        if (t >= ntasks) break;
+        const JoinTask tk = tasks[t];
+        const uint32_t n = tk.r_len;
+        const uint32_t np2 = next_pow2_u32(n);
+        for (uint32_t j = threadIdx.x; j < np2; j += kJoinBlock) {
+            if (j < n) {
+                const uint2 v = Rb[tk.r_off + j];
+                srt[j] = ((unsigned long long)v.x << 32) | v.y;
+            } else {
+                srt[j] = ~0ull;
+            }
+        }
+        __syncthreads();
+        for (uint32_t k = 2; k <= np2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = threadIdx.x; i < np2; i += kJoinBlock) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & k) == 0;
+                        const unsigned long long a = srt[i], b = srt[ixj];
+                        if ((a > b) == up) {
+                            srt[i] = b;
+                            srt[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (uint32_t base = 0; base < tk.s_len; base += kJoinBlock * kJoinProbeItems) {
+            uint2 sv[kJoinProbeItems];
+            uint32_t lo[kJoinProbeItems];
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < kJoinProbeItems; k++) {
+                const uint32_t j = base + k * kJoinBlock + threadIdx.x;
+                sv[k] = j < tk.s_len ? Sb[tk.s_off + j] : make_uint2(0, 0);
+            }
+#pragma unroll
+            for (int k = 0; k < kJoinProbeItems; k++) {
+                const uint32_t j = base + k * kJoinBlock + threadIdx.x;
+                lo[k] = n;
+                if (j >= tk.s_len) continue;
+                const unsigned long long probe = (unsigned long long)sv[k].x << 32;
+                uint32_t L = 0, H = n;  // first sorted entry with key >= S key
+                while (L < H) {
+                    const uint32_t m = (L + H) >> 1;
+                    if (srt[m] < probe) L = m + 1;
+                    else H = m;
+                }
+                lo[k] = L;
+                for (uint32_t e = L; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
+                    c++;
+                    cks += (uint64_t)(uint32_t)srt[e] + sv[k].y;
+                }
+            }
+            matches += c;
+            if (mat) {
+                uint32_t total;
+                const uint32_t ex = block_exclusive_scan<kJoinBlock>(c, scratch, &total);
+                if (threadIdx.x == 0)
+                    obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
+                __syncthreads();
+                uint64_t o = obase_s + ex;
+                if (c) {
+#pragma unroll
+                    for (int k = 0; k < kJoinProbeItems; k++) {
+                        for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
+                            if (o < cap) pairs[o] = make_uint2((uint32_t)srt[e], sv[k].y);
+                            o++;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    matches = wave_reduce_u64(matches);
+    cks = wave_reduce_u64(cks);
+    if (lane_id() == 0) {
+        red_m[wave_id()] = matches;
+        red_c[wave_id()] = cks;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0, c2 = 0;
+        for (int w = 0; w < kJoinBlock / TBJ_WAVE; w++) {
+            m += red_m[w];
+            c2 += red_c[w];
+        }
+        if (m) atomicAdd(&counters[0], m);
+        if (c2) atomicAdd(&counters[1], c2);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 
 void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs,
@@ -300,9 +422,9 @@ void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* 
 void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
                         uint32_t* ntasks_dev, uint32_t* stats) {
-    (void)partial;
-    hipLaunchKernelGGL(k_count_tasks, dim3(1), dim3(kScanBlock), 0, st, bounds, nparts, cap,
-                       s_chunk, task_off, ntasks_dev, stats);
+    hipLaunchKernelGGL(k_task_counts, dim3((nparts + 1 + 255) / 256), dim3(256), 0, st, bounds,
+                       nparts, cap, s_chunk, task_off, stats);
+    launch_scan(st, task_off, (uint64_t)nparts + 1, partial, ntasks_dev);
 }
 
 void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
@@ -324,6 +446,11 @@ void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, cons
                  const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
                  uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
                  unsigned long long* counters) {
+    if (mode == kModeSort) {
+        hipLaunchKernelGGL(k_join_sort, dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
+                           ntasks_dev, task_ctr, pairs, cap, materialize, counters);
+        return;
+    }
     if (mode == kModeChain) {
         if (materialize)
             launch_join_t<kModeChain, true>(st, grid, Rb, Sb, tasks, ntasks_dev, task_ctr, hshift,
@@ -344,7 +471,9 @@ void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, cons
 int join_occupancy(int mode, bool materialize) {
     int blocks = 0;
     hipError_t e;
-    if (mode == kModeChain)
+    if (mode == kModeSort)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_join_sort, kJoinBlock, 0);
+    else if (mode == kModeChain)
         e = materialize ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
                               &blocks, k_join<kModeChain, true>, kJoinBlock, 0)
                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(

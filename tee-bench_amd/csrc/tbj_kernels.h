@@ -26,14 +26,18 @@ constexpr int kFitMax = 16384;
 constexpr int kJoinBlock = 512;
 constexpr int kJoinCap = 4096;   // max R tuples of one build (LDS hash table)
 constexpr int kJoinProbeItems = 4;
+constexpr int kJoinSItems = 16;  // S tuples per thread of one task
+constexpr int kJoinSChunk = kJoinBlock * kJoinSItems;  // 8192
 
-enum JoinMode : int { kModeChain = 0, kModeHist = 1 };
+enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
 size_t lds_model_bytes();
 
+// fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
 void launch_hist(hipStream_t st, uint32_t grid, const uint2* in, const TileDesc* tiles,
                  const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* minmax);
+                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
+                 const BinSpec* bs2 = nullptr, uint32_t nbins2 = 1);
 void launch_scatter(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
                     const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
                     uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
@@ -47,9 +51,19 @@ void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDes
                           TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2);
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
 void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nseg,
-                uint32_t bits, CdfModel* model);
-void launch_set_bounds(hipStream_t st, CdfModel* model, const uint32_t* minmax, uint32_t drop);
+                uint32_t bits, const uint32_t* minmax, CdfModel* model);
+void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// Sample-sized second pass: caps (nseg2 * nbins2 + 1 words), then an
+// exclusive scan of caps gives region starts; cursors start at zero.
+void launch_region_caps(hipStream_t st, const uint32_t* fine, const SegDesc* segs2,
+                        uint32_t nseg2, uint32_t nbins2, uint32_t* caps);
+void launch_scatter_est(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
+                        const TileDesc* tiles, const uint32_t* ntiles_dev, const uint32_t* starts,
+                        uint32_t* cursors, uint32_t* overflow, const BinSpec& bs,
+                        uint32_t nbins);
+void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                       uint32_t n, uint2* bounds);
 void launch_make_tiles1(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                         uint32_t nbins, uint32_t* ntiles_dev);
 // Zeroes hist[mult * (*count_dev) .. n); max_tail bounds the zeroed length.
@@ -60,6 +74,9 @@ void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t
 void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs,
                         uint32_t nbins_outer, uint32_t nbins_inner, uint32_t passes,
                         uint32_t nR, uint32_t nS, uint2* bounds);
+// Task offsets (task_off needs nparts + 1 words, partial scan_partials(nparts + 1)),
+// their total in *ntasks_dev, partition maxima folded into stats[0..1]
+// (stats must start at zero).
 void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
                         uint32_t* ntasks_dev, uint32_t* stats);
@@ -70,5 +87,31 @@ void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, cons
                  uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
                  unsigned long long* counters);
 int join_occupancy(int mode, bool materialize);
+
+// No-partitioning joins (tbj_nopart.hip)
+void launch_fill_u4(hipStream_t st, void* p, uint64_t bytes, uint32_t v);
+size_t pht_bucket_bytes();
+size_t pht_node_bytes();
+void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, uint32_t mask,
+                      void* pool, uint32_t* pool_ctr);
+void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,
+                      uint32_t mask, const void* pool, uint2* pairs, uint64_t cap,
+                      unsigned long long* counters);
+// CHT build: bits (nwords), cnt (nwords + 1), taken ((|R| + 2 + 31) / 32 words),
+// arr (|R| + 2 tuples), spill (|R|), spilled ((|R| + 31) / 32) must be zeroed
+// where noted by the caller (bits, taken, spilled, spill_ctr).
+void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmask,
+                      uint32_t pmask, uint32_t* bits, uint32_t* cnt, uint32_t* partial,
+                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spill_ctr,
+                      uint32_t* spilled);
+void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,
+                         const uint32_t* spill_ctr, uint32_t nspill, uint32_t ov_mask,
+                         uint32_t* ov_used, uint2* ov);
+void launch_cht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, uint32_t bmask,
+                      uint32_t pmask, uint32_t ov_mask, const uint32_t* bits,
+                      const uint32_t* cnt, const uint32_t* taken, const uint2* arr,
+                      uint32_t nslots, const uint32_t* ov_used, const uint2* ov,
+                      const uint32_t* spill_ctr, uint2* pairs, uint64_t cap,
+                      unsigned long long* counters);
 
 }  // namespace tbj

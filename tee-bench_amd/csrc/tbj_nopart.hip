@@ -1,0 +1,433 @@
+// No-partitioning joins: PHT / NPO_st (shared hash table) and CHT (concise
+// hash table), each built in HBM with global atomics and probed by streaming S.
+//
+// PHT restates no_partitioning_join.h:119-356,219-259: nextpow2(|R|/2)
+// buckets, identity hash h = key & (buckets-1), two tuples per bucket, more
+// in overflow chains.  A bucket here is one 32-byte record {count, overflow
+// head, 2 tuples}; the reference's per-bucket spin latch becomes an
+// atomicAdd on count, and the per-thread overflow arenas become one pool
+// claimed with atomicAdd.
+//
+// CHT restates CHT.hpp:26-254 / CHTJoin.hpp:46-322: a bitmap of
+// nextpow2(|R|) bits in 32-bit words with a running popcount per word, a
+// dense tuple array indexed by the popcount of the home bit, one linear-probe
+// step inside a partition of the bitmap, and a two-slot probe.  Where the
+// reference silently drops a tuple (a third colliding key, CHT.hpp:155-192)
+// or aborts (a miss in the probe tail, CHT.hpp:194-206), this version keeps
+// the tuple in an overflow hash table, so the result is the exact equi-join.
+#include "tbj_device.h"
+#include "tbj_kernels.h"
+
+namespace tbj {
+
+struct __align__(32) PhtBucket {
+    uint32_t count;
+    uint32_t ovf;  // 1-based index into the overflow pool, 0 = none
+    uint2 slot[2];
+    uint32_t pad[2];
+};
+
+struct __align__(16) OvNode {
+    uint2 t;
+    uint32_t next;  // 1-based
+    uint32_t pad;
+};
+
+__global__ void k_fill_u4(uint4* __restrict__ p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = make_uint4(v, v, v, v);
+}
+
+__global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, uint64_t nR,
+                                                   PhtBucket* __restrict__ tab, uint32_t mask,
+                                                   OvNode* __restrict__ pool,
+                                                   uint32_t* __restrict__ pool_ctr) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint2 t = R[i];
+        PhtBucket* b = tab + (t.x & mask);
+        const uint32_t pos = atomicAdd(&b->count, 1u);
+        if (pos < 2) {
+            b->slot[pos] = t;
+        } else {
+            const uint32_t n = atomicAdd(pool_ctr, 1u);
+            pool[n].t = t;
+            pool[n].next = atomicExch(&b->ovf, n + 1);
+        }
+    }
+}
+
+// Block-aggregated output: every round each thread handles kProbeItems S
+// tuples; matches are counted, the block claims one output range, and the
+// matches are walked again to write them.
+constexpr int kProbeBlock = 256;
+constexpr int kProbeItems = 4;
+
+template <bool MAT>
+__global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
+    const uint2* __restrict__ S, uint64_t nS, const PhtBucket* __restrict__ tab, uint32_t mask,
+    const OvNode* __restrict__ pool, uint2* __restrict__ pairs, uint64_t cap,
+    unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t scratch[kProbeBlock / TBJ_WAVE + 1];
+    __shared__ unsigned long long obase_s;
+    __shared__ unsigned long long red_m[kProbeBlock / TBJ_WAVE], red_c[kProbeBlock / TBJ_WAVE];
+    uint64_t matches = 0, cks = 0;
+    const uint64_t per_round = (uint64_t)kProbeBlock * kProbeItems;
+    for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < nS;
+         base += (uint64_t)gridDim.x * per_round) {
+        uint2 sv[kProbeItems];
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+            sv[k] = j < nS ? S[j] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+            if (j >= nS) continue;
+            const PhtBucket b = tab[sv[k].x & mask];
+            const uint32_t n = min(b.count, 2u);
+            for (uint32_t q = 0; q < n; q++)
+                if (b.slot[q].x == sv[k].x) {
+                    c++;
+                    cks += (uint64_t)b.slot[q].y + sv[k].y;
+                }
+            for (uint32_t e = b.ovf; e; e = pool[e - 1].next)
+                if (pool[e - 1].t.x == sv[k].x) {
+                    c++;
+                    cks += (uint64_t)pool[e - 1].t.y + sv[k].y;
+                }
+        }
+        matches += c;
+        if (MAT) {
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<kProbeBlock>(c, scratch, &total);
+            if (threadIdx.x == 0) obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
+            __syncthreads();
+            uint64_t o = obase_s + ex;
+            if (c) {
+#pragma unroll
+                for (int k = 0; k < kProbeItems; k++) {
+                    const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+                    if (j >= nS) continue;
+                    const PhtBucket b = tab[sv[k].x & mask];
+                    const uint32_t n = min(b.count, 2u);
+                    for (uint32_t q = 0; q < n; q++)
+                        if (b.slot[q].x == sv[k].x) {
+                            if (o < cap) pairs[o] = make_uint2(b.slot[q].y, sv[k].y);
+                            o++;
+                        }
+                    for (uint32_t e = b.ovf; e; e = pool[e - 1].next)
+                        if (pool[e - 1].t.x == sv[k].x) {
+                            if (o < cap) pairs[o] = make_uint2(pool[e - 1].t.y, sv[k].y);
+                            o++;
+                        }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    matches = wave_reduce_u64(matches);
+    cks = wave_reduce_u64(cks);
+    if (lane_id() == 0) {
+        red_m[wave_id()] = matches;
+        red_c[wave_id()] = cks;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0, cc = 0;
+        for (int w = 0; w < kProbeBlock / TBJ_WAVE; w++) {
+            m += red_m[w];
+            cc += red_c[w];
+        }
+        if (m) atomicAdd(&counters[0], m);
+        if (cc) atomicAdd(&counters[1], cc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CHT
+
+struct ChtParams {
+    uint32_t bmask;   // bitmap size - 1 (bitmap size = nextpow2(|R|), >= 64)
+    uint32_t pmask;   // partition size - 1 (partition = bitmap / 128 bits, >= 32)
+    uint32_t ov_mask; // overflow hash table slots - 1
+};
+
+__device__ __forceinline__ uint32_t cht_next_in_part(uint32_t pos, uint32_t pmask) {
+    return (pos & ~pmask) | ((pos + 1) & pmask);
+}
+
+// Phase A: set the home bit, or the next bit inside the partition
+// (CHT::setBit).  Tuples that find both taken go to the overflow list.
+__global__ __launch_bounds__(256) void k_cht_bits(const uint2* __restrict__ R, uint64_t nR,
+                                                  ChtParams p, uint32_t* __restrict__ bits,
+                                                  uint32_t* __restrict__ spill,
+                                                  uint32_t* __restrict__ spill_ctr) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t h = R[i].x & p.bmask;
+        const uint32_t m = 1u << (h & 31);
+        if (!(atomicOr(&bits[h >> 5], m) & m)) continue;
+        const uint32_t h2 = cht_next_in_part(h, p.pmask);
+        const uint32_t m2 = 1u << (h2 & 31);
+        if (!(atomicOr(&bits[h2 >> 5], m2) & m2)) continue;
+        spill[atomicAdd(spill_ctr, 1u)] = (uint32_t)i;
+    }
+}
+
+__global__ void k_cht_popc(const uint32_t* __restrict__ bits, uint32_t nwords,
+                           uint32_t* __restrict__ cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) cnt[i] = __popc(bits[i]);
+    else if (i == nwords) cnt[i] = 0;
+}
+
+__device__ __forceinline__ uint32_t cht_pos(const uint32_t* bits, const uint32_t* cnt,
+                                            uint32_t h) {
+    const uint32_t w = h >> 5;
+    return cnt[w] + __popc(bits[w] & ((1u << (h & 31)) - 1u));
+}
+
+// Phase C: place each tuple at the dense slot of its home bit or the next one
+// (CHT::setTuple); a slot is claimed through a taken-bit.  Tuples already in
+// the overflow list from phase A are skipped (marked in `spilled`).
+__global__ __launch_bounds__(256) void k_cht_place(const uint2* __restrict__ R, uint64_t nR,
+                                                   ChtParams p, const uint32_t* __restrict__ bits,
+                                                   const uint32_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ spilled,
+                                                   uint32_t* __restrict__ taken,
+                                                   uint2* __restrict__ arr,
+                                                   uint32_t* __restrict__ spill,
+                                                   uint32_t* __restrict__ spill_ctr) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (spilled[i >> 5] & (1u << (i & 31))) continue;
+        const uint2 t = R[i];
+        const uint32_t pos = cht_pos(bits, cnt, t.x & p.bmask);
+        uint32_t m = 1u << (pos & 31);
+        if (!(atomicOr(&taken[pos >> 5], m) & m)) {
+            arr[pos] = t;
+            continue;
+        }
+        const uint32_t pos2 = pos + 1;
+        m = 1u << (pos2 & 31);
+        if (!(atomicOr(&taken[pos2 >> 5], m) & m)) {
+            arr[pos2] = t;
+            continue;
+        }
+        spill[atomicAdd(spill_ctr, 1u)] = (uint32_t)i;
+    }
+}
+
+__global__ void k_cht_mark_spilled(const uint32_t* __restrict__ spill,
+                                   const uint32_t* __restrict__ spill_ctr,
+                                   uint32_t* __restrict__ spilled) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *spill_ctr) atomicOr(&spilled[spill[i] >> 5], 1u << (spill[i] & 31));
+}
+
+// Overflow table: open addressing, linear probing, (key, payload) with an
+// occupancy word per slot.
+__global__ void k_cht_overflow(const uint2* __restrict__ R, const uint32_t* __restrict__ spill,
+                               const uint32_t* __restrict__ spill_ctr, uint32_t ov_mask,
+                               uint32_t* __restrict__ ov_used, uint2* __restrict__ ov) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= *spill_ctr) return;
+    const uint2 t = R[spill[i]];
+    uint32_t h = (t.x * 2654435761u) & ov_mask;
+    while (atomicCAS(&ov_used[h], 0u, 1u) != 0u) h = (h + 1) & ov_mask;
+    ov[h] = t;
+}
+
+// Looks one S tuple up; EMIT=false counts (and sums the checksum), EMIT=true
+// writes the matches from output slot o on.
+template <bool EMIT>
+__device__ __forceinline__ void cht_lookup(uint2 s, const ChtParams& p,
+                                           const uint32_t* __restrict__ bits,
+                                           const uint32_t* __restrict__ cnt,
+                                           const uint32_t* __restrict__ taken,
+                                           const uint2* __restrict__ arr, uint32_t nslots,
+                                           bool any_ov, const uint32_t* __restrict__ ov_used,
+                                           const uint2* __restrict__ ov, uint32_t& c,
+                                           uint64_t& cks, uint2* __restrict__ pairs,
+                                           uint64_t cap, uint64_t& o) {
+    const uint32_t key = s.x;
+    const uint32_t pos = cht_pos(bits, cnt, key & p.bmask);
+#pragma unroll
+    for (uint32_t q = 0; q < 2; q++) {
+        const uint32_t ps = pos + q;
+        if (ps < nslots && (taken[ps >> 5] & (1u << (ps & 31))) && arr[ps].x == key) {
+            if (EMIT) {
+                if (o < cap) pairs[o] = make_uint2(arr[ps].y, s.y);
+                o++;
+            } else {
+                c++;
+                cks += (uint64_t)arr[ps].y + s.y;
+            }
+        }
+    }
+    if (any_ov) {
+        for (uint32_t e = (key * 2654435761u) & p.ov_mask; ov_used[e]; e = (e + 1) & p.ov_mask) {
+            if (ov[e].x == key) {
+                if (EMIT) {
+                    if (o < cap) pairs[o] = make_uint2(ov[e].y, s.y);
+                    o++;
+                } else {
+                    c++;
+                    cks += (uint64_t)ov[e].y + s.y;
+                }
+            }
+        }
+    }
+}
+
+template <bool MAT>
+__global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
+    const uint2* __restrict__ S, uint64_t nS, ChtParams p, const uint32_t* __restrict__ bits,
+    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ taken,
+    const uint2* __restrict__ arr, uint32_t nslots, const uint32_t* __restrict__ ov_used,
+    const uint2* __restrict__ ov, const uint32_t* __restrict__ spill_ctr, uint2* __restrict__ pairs,
+    uint64_t cap, unsigned long long* __restrict__ counters) {
+    __shared__ uint32_t scratch[kProbeBlock / TBJ_WAVE + 1];
+    __shared__ unsigned long long obase_s;
+    __shared__ unsigned long long red_m[kProbeBlock / TBJ_WAVE], red_c[kProbeBlock / TBJ_WAVE];
+    const bool any_ov = *spill_ctr != 0;
+    uint64_t matches = 0, cks = 0;
+    const uint64_t per_round = (uint64_t)kProbeBlock * kProbeItems;
+    for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < nS;
+         base += (uint64_t)gridDim.x * per_round) {
+        uint2 sv[kProbeItems];
+        uint32_t c = 0;
+        uint64_t o = 0;
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+            sv[k] = j < nS ? S[j] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+            if (j < nS)
+                cht_lookup<false>(sv[k], p, bits, cnt, taken, arr, nslots, any_ov, ov_used, ov, c,
+                                  cks, pairs, cap, o);
+        }
+        matches += c;
+        if (MAT) {
+            uint32_t total;
+            const uint32_t ex = block_exclusive_scan<kProbeBlock>(c, scratch, &total);
+            if (threadIdx.x == 0)
+                obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
+            __syncthreads();
+            o = obase_s + ex;
+            if (c) {
+                uint64_t dummy = 0;
+#pragma unroll
+                for (int k = 0; k < kProbeItems; k++) {
+                    const uint64_t j = base + k * kProbeBlock + threadIdx.x;
+                    if (j < nS)
+                        cht_lookup<true>(sv[k], p, bits, cnt, taken, arr, nslots, any_ov, ov_used,
+                                         ov, c, dummy, pairs, cap, o);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    matches = wave_reduce_u64(matches);
+    cks = wave_reduce_u64(cks);
+    if (lane_id() == 0) {
+        red_m[wave_id()] = matches;
+        red_c[wave_id()] = cks;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0, cc = 0;
+        for (int w = 0; w < kProbeBlock / TBJ_WAVE; w++) {
+            m += red_m[w];
+            cc += red_c[w];
+        }
+        if (m) atomicAdd(&counters[0], m);
+        if (cc) atomicAdd(&counters[1], cc);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+static uint32_t grid_for(uint64_t n, uint32_t block, uint32_t cap_blocks) {
+    uint64_t g = (n + block - 1) / block;
+    if (g > cap_blocks) g = cap_blocks;
+    return g ? (uint32_t)g : 1;
+}
+
+void launch_fill_u4(hipStream_t st, void* p, uint64_t bytes, uint32_t v) {
+    const uint64_t n = bytes / 16;
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill_u4, dim3(grid_for(n, 256, 8192)), dim3(256), 0, st, (uint4*)p, n, v);
+}
+
+size_t pht_bucket_bytes() { return sizeof(PhtBucket); }
+size_t pht_node_bytes() { return sizeof(OvNode); }
+
+void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, uint32_t mask,
+                      void* pool, uint32_t* pool_ctr) {
+    hipLaunchKernelGGL(k_pht_build, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR,
+                       (PhtBucket*)tab, mask, (OvNode*)pool, pool_ctr);
+}
+
+void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,
+                      uint32_t mask, const void* pool, uint2* pairs, uint64_t cap,
+                      unsigned long long* counters) {
+    const uint32_t g = grid_for(nS, kProbeBlock * kProbeItems, 16384);
+    if (mat)
+        hipLaunchKernelGGL(k_pht_probe<true>, dim3(g), dim3(kProbeBlock), 0, st, S, nS,
+                           (const PhtBucket*)tab, mask, (const OvNode*)pool, pairs, cap, counters);
+    else
+        hipLaunchKernelGGL(k_pht_probe<false>, dim3(g), dim3(kProbeBlock), 0, st, S, nS,
+                           (const PhtBucket*)tab, mask, (const OvNode*)pool, pairs, cap, counters);
+}
+
+void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmask,
+                      uint32_t pmask, uint32_t* bits, uint32_t* cnt, uint32_t* partial,
+                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spill_ctr,
+                      uint32_t* spilled) {
+    ChtParams p{bmask, pmask, 0};
+    const uint32_t nwords = (bmask + 1) / 32;
+    hipLaunchKernelGGL(k_cht_bits, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR, p,
+                       bits, spill, spill_ctr);
+    hipLaunchKernelGGL(k_cht_popc, dim3((nwords + 1 + 255) / 256), dim3(256), 0, st, bits, nwords,
+                       cnt);
+    launch_scan(st, cnt, (uint64_t)nwords + 1, partial, nullptr);
+    hipLaunchKernelGGL(k_cht_mark_spilled, dim3(grid_for(nR, 256, 1u << 30)), dim3(256), 0, st,
+                       spill, spill_ctr, spilled);
+    hipLaunchKernelGGL(k_cht_place, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR, p,
+                       bits, cnt, spilled, taken, arr, spill, spill_ctr);
+}
+
+void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,
+                         const uint32_t* spill_ctr, uint32_t nspill, uint32_t ov_mask,
+                         uint32_t* ov_used, uint2* ov) {
+    if (!nspill) return;
+    hipLaunchKernelGGL(k_cht_overflow, dim3((nspill + 255) / 256), dim3(256), 0, st, R, spill,
+                       spill_ctr, ov_mask, ov_used, ov);
+}
+
+void launch_cht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, uint32_t bmask,
+                      uint32_t pmask, uint32_t ov_mask, const uint32_t* bits,
+                      const uint32_t* cnt, const uint32_t* taken, const uint2* arr,
+                      uint32_t nslots, const uint32_t* ov_used, const uint2* ov,
+                      const uint32_t* spill_ctr, uint2* pairs, uint64_t cap,
+                      unsigned long long* counters) {
+    ChtParams p{bmask, pmask, ov_mask};
+    const uint32_t g = grid_for(nS, kProbeBlock * kProbeItems, 16384);
+    if (mat)
+        hipLaunchKernelGGL(k_cht_probe<true>, dim3(g), dim3(kProbeBlock), 0, st, S, nS, p, bits,
+                           cnt, taken, arr, nslots, ov_used, ov, spill_ctr, pairs, cap, counters);
+    else
+        hipLaunchKernelGGL(k_cht_probe<false>, dim3(g), dim3(kProbeBlock), 0, st, S, nS, p, bits,
+                           cnt, taken, arr, nslots, ov_used, ov, spill_ctr, pairs, cap, counters);
+}
+
+}  // namespace tbj

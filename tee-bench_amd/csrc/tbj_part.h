@@ -33,12 +33,19 @@ struct JoinTask {
 // monotone).  kmin/kmax are R's exact key bounds: keys outside are dropped
 // from S because they cannot match.
 constexpr int kMaxKnots = 4096;
+constexpr int kGuideCells = 4096;  // equal-width cells over [kmin, kmax]
 struct CdfModel {
     uint32_t nseg;
     uint32_t frac;        // fixed-point fraction bits of y
     uint32_t kmin, kmax;  // exact R key range (inclusive)
     uint32_t drop_outside;
-    uint32_t pad[3];
+    uint32_t gshift;      // cell of a key = (key - kmin) >> gshift
+    uint32_t ncells;
+    uint32_t pad;
+    // guide[c] = segment holding the first key of cell c; guide[ncells] =
+    // nseg - 1.  A key of cell c lies in a segment of [guide[c], guide[c+1]].
+    uint16_t guide[kGuideCells + 1];
+    uint16_t pad2[7];
     uint64_t x[kMaxKnots + 1];      // knot keys (x[nseg] may be 2^32)
     uint32_t y[kMaxKnots + 1];      // cumulative partition position
     uint64_t slope[kMaxKnots];      // (y[i+1]-y[i]) * 2^32 / (x[i+1]-x[i])

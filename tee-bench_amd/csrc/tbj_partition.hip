@@ -17,36 +17,40 @@ namespace tbj {
 // ---------------------------------------------------------------------------
 // Partition function
 
-__device__ __forceinline__ uint32_t model_eval(const uint32_t* __restrict__ xs,
-                                               const uint32_t* __restrict__ ys,
-                                               const uint64_t* __restrict__ sl, uint32_t nseg,
-                                               uint32_t frac, uint32_t key) {
-    // largest i in [0, nseg) with xs[i] <= key (xs[0] <= key is ensured by clamping)
-    uint32_t lo = 0, hi = nseg;  // invariant: xs[lo] <= key, answer < hi
-    if (key < xs[0]) return 0;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (xs[mid] <= key) lo = mid;
-        else hi = mid;
-    }
-    uint64_t d = (uint64_t)(key - xs[lo]);
-    uint64_t s = sl[lo];
-    // (d * s) >> 32 with a 128-bit product; the result stays below y[lo+1]-y[lo]
-    uint64_t lo64 = d * s;
-    uint64_t hi64 = __umul64hi(d, s);
-    uint64_t add = (hi64 << 32) | (lo64 >> 32);
-    uint64_t v = (uint64_t)ys[lo] + add;
-    uint64_t vmax = (uint64_t)ys[lo + 1] - (ys[lo + 1] > ys[lo] ? 1 : 0);
-    if (v > vmax) v = vmax;
-    return (uint32_t)(v >> frac);
-}
-
-// LDS copy of a CdfModel's knots (at most kLdsKnots segments).
+// LDS copy of a CdfModel's knots (at most kLdsKnots segments) and guide.
 struct LdsModel {
     uint32_t x[kLdsKnots + 1];
     uint32_t y[kLdsKnots + 1];
     uint64_t s[kLdsKnots];
+    uint16_t guide[kGuideCells + 1];
 };
+
+// Monotone model value of a key, in partitions: find the segment through the
+// guide cell (usually one or two steps), then interpolate in fixed point.
+__device__ __forceinline__ uint32_t model_eval(const LdsModel* m, uint32_t nseg, uint32_t frac,
+                                               uint32_t kmin, uint32_t gshift, uint32_t ncells,
+                                               uint32_t key) {
+    if (key < m->x[0]) return 0;
+    uint32_t c = (key - kmin) >> gshift;
+    if (c >= ncells) c = ncells - 1;
+    uint32_t lo = m->guide[c], hi = (uint32_t)m->guide[c + 1] + 1;  // answer in [lo, hi)
+    if (hi > nseg) hi = nseg;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (m->x[mid] <= key) lo = mid;
+        else hi = mid;
+    }
+    const uint64_t d = (uint64_t)(key - m->x[lo]);
+    const uint64_t sl = m->s[lo];
+    // (d * sl) >> 32 with a 128-bit product; the result stays below y[lo+1]-y[lo]
+    const uint64_t lo64 = d * sl;
+    const uint64_t hi64 = __umul64hi(d, sl);
+    const uint64_t add = (hi64 << 32) | (lo64 >> 32);
+    uint64_t v = (uint64_t)m->y[lo] + add;
+    const uint64_t vmax = (uint64_t)m->y[lo + 1] - (m->y[lo + 1] > m->y[lo] ? 1 : 0);
+    if (v > vmax) v = vmax;
+    return (uint32_t)(v >> frac);
+}
 
 __device__ __forceinline__ void load_model(LdsModel* lm, const CdfModel* m) {
     const uint32_t n = m->nseg;
@@ -55,12 +59,13 @@ __device__ __forceinline__ void load_model(LdsModel* lm, const CdfModel* m) {
         lm->y[i] = m->y[i];
         if (i < n) lm->s[i] = m->slope[i];
     }
+    for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->guide[i] = m->guide[i];
 }
 
 struct BinFn {
     BinSpec bs;
     const LdsModel* lm;
-    uint32_t nseg, frac, kmin, kmax;
+    uint32_t nseg, frac, kmin, kmax, gshift, ncells;
 
     __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
         uint32_t full;
@@ -68,7 +73,7 @@ struct BinFn {
             full = bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
         } else {
             if (bs.drop && (key < kmin || key > kmax)) return TBJ_DROP;
-            full = model_eval(lm->x, lm->y, lm->s, nseg, frac, key);
+            full = model_eval(lm, nseg, frac, kmin, gshift, ncells, key);
         }
         return (full >> bs.pass_shift) & bs.pass_mask;
     }
@@ -78,21 +83,48 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     BinFn f;
     f.bs = bs;
     f.lm = lm;
-    f.nseg = f.frac = f.kmin = f.kmax = 0;
+    f.nseg = f.frac = f.kmin = f.kmax = f.gshift = f.ncells = 0;
     if (bs.kind == kBinLearned) {
         load_model(lm, bs.model);
         f.nseg = bs.model->nseg;
         f.frac = bs.model->frac;
         f.kmin = bs.model->kmin;
         f.kmax = bs.model->kmax;
+        f.gshift = bs.model->gshift;
+        f.ncells = bs.model->ncells;
     }
     return f;
 }
 
 // ---------------------------------------------------------------------------
 // Histogram: one workgroup per tile, LDS-privatised counts, written
-// bin-major per segment.  With minmax != NULL the tile also folds its key range
-// into minmax[0..1] (used to pin the learned model's exact R bounds).
+// bin-major per segment.  With fine != NULL, a pseudo-random 1/16 of the
+// tuples (by their index in the relation) also count into
+// fine[seg * nbins * nbins2 + bin * nbins2 + bin2(key)]: the sample from
+// which the second pass sizes its sub-partitions.
+
+__device__ __forceinline__ bool sampled16(uint32_t idx) {
+    return ((idx * 2654435761u) >> 28) == 0;
+}
+
+// Slot of a tile in its segment's offset order.  Workgroups are dispatched
+// to the 8 XCDs round-robin by blockIdx.x, so tiles whose block indices are
+// congruent mod 8 share an XCD and its L2.  Ordering a segment's tiles by
+// (XCD, tile) makes each XCD's runs of one partition adjacent in HBM, so a
+// cache line of a partition is filled inside one L2 instead of being split
+// between two XCDs' partial write-backs.
+constexpr uint32_t kNumXcd = 8;
+__device__ __forceinline__ uint32_t xcd_slot(uint32_t tis, uint32_t ntiles, uint32_t block) {
+    const uint32_t base_b = block - tis;  // block index of the segment's first tile
+    const uint32_t g = block % kNumXcd;
+    uint32_t before = 0;
+    for (uint32_t h = 0; h < g; h++) {
+        const uint32_t r0 = (h + kNumXcd - base_b % kNumXcd) % kNumXcd;
+        before += ntiles > r0 ? (ntiles - r0 + kNumXcd - 1) / kNumXcd : 0;
+    }
+    const uint32_t r0g = (g + kNumXcd - base_b % kNumXcd) % kNumXcd;
+    return before + (tis - r0g) / kNumXcd;
+}
 
 __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ in,
                                                      const TileDesc* __restrict__ tiles,
@@ -100,54 +132,63 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
                                                      const uint32_t* __restrict__ ntiles_dev,
                                                      uint32_t tile_base, uint32_t* __restrict__ hist,
                                                      BinSpec bs, uint32_t nbins,
-                                                     uint32_t* __restrict__ minmax) {
+                                                     uint32_t* __restrict__ fine, BinSpec bs2,
+                                                     uint32_t nbins2) {
     const uint32_t t = blockIdx.x + tile_base;
     if (t >= *ntiles_dev) return;
     __shared__ uint32_t cnt[kMaxBins];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
-    __shared__ uint32_t wmin[kPartBlock / TBJ_WAVE], wmax[kPartBlock / TBJ_WAVE];
     for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
     BinFn bin = make_binfn(bs, &lm);
+    BinFn bin2 = bin;
+    bin2.bs = bs2;
     __syncthreads();
     const TileDesc td = tiles[t];
-    uint32_t kmn = 0xFFFFFFFFu, kmx = 0;
-#pragma unroll 4
+    const SegDesc sd = segs[td.seg];
+    uint32_t key[kPartItems];
+#pragma unroll
     for (uint32_t k = 0; k < kPartItems; k++) {
-        uint32_t j = k * kPartBlock + threadIdx.x;
-        if (j < td.len) {
-            uint32_t key = in[td.in_off + j].x;
-            kmn = min(kmn, key);
-            kmx = max(kmx, key);
-            uint32_t b = bin(key);
-            if (b != TBJ_DROP) atomicAdd(&cnt[b], 1u);
-        }
+        const uint32_t j = k * kPartBlock + threadIdx.x;
+        key[k] = j < td.len ? in[td.in_off + j].x : 0;
     }
-    if (minmax) {
-        for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
-            kmn = min(kmn, (uint32_t)__shfl_xor(kmn, d, TBJ_WAVE));
-            kmx = max(kmx, (uint32_t)__shfl_xor(kmx, d, TBJ_WAVE));
-        }
-        if (lane_id() == 0) {
-            wmin[wave_id()] = kmn;
-            wmax[wave_id()] = kmx;
+#pragma unroll
+    for (uint32_t k = 0; k < kPartItems; k++) {
+        const uint32_t j = k * kPartBlock + threadIdx.x;
+        if (j < td.len) {
+            const uint32_t b = bin(key[k]);
+            if (b != TBJ_DROP) {
+                atomicAdd(&cnt[b], 1u);
+                if (fine && sampled16(td.in_off + j))
+                    atomicAdd(&fine[((uint64_t)td.seg * nbins + b) * nbins2 + bin2(key[k])], 1u);
+            }
         }
     }
     __syncthreads();
-    if (minmax && threadIdx.x == 0) {
-        for (int w = 1; w < kPartBlock / TBJ_WAVE; w++) {
-            kmn = min(kmn, wmin[w]);
-            kmx = max(kmx, wmax[w]);
-        }
-        kmn = min(kmn, wmin[0]);
-        kmx = max(kmx, wmax[0]);
-        atomicMin(&minmax[0], kmn);
-        atomicMax(&minmax[1], kmx);
-    }
-    const SegDesc sd = segs[td.seg];
-    const uint32_t tis = t - sd.first_tile;
+    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, blockIdx.x);
     for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
         hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
+}
+
+// Exact key range of a relation: minmax[0] = min, minmax[1] = max (must start
+// at {UINT_MAX, 0}).
+__global__ __launch_bounds__(256) void k_minmax(const uint2* __restrict__ in, uint64_t n,
+                                                uint32_t* __restrict__ minmax) {
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t k = in[i].x;
+        mn = min(mn, k);
+        mx = max(mx, k);
+    }
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
+    }
+    if (lane_id() == 0) {
+        atomicMin(&minmax[0], mn);
+        atomicMax(&minmax[1], mx);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -189,24 +230,38 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restri
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(uint32_t* __restrict__ a, uint64_t n,
                                                           const uint32_t* __restrict__ partial) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t buf[kScanChunk + kScanChunk / 32];  // padded against bank conflicts
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk;
-    // each thread scans kScanItems consecutive elements
-    uint32_t v[kScanItems];
-    uint32_t s = 0;
-    const uint64_t tb = base + (uint64_t)threadIdx.x * kScanItems;
+    // coalesced load into LDS; each thread then scans kScanItems consecutive words
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; k++) {
-        uint64_t i = tb + k;
-        v[k] = i < n ? a[i] : 0;
+        const uint32_t j = k * kScanBlock + threadIdx.x;
+        const uint64_t i = base + j;
+        buf[j + (j >> 5)] = i < n ? a[i] : 0;
+    }
+    __syncthreads();
+    uint32_t v[kScanItems];
+    uint32_t s = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; k++) {
+        const uint32_t j = threadIdx.x * kScanItems + k;
+        v[k] = buf[j + (j >> 5)];
         s += v[k];
     }
     uint32_t total;
     uint32_t run = block_exclusive_scan<kScanBlock>(s, scratch, &total) + partial[blockIdx.x];
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; k++) {
-        uint64_t i = tb + k;
-        if (i < n) a[i] = run;
+        const uint32_t j = threadIdx.x * kScanItems + k;
+        buf[j + (j >> 5)] = run;
         run += v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; k++) {
+        const uint32_t j = k * kScanBlock + threadIdx.x;
+        const uint64_t i = base + j;
+        if (i < n) a[i] = buf[j + (j >> 5)];
     }
 }
 
@@ -256,7 +311,7 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
     __syncthreads();
     const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
     const SegDesc sd = segs[td.seg];
-    const uint32_t tis = t - sd.first_tile;
+    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, blockIdx.x);
     for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
         goff[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis] - loff[i];
 #pragma unroll
@@ -275,54 +330,194 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Second pass sized from the sample instead of per-tile histograms.  Every
+// sub-partition (segment s = rel * nbins1 + p1, bin p2) owns the region
+// [start[i], start[i+1]) of the output, i = s * nbins2 + p2; tiles claim
+// space in it with one atomicAdd per (tile, bin).  A region that turns out
+// too small sets *overflow; the caller then redoes the pass with exact
+// histograms.
+
+// caps[i] for every sub-partition (one workgroup per segment); caps[n] = 0
+// is the scan sentinel.
+__global__ __launch_bounds__(256) void k_region_caps(const uint32_t* __restrict__ fine,
+                                                     const SegDesc* __restrict__ segs2,
+                                                     uint32_t nbins2,
+                                                     uint32_t* __restrict__ caps,
+                                                     uint32_t nsub) {
+    __shared__ uint32_t red[256 / TBJ_WAVE];
+    const uint32_t s = blockIdx.x;
+    const uint32_t* f = fine + (uint64_t)s * nbins2;
+    uint32_t tot = 0;
+    for (uint32_t b = threadIdx.x; b < nbins2; b += 256) tot += f[b];
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) tot += __shfl_xor(tot, d, TBJ_WAVE);
+    if (lane_id() == 0) red[wave_id()] = tot;
+    __syncthreads();
+    tot = 0;
+    for (int w = 0; w < 256 / TBJ_WAVE; w++) tot += red[w];
+    const double n = (double)segs2[s].n;
+    for (uint32_t b = threadIdx.x; b < nbins2; b += 256) {
+        uint32_t cap = 0;
+        if (n > 0) {
+            // ratio estimate of the bin's share of the exact segment size;
+            // bins with few samples are shrunk toward an even split by 8
+            // pseudo-samples, the larger estimate wins; room for 6 standard
+            // deviations of a 1/16 sample's estimate
+            const double raw = tot ? n * (double)f[b] / (double)tot : 0.0;
+            const double shrunk = n * ((double)f[b] + 8.0) / ((double)tot + 8.0 * nbins2);
+            const double e = raw > shrunk ? raw : shrunk;
+            cap = (uint32_t)min(n, e * 1.1 + 6.0 * sqrt(16.0 * e) + 64.0);
+        }
+        caps[(uint64_t)s * nbins2 + b] = cap;
+    }
+    if (s == 0 && threadIdx.x == 0) caps[nsub] = 0;
+}
+
+__global__ __launch_bounds__(kPartBlock) void k_scatter_est(
+    const uint2* __restrict__ in, uint2* __restrict__ out, const TileDesc* __restrict__ tiles,
+    const uint32_t* __restrict__ ntiles_dev, const uint32_t* __restrict__ starts,
+    uint32_t* __restrict__ cursors, uint32_t* __restrict__ overflow, BinSpec bs,
+    uint32_t nbins) {
+    const uint32_t t = blockIdx.x;
+    if (t >= *ntiles_dev) return;
+    __shared__ uint2 stage[kTile];
+    __shared__ uint16_t sbin[kTile];
+    __shared__ uint32_t loff[kMaxBins];
+    __shared__ uint32_t goff[kMaxBins];
+    __shared__ uint32_t glim[kMaxBins];
+    __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
+    extern __shared__ uint8_t dyn_lds[];
+    LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
+    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
+    BinFn bin = make_binfn(bs, &lm);
+    __syncthreads();
+    const TileDesc td = tiles[t];
+    uint2 v[kPartItems];
+    uint32_t b[kPartItems];
+    uint32_t rank[kPartItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kPartItems; k++) {
+        const uint32_t j = k * kPartBlock + threadIdx.x;
+        b[k] = TBJ_DROP;
+        if (j < td.len) v[k] = in[td.in_off + j];
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPartItems; k++) {
+        const uint32_t j = k * kPartBlock + threadIdx.x;
+        if (j < td.len) {
+            b[k] = bin(v[k].x);
+            if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
+        }
+    }
+    __syncthreads();
+    // claim space: one atomic per non-empty bin, before loff becomes offsets
+    const uint64_t sub0 = (uint64_t)td.seg * nbins;
+    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) {
+        const uint32_t c = loff[i];
+        uint32_t base = 0;
+        const uint32_t st = starts[sub0 + i], cap = starts[sub0 + i + 1] - st;
+        if (c) {
+            base = atomicAdd(&cursors[sub0 + i], c);
+            if (base + c > cap) atomicOr(overflow, 1u);
+        }
+        goff[i] = st + base;
+        glim[i] = st + cap;
+    }
+    __syncthreads();
+    const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
+    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) goff[i] -= loff[i];
+#pragma unroll
+    for (uint32_t k = 0; k < kPartItems; k++) {
+        if (b[k] != TBJ_DROP) {
+            const uint32_t slot = loff[b[k]] + rank[k];
+            stage[slot] = v[k];
+            sbin[slot] = (uint16_t)b[k];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < nvalid; j += kPartBlock) {
+        const uint32_t bb = sbin[j];
+        const uint32_t pos = goff[bb] + j;
+        if (pos < glim[bb]) out[pos] = stage[j];
+    }
+}
+
+// bounds[i] = {start, start + min(cursor, cap)} for the 2 * nparts partitions.
+__global__ void k_bounds_est(const uint32_t* __restrict__ starts,
+                             const uint32_t* __restrict__ cursors, uint32_t n,
+                             uint2* __restrict__ bounds) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t st = starts[i], cap = starts[i + 1] - st;
+    bounds[i] = make_uint2(st, st + min(cursors[i], cap));
+}
+
+// ---------------------------------------------------------------------------
 // Second-pass work descriptors: each (relation, first-pass partition) becomes
 // a segment cut into tiles.  One workgroup; npart1 <= kMaxBins.
 
 __global__ __launch_bounds__(kScanBlock) void k_make_segments(
     const uint32_t* __restrict__ scanned1, const SegDesc* __restrict__ segs1, uint32_t nseg1,
-    uint32_t nbins1, uint32_t nbins2, SegDesc* __restrict__ segs2, TileDesc* __restrict__ tiles2,
-    uint32_t* __restrict__ ntiles2_dev, uint32_t max_tiles2) {
+    uint32_t nbins1, uint32_t nbins2, SegDesc* __restrict__ segs2,
+    uint32_t* __restrict__ ntiles2_dev) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
     __shared__ uint32_t carry_s;
     const uint32_t nseg2 = nseg1 * nbins1;
     if (threadIdx.x == 0) carry_s = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nseg2; base += kScanBlock) {
-        uint32_t s2 = base + threadIdx.x;
+        const uint32_t s2 = base + threadIdx.x;
         uint32_t st = 0, n = 0, nt = 0;
         if (s2 < nseg2) {
-            uint32_t s1 = s2 / nbins1, b = s2 % nbins1;
+            const uint32_t s1 = s2 / nbins1, b = s2 % nbins1;
             const SegDesc sd = segs1[s1];
             st = scanned1[sd.hist_base + (uint64_t)b * sd.ntiles];
-            uint32_t en = scanned1[sd.hist_base + (uint64_t)(b + 1) * sd.ntiles];
+            const uint32_t en = scanned1[sd.hist_base + (uint64_t)(b + 1) * sd.ntiles];
             n = en - st;
             nt = (n + kTile - 1) / kTile;
         }
         uint32_t total;
-        uint32_t ex = block_exclusive_scan<kScanBlock>(nt, scratch, &total);
-        uint32_t first = carry_s + ex;
+        const uint32_t ex = block_exclusive_scan<kScanBlock>(nt, scratch, &total);
         if (s2 < nseg2) {
             SegDesc d;
             d.in_off = st;
             d.n = n;
             d.ntiles = nt;
-            d.first_tile = first;
-            d.hist_base = (uint64_t)first * nbins2;
+            d.first_tile = carry_s + ex;
+            d.hist_base = (uint64_t)d.first_tile * nbins2;
             segs2[s2] = d;
-            for (uint32_t k = 0; k < nt && first + k < max_tiles2; k++) {
-                TileDesc td;
-                td.in_off = st + k * kTile;
-                td.len = min((uint32_t)kTile, n - k * kTile);
-                td.seg = s2;
-                td.pad = 0;
-                tiles2[first + k] = td;
-            }
         }
         __syncthreads();
         if (threadIdx.x == 0) carry_s += total;
         __syncthreads();
     }
     if (threadIdx.x == 0) *ntiles2_dev = carry_s;
+}
+
+// One thread per second-pass tile: find the owning segment by binary search
+// over first_tile, then describe the tile.
+__global__ void k_make_tiles2(const SegDesc* __restrict__ segs2, uint32_t nseg2,
+                              const uint32_t* __restrict__ ntiles2_dev,
+                              TileDesc* __restrict__ tiles2) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *ntiles2_dev) return;
+    uint32_t lo = 0, hi = nseg2;  // last segment with first_tile <= t and ntiles > 0
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs2[mid].first_tile <= t) lo = mid;
+        else hi = mid;
+    }
+    // empty segments share first_tile with their successor: step forward
+    while (lo + 1 < nseg2 &&
+           (segs2[lo].ntiles == 0 || t >= segs2[lo].first_tile + segs2[lo].ntiles))
+        lo++;
+    const SegDesc sd = segs2[lo];
+    const uint32_t k = t - sd.first_tile;
+    TileDesc td;
+    td.in_off = sd.in_off + k * kTile;
+    td.len = min((uint32_t)kTile, sd.n - k * kTile);
+    td.seg = lo;
+    td.pad = 0;
+    tiles2[t] = td;
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +540,9 @@ __global__ void k_sample_keys(const uint2* __restrict__ in, uint64_t n, uint32_t
 // frac fixed-point bits (bits + frac == 32).
 __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restrict__ sample,
                                                        uint32_t m, uint32_t nseg_req,
-                                                       uint32_t bits, CdfModel* __restrict__ model) {
+                                                       uint32_t bits,
+                                                       const uint32_t* __restrict__ minmax,
+                                                       CdfModel* __restrict__ model) {
     __shared__ uint32_t a[kFitMax];
     __shared__ uint32_t knots[kLdsKnots + 1];
     __shared__ uint32_t nk_s;
@@ -354,10 +551,10 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
     for (uint32_t k = 2; k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) {
-                uint32_t ixj = i ^ j;
+                const uint32_t ixj = i ^ j;
                 if (ixj > i) {
-                    bool up = (i & k) == 0;
-                    uint32_t x = a[i], y = a[ixj];
+                    const bool up = (i & k) == 0;
+                    const uint32_t x = a[i], y = a[ixj];
                     if ((x > y) == up) {
                         a[i] = y;
                         a[ixj] = x;
@@ -367,12 +564,14 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
             __syncthreads();
         }
     }
+    const uint32_t kmin = minmax[0], kmax = minmax[1];
     if (threadIdx.x == 0) {
-        // distinct quantile knots; the first knot is the sample minimum
+        // first knot at R's exact minimum, then distinct sample quantiles
         uint32_t nk = 0;
-        for (uint32_t j = 0; j < nseg_req; j++) {
-            uint32_t q = a[(uint64_t)j * m / nseg_req];
-            if (nk == 0 || q > knots[nk - 1]) knots[nk++] = q;
+        knots[nk++] = kmin;
+        for (uint32_t j = 1; j < nseg_req; j++) {
+            const uint32_t q = a[(uint64_t)j * m / nseg_req];
+            if (q > knots[nk - 1] && q <= kmax) knots[nk++] = q;
         }
         nk_s = nk;
     }
@@ -381,7 +580,8 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
     const uint32_t frac = 32 - bits;
     const uint64_t span = (uint64_t)1 << 32;  // y spans [0, 2^32): 2^bits partitions
     for (uint32_t i = threadIdx.x; i <= nseg; i += kFitBlock) {
-        uint64_t xi = i < nseg ? (uint64_t)knots[i] : ((uint64_t)a[m - 1] + 1);
+        // the last knot closes the model at R's exact maximum
+        const uint64_t xi = i < nseg ? (uint64_t)knots[i] : ((uint64_t)kmax + 1);
         uint64_t yi = (span * i) / nseg;
         if (yi > 0xFFFFFFFFull) yi = 0xFFFFFFFFull;
         model->x[i] = xi;
@@ -389,22 +589,39 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nseg; i += kFitBlock) {
-        uint64_t dx = model->x[i + 1] - model->x[i];
-        uint64_t dy = (uint64_t)model->y[i + 1] - model->y[i];
+        const uint64_t dx = model->x[i + 1] - model->x[i];
+        const uint64_t dy = (uint64_t)model->y[i + 1] - model->y[i];
         model->slope[i] = dx ? ((dy << 32) / dx) : 0;
+    }
+    // guide cells: equal-width over [kmin, kmax]
+    uint32_t gshift = 0;
+    while ((((uint64_t)kmax - kmin) >> gshift) + 1 > (uint64_t)kGuideCells) gshift++;
+    const uint32_t ncells = (uint32_t)((((uint64_t)kmax - kmin) >> gshift) + 1);
+    for (uint32_t c = threadIdx.x; c <= ncells; c += kFitBlock) {
+        uint32_t g = nseg - 1;
+        if (c < ncells) {
+            const uint64_t key = (uint64_t)kmin + ((uint64_t)c << gshift);
+            uint32_t lo = 0, hi = nseg;  // largest i with knots[i] <= key
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (knots[mid] <= key) lo = mid;
+                else hi = mid;
+            }
+            g = lo;
+        }
+        model->guide[c] = (uint16_t)g;
     }
     if (threadIdx.x == 0) {
         model->nseg = nseg;
         model->frac = frac;
+        model->kmin = kmin;
+        model->kmax = kmax;
+        model->drop_outside = 1;
+        model->gshift = gshift;
+        model->ncells = ncells;
     }
 }
 
-// Fold the exact R key range found by the histogram pass into the model.
-__global__ void k_set_bounds(CdfModel* model, const uint32_t* minmax, uint32_t drop) {
-    model->kmin = minmax[0];
-    model->kmax = minmax[1];
-    model->drop_outside = drop;
-}
 
 // First-pass work descriptors: segment 0 = R, segment 1 = S, tiles of R
 // first.  Tile offsets are relative to the owning relation's base pointer.
@@ -466,10 +683,12 @@ static inline size_t model_lds(const BinSpec& bs) {
 
 void launch_hist(hipStream_t st, uint32_t grid, const uint2* in, const TileDesc* tiles,
                  const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* minmax) {
+                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine,
+                 const BinSpec* bs2, uint32_t nbins2) {
     if (!grid) return;
+    const BinSpec b2 = bs2 ? *bs2 : bs;
     hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles, segs,
-                       ntiles_dev, tile_base, hist, bs, nbins, minmax);
+                       ntiles_dev, tile_base, hist, bs, nbins, fine, b2, nbins2);
 }
 
 void launch_scatter(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
@@ -497,7 +716,10 @@ void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDes
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
                           TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2) {
     hipLaunchKernelGGL(k_make_segments, dim3(1), dim3(kScanBlock), 0, st, scanned1, segs1, nseg1,
-                       nbins1, nbins2, segs2, tiles2, ntiles2_dev, max_tiles2);
+                       nbins1, nbins2, segs2, ntiles2_dev);
+    if (max_tiles2)
+        hipLaunchKernelGGL(k_make_tiles2, dim3((max_tiles2 + 255) / 256), dim3(256), 0, st, segs2,
+                           nseg1 * nbins1, ntiles2_dev, tiles2);
 }
 
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample) {
@@ -505,12 +727,16 @@ void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint
 }
 
 void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nseg, uint32_t bits,
-                CdfModel* model) {
-    hipLaunchKernelGGL(k_fit_cdf, dim3(1), dim3(kFitBlock), 0, st, sample, m, nseg, bits, model);
+                const uint32_t* minmax, CdfModel* model) {
+    hipLaunchKernelGGL(k_fit_cdf, dim3(1), dim3(kFitBlock), 0, st, sample, m, nseg, bits, minmax,
+                       model);
 }
 
-void launch_set_bounds(hipStream_t st, CdfModel* model, const uint32_t* minmax, uint32_t drop) {
-    hipLaunchKernelGGL(k_set_bounds, dim3(1), dim3(1), 0, st, model, minmax, drop);
+void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax) {
+    if (!n) return;
+    uint64_t g = (n + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(k_minmax, dim3((uint32_t)g), dim3(256), 0, st, in, n, minmax);
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
@@ -531,6 +757,27 @@ void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t
     if (!max_tail) return;
     hipLaunchKernelGGL(k_zero_tail, dim3((uint32_t)((max_tail + 255) / 256)), dim3(256), 0, st,
                        hist, n, count_dev, mult);
+}
+
+void launch_region_caps(hipStream_t st, const uint32_t* fine, const SegDesc* segs2,
+                        uint32_t nseg2, uint32_t nbins2, uint32_t* caps) {
+    hipLaunchKernelGGL(k_region_caps, dim3(nseg2), dim3(256), 0, st, fine, segs2, nbins2, caps,
+                       nseg2 * nbins2);
+}
+
+void launch_scatter_est(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
+                        const TileDesc* tiles, const uint32_t* ntiles_dev, const uint32_t* starts,
+                        uint32_t* cursors, uint32_t* overflow, const BinSpec& bs,
+                        uint32_t nbins) {
+    if (!grid) return;
+    hipLaunchKernelGGL(k_scatter_est, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, out,
+                       tiles, ntiles_dev, starts, cursors, overflow, bs, nbins);
+}
+
+void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                       uint32_t n, uint2* bounds) {
+    hipLaunchKernelGGL(k_bounds_est, dim3((n + 255) / 256), dim3(256), 0, st, starts, cursors, n,
+                       bounds);
 }
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {

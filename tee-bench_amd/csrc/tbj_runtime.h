@@ -35,6 +35,24 @@ struct Context {
     hipStream_t own_stream = nullptr;
     Workspace ws;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t pev[24] = {};
+};
+
+// Per-phase device timing (tbj_options.profile): mark(phase) records an event
+// that closes the time since the previous mark into phase_ms[phase].
+class PhaseTimer {
+public:
+    PhaseTimer(Context& c, hipStream_t st, bool on) : c_(c), st_(st), on_(on) {}
+    int begin();
+    int mark(int phase);
+    int finish(tbj_result* res);
+
+private:
+    Context& c_;
+    hipStream_t st_;
+    bool on_;
+    int n_ = 0;
+    int phase_[24] = {};
 };
 
 Context& ctx();
@@ -60,5 +78,9 @@ void set_last_error(const char* what, hipError_t e, const char* file, int line);
 int run_radix_join(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
                    tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
                    hipStream_t st);
+int run_pht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
+            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
+int run_cht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
+            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
 
 }  // namespace tbj

@@ -29,7 +29,8 @@ class Options(ctypes.Structure):
                 ("passes", ctypes.c_int32), ("materialize", ctypes.c_int32),
                 ("workspace_cap", ctypes.c_uint64), ("learned_bound", ctypes.c_uint32),
                 ("learned_samples", ctypes.c_int32), ("s_chunk", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("sampled_regions", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 3)]
 
 
 class Result(ctypes.Structure):
@@ -39,10 +40,15 @@ class Result(ctypes.Structure):
                 ("passes", ctypes.c_int32), ("num_partitions", ctypes.c_uint32),
                 ("max_r_partition", ctypes.c_uint32), ("s_dropped", ctypes.c_uint64),
                 ("ms_partition", ctypes.c_float), ("ms_join", ctypes.c_float),
-                ("ms_total", ctypes.c_float), ("reserved_f", ctypes.c_float)]
+                ("ms_total", ctypes.c_float), ("redone_exact", ctypes.c_int32),
+                ("phase_ms", ctypes.c_float * 8)]
+
+    PHASES = ("hist1", "scan1", "scatter1", "size2", "scatter2", "tasks", "join", "other")
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved_f"}
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "phase_ms"}
+        d["phase_ms"] = dict(zip(self.PHASES, list(self.phase_ms)))
+        return d
 
 
 class TbjError(RuntimeError):
@@ -94,7 +100,8 @@ def init(device=0):
 
 
 def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
-                 workspace_cap=0, learned_bound=0, learned_samples=0, s_chunk=0):
+                 workspace_cap=0, learned_bound=0, learned_samples=0, s_chunk=0,
+                 sampled_regions=False, profile=False):
     o = Options()
     lib().tbj_default_options(ctypes.byref(o))
     o.partitioner = PARTITIONERS[partitioner] if isinstance(partitioner, str) else int(partitioner)
@@ -102,6 +109,8 @@ def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
     o.materialize = 1 if materialize else 0
     o.workspace_cap, o.learned_bound = workspace_cap, learned_bound
     o.learned_samples, o.s_chunk = learned_samples, s_chunk
+    o.sampled_regions = 1 if sampled_regions else 0
+    o.profile = 1 if profile else 0
     return o
 
 

@@ -123,3 +123,33 @@ def test_reference_config1_count(tbj):
     S = tbj.gen_fk(22222, 1 << 21, 1 << 21, payload=tbj.PAYLOAD_ZERO)
     res, _ = gu.run(tbj, "RHO", R, S, materialize=False)
     assert res.matches == 2097152 and res.checksum == 0
+
+
+@pytest.mark.parametrize("algo", ["RHO", "RHT"])
+def test_sampled_second_pass(tbj, oracle, algo):
+    nR, nS = 1 << 18, 1 << 20
+    R = tbj.gen_pk(11111, nR)
+    S = tbj.gen_zipf(22222, nS, nR, 1.2)
+    res = assert_join(tbj, oracle, algo, R, S, sampled_regions=True, radix_bits=14, passes=2)
+    assert res.passes == 2 and res.redone_exact == 0
+
+
+def test_sample_sized_region_overflow_redoes_exact(tbj, oracle):
+    # tuples the 1/16 sample sees all carry key 1; the rest carry key 2, whose
+    # sub-partition the sample believes empty
+    nR, nS = 1 << 14, 1 << 16
+    R = tbj.gen_pk(1, nR)
+    idx = np.arange(nS, dtype=np.uint64)
+    sampled = ((idx * 2654435761) & 0xFFFFFFFF) >> 28 == 0
+    S = rel(np.where(sampled, 1, 2))
+    res = assert_join(tbj, oracle, "RHO", R, S, radix_bits=12, passes=2, sampled_regions=True)
+    assert res.redone_exact == 1 and res.matches == nS
+
+
+def test_profile_phases(tbj, oracle):
+    nR, nS = 1 << 18, 1 << 20
+    R = tbj.gen_pk(3, nR)
+    S = tbj.gen_fk(4, nS, nR)
+    res = assert_join(tbj, oracle, "RHO", R, S, profile=True, radix_bits=14, passes=2)
+    ph = res.as_dict()["phase_ms"]
+    assert ph["join"] > 0 and ph["scatter1"] > 0 and ph["scatter2"] > 0

@@ -1,0 +1,147 @@
+"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+
+Each oracle function is checked against (a) golden vectors the reference's
+own code produced (tests/golden/golden.json, made by make_golden.py from the
+reference build), (b) glibc's rand() itself, and (c) what the mathematics
+fixes: brute-force joins on tiny inputs, closed-form counts, permutation
+and range invariants.  CPU only."""
+import ctypes
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, np.uint32).tobytes()).hexdigest()
+
+
+def gen(oracle, g):
+    if g["kind"] == "pk":
+        return oracle.gen_pk(g["seed"], g["n"])
+    if g["kind"] == "fk":
+        return oracle.gen_fk(g["seed"], g["n"], g["p"])
+    if g["kind"] == "fk_sel":
+        return oracle.gen_fk_sel(g["seed"], g["n"], g["p"])
+    return oracle.gen_zipf(g["seed"], g["n"], g["p"], g["z"])
+
+
+def test_rand_matches_golden_and_libc(oracle):
+    libc = ctypes.CDLL("libc.so.6")
+    for seed, vals in GOLDEN["rand"].items():
+        assert oracle.rand_stream(int(seed), 64).tolist() == vals
+        libc.srand(ctypes.c_uint(int(seed)))
+        assert [libc.rand() for _ in range(64)] == vals
+
+
+@pytest.mark.parametrize("g", GOLDEN["generators"], ids=lambda g: f"{g['kind']}-{g['n']}-{g['z']}")
+def test_generators_match_reference(oracle, g):
+    k = gen(oracle, g)
+    assert k[:16].tolist() == g["head"]
+    assert sha(k) == g["sha256"]
+
+
+def test_generator_invariants(oracle):
+    n = 100003
+    pk = oracle.gen_pk(5, n)
+    assert np.array_equal(np.sort(pk), np.arange(1, n + 1, dtype=np.uint32))
+    fk = oracle.gen_fk(6, 3 * n + 17, n)
+    for b in range(3):  # each full block is a permutation of 1..maxid
+        assert np.array_equal(np.sort(fk[b * n:(b + 1) * n]), np.arange(1, n + 1))
+    assert np.array_equal(np.sort(fk[3 * n:]), np.arange(1, 18))
+    z = oracle.gen_zipf(7, 50000, 1000, 1.0)
+    assert z.min() >= 1 and z.max() <= 1000
+    # the overflowed alphabet shuffle keeps hot ranks on small keys
+    assert np.bincount(z).argmax() <= 3
+
+
+def brute_count(rk, sk):
+    return int((rk[:, None] == sk[None, :]).sum())
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_joins_against_brute_force(oracle, seed):
+    rng = np.random.default_rng(seed)
+    rk = rng.integers(0, 60, 300).astype(np.uint32)
+    sk = rng.integers(0, 80, 500).astype(np.uint32)
+    rp = rng.integers(0, 1 << 32, 300, dtype=np.uint64).astype(np.uint32)
+    sp = rng.integers(0, 1 << 32, 500, dtype=np.uint64).astype(np.uint32)
+    want = brute_count(rk, sk)
+    assert oracle.count(rk, sk) == want
+    assert oracle.rho(rk, rp, sk, sp) == want
+    assert oracle.rht(rk, rp, sk, sp) == want
+    assert oracle.pht(rk, sk) == want
+    assert oracle.sortmerge(rk, sk) == want
+    ii, jj = np.nonzero(rk[:, None] == sk[None, :])
+    bw = sorted(zip(rp[ii].tolist(), sp[jj].tolist()))
+    for fn in (oracle.rho, oracle.rht):
+        n, a, b = fn(rk, rp, sk, sp, pairs=True)
+        assert n == want and list(zip(a.tolist(), b.tolist())) == bw
+    n, a, b = oracle.pairs(rk, rp, sk, sp)
+    assert list(zip(a.tolist(), b.tolist())) == bw
+
+
+def test_pairs64_order_and_content(oracle):
+    rng = np.random.default_rng(9)
+    rk = rng.integers(0, 40, 200).astype(np.uint64) << np.uint64(40)
+    sk = rng.integers(0, 50, 300).astype(np.uint64) << np.uint64(40)
+    rp = np.arange(200, dtype=np.uint64) + np.uint64(1 << 50)
+    sp = np.arange(300, dtype=np.uint64)
+    n, a, b = oracle.pairs64(rk, rp, sk, sp)
+    ii, jj = np.nonzero(rk[:, None] == sk[None, :])
+    assert n == len(ii)
+    want = sorted(zip(rk[ii].tolist(), rp[ii].tolist(), sp[jj].tolist()))
+    assert list(zip(a.tolist(), b.tolist())) == [(x[1], x[2]) for x in want]
+
+
+@pytest.mark.parametrize("case", GOLDEN["joins"], ids=lambda c: c["name"])
+def test_join_counts_match_reference(oracle, case):
+    rk = oracle.gen_pk(11111, case["nR"])
+    g = {"kind": case["S"], "seed": 22222, "n": case["nS"], "p": case["p"], "z": case["z"]}
+    sk = gen(oracle, g)
+    assert sha(rk) == case["r_sha256"] and sha(sk) == case["s_sha256"]
+    c = case["counts"]
+    true = oracle.count(rk, sk)
+    # RHO, RHT, RJ, PHT and NPO_st compute the true join in the reference
+    for algo in ("RHO", "RHT", "RJ", "PHT", "NPO_st"):
+        assert c[algo] == true
+    assert oracle.rho(rk, None, sk, None) == true
+    assert oracle.rht(rk, None, sk, None) == true
+    assert oracle.pht(rk, sk) == true
+    assert oracle.sortmerge(rk, sk) == true
+    # PSM's unbounded merge may overrun by one; RSM's drifts (reference quirks)
+    assert c["PSM"] in (true, true + 1, "crash")
+    if c["CHT"] is not None:
+        rp = np.arange(case["nR"], dtype=np.uint32)
+        sp = np.arange(case["nS"], dtype=np.uint32)
+        m, cks, unresolved = oracle.cht(rk, rp, sk, sp)
+        assert unresolved == 0 and m == c["CHT"] == true and cks == c["CHT_checksum"]
+
+
+@pytest.mark.parametrize("case", GOLDEN["pairs"], ids=lambda c: c["name"])
+def test_rho_pairs_match_reference(oracle, case):
+    rk = oracle.gen_pk(11111, case["nR"])
+    g = {"kind": case["S"], "seed": 22222, "n": case["nS"], "p": case["p"], "z": case["z"]}
+    sk = gen(oracle, g)
+    rp = np.arange(case["nR"], dtype=np.uint32)
+    sp = np.arange(case["nS"], dtype=np.uint32)
+    n, a, b = oracle.rho(rk, rp, sk, sp, pairs=True)
+    assert n == case["matches"]
+    w = (a.astype(np.uint64) << np.uint64(32)) | b.astype(np.uint64)
+    assert hashlib.sha256(np.sort(w).astype("<u8").tobytes()).hexdigest() == case["sorted_pairs_sha256"]
+    if "pairs" in case:
+        assert [[int(x), int(y)] for x, y in zip(a, b)] == case["pairs"]
+
+
+def test_cht_restatement_flags_nondense_drops(oracle):
+    # three keys with the same home bit: the reference drops the third
+    # |R| = 103 -> a 128-bit bitmap; 5, 133 and 261 all have home bit 5
+    rk = np.array([5, 133, 261] + list(range(1000, 1100)), np.uint32)
+    sk = np.array([5, 133, 261], np.uint32)
+    m, _, unresolved = oracle.cht(rk, None, sk, None)
+    assert oracle.count(rk, sk) == 3
+    assert unresolved >= 1 and m < 3

@@ -110,8 +110,8 @@ template <int MODE, bool EMIT>
 __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hmask,
                                           const uint32_t* rkey, const uint32_t* rpay,
                                           const uint32_t* head, const uint16_t* nxt,
-                                          uint32_t& c, uint64_t& cks, uint2* __restrict__ pairs,
-                                          uint64_t cap, uint64_t& o) {
+                                          uint32_t& c, uint64_t& cks, uint32_t& first,
+                                          uint2* __restrict__ pairs, uint64_t cap, uint64_t& o) {
     const uint32_t key = sv.x;
     const uint32_t h = (key >> hshift) & hmask;
     if (MODE == kModeChain) {
@@ -121,8 +121,9 @@ __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hm
                     if (o < cap) pairs[o] = make_uint2(rpay[e - 1], sv.y);
                     o++;
                 } else {
+                    first = rpay[e - 1];
                     c++;
-                    cks += (uint64_t)rpay[e - 1] + sv.y;
+                    cks += (uint64_t)first + sv.y;
                 }
             }
         }
@@ -134,8 +135,9 @@ __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hm
                     if (o < cap) pairs[o] = make_uint2(rpay[e], sv.y);
                     o++;
                 } else {
+                    first = rpay[e];
                     c++;
-                    cks += (uint64_t)rpay[e] + sv.y;
+                    cks += (uint64_t)first + sv.y;
                 }
             }
         }
@@ -145,9 +147,10 @@ __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hm
 // One task per loop trip: every R and S tuple of the task is loaded into
 // registers first (one memory phase), the R range is built into LDS, the S
 // tuples are probed, and with MAT the block claims one output range for
-// the whole task.  S ranges are at most kJoinBlock * kJoinSItems tuples.
-template <int MODE, bool MAT>
-__global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ Rb,
+// the whole task.  S ranges are at most kJoinBlock * join_s_items(MAT)
+// tuples (fewer with MAT, whose per-item output slots need registers).
+template <int MODE, bool MAT, int SITEMS = MAT ? kJoinSItems / 2 : kJoinSItems>
+__global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join(const uint2* __restrict__ Rb,
                                                      const uint2* __restrict__ Sb,
                                                      const JoinTask* __restrict__ tasks,
                                                      const uint32_t* __restrict__ ntasks_dev,
@@ -165,6 +168,9 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
     __shared__ uint32_t task_s;
     __shared__ unsigned long long obase_s;
     __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
+    constexpr int NCLAIM = SITEMS > 8 ? 2 : 1;  // output claims per task
+    constexpr int HALF = SITEMS / NCLAIM;       // items per output claim
+    __shared__ uint32_t wtot[kJoinBlock / TBJ_WAVE][HALF];
 
     const uint32_t ntasks = *ntasks_dev;
     uint64_t matches = 0, cks = 0;
@@ -177,14 +183,14 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
         const JoinTask tk = tasks[t];
         const uint32_t n = tk.r_len;
         // ---- loads: R and S of the task in flight together
-        uint2 rv[RITEMS], sv[kJoinSItems];
+        uint2 rv[RITEMS], sv[SITEMS];
 #pragma unroll
         for (int k = 0; k < RITEMS; k++) {
             const uint32_t j = k * kJoinBlock + threadIdx.x;
             if (j < n) rv[k] = Rb[tk.r_off + j];
         }
 #pragma unroll
-        for (int k = 0; k < kJoinSItems; k++) {
+        for (int k = 0; k < SITEMS; k++) {
             const uint32_t j = k * kJoinBlock + threadIdx.x;
             if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
         }
@@ -231,33 +237,72 @@ __global__ __launch_bounds__(kJoinBlock) void k_join(const uint2* __restrict__ R
         }
 
         // ---- probe
-        uint32_t c = 0;
-        uint64_t o = 0;
+        if (!MAT) {
+            uint32_t c = 0, fdummy = 0;
+            uint64_t o = 0;
 #pragma unroll
-        for (int k = 0; k < kJoinSItems; k++) {
-            const uint32_t j = k * kJoinBlock + threadIdx.x;
-            if (j < tk.s_len)
-                lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c, cks, pairs,
-                                       cap, o);
-        }
-        matches += c;
-        if (MAT) {
-            uint32_t total;
-            const uint32_t ex = block_exclusive_scan<kJoinBlock>(c, scratch, &total);
-            if (threadIdx.x == 0)
-                obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
-            __syncthreads();
-            o = obase_s + ex;
-            if (c) {
-                uint32_t c2 = 0;
-                uint64_t k2 = 0;
+            for (int k = 0; k < SITEMS; k++) {
+                const uint32_t j = k * kJoinBlock + threadIdx.x;
+                if (j < tk.s_len)
+                    lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c, cks,
+                                           fdummy, pairs, cap, o);
+            }
+            matches += c;
+        } else {
+            // NCLAIM groups of HALF items: count (remembering the first
+            // match), claim one output range per group, write.  Slots go in
+            // item-major order (all threads' item 0, then item 1, ...) so that
+            // a wave's stores of one item are contiguous.
+            const uint32_t lane = lane_id(), w = wave_id();
 #pragma unroll
-                for (int k = 0; k < kJoinSItems; k++) {
+            for (int hh = 0; hh < NCLAIM; hh++) {
+                uint32_t ck[HALF], fp[HALF];
+#pragma unroll
+                for (int q = 0; q < HALF; q++) {
+                    const int k = hh * HALF + q;
                     const uint32_t j = k * kJoinBlock + threadIdx.x;
+                    uint32_t ci = 0;
+                    uint64_t od = 0;
+                    fp[q] = 0;
                     if (j < tk.s_len)
-                        lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
-                                              pairs, cap, o);
+                        lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, ci, cks,
+                                               fp[q], pairs, cap, od);
+                    ck[q] = ci;
+                    matches += ci;
+                    const uint32_t incl = wave_inclusive_scan(ci);
+                    if (lane == TBJ_WAVE - 1) wtot[w][q] = incl;
                 }
+                __syncthreads();
+                if (threadIdx.x == 0) {
+                    uint32_t tot = 0;
+                    for (int q = 0; q < HALF; q++)
+                        for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) tot += wtot[ww][q];
+                    obase_s = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
+                }
+                __syncthreads();
+                uint64_t item_base = obase_s;
+#pragma unroll
+                for (int q = 0; q < HALF; q++) {
+                    const int k = hh * HALF + q;
+                    uint32_t before = 0, all = 0;
+                    for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) {
+                        const uint32_t v = wtot[ww][q];
+                        before += (uint32_t)ww < w ? v : 0;
+                        all += v;
+                    }
+                    const uint32_t incl = wave_inclusive_scan(ck[q]);
+                    uint64_t ok = item_base + before + incl - ck[q];
+                    if (ck[q] == 1) {
+                        if (ok < cap) pairs[ok] = make_uint2(fp[q], sv[k].y);
+                    } else if (ck[q] > 1) {
+                        uint32_t c2 = 0, f2 = 0;
+                        uint64_t k2 = 0;
+                        lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
+                                              f2, pairs, cap, ok);
+                    }
+                    item_base += all;
+                }
+                __syncthreads();
             }
         }
         __syncthreads();

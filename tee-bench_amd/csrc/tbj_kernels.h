@@ -19,7 +19,7 @@ constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanChunk = kScanBlock * kScanItems;
 // CDF model fit
-constexpr int kLdsKnots = 1024;
+constexpr int kLdsKnots = 256;
 constexpr int kFitBlock = 1024;
 constexpr int kFitMax = 16384;
 // Join kernels
@@ -28,17 +28,21 @@ constexpr int kJoinCap = 4096;   // max R tuples of one build (LDS hash table)
 constexpr int kJoinProbeItems = 4;
 constexpr int kJoinSItems = 16;  // S tuples per thread of one task
 constexpr int kJoinSChunk = kJoinBlock * kJoinSItems;  // 8192
+// S tuples per join task: kJoinSChunk counting, half of it materialising
+// (the per-item output slots need the registers)
+inline int join_s_chunk(bool materialize) { return materialize ? kJoinSChunk / 2 : kJoinSChunk; }
 
 enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
 size_t lds_model_bytes();
 
+// Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev).
 // fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
-void launch_hist(hipStream_t st, uint32_t grid, const uint2* in, const TileDesc* tiles,
+void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
                  uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
                  const BinSpec* bs2 = nullptr, uint32_t nbins2 = 1);
-void launch_scatter(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
+void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
                     uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
                     uint32_t nbins);

@@ -33,7 +33,7 @@ struct JoinTask {
 // monotone).  kmin/kmax are R's exact key bounds: keys outside are dropped
 // from S because they cannot match.
 constexpr int kMaxKnots = 4096;
-constexpr int kGuideCells = 4096;  // equal-width cells over [kmin, kmax]
+constexpr int kGuideCells = 2048;  // equal-width cells over [kmin, kmax]
 struct CdfModel {
     uint32_t nseg;
     uint32_t frac;        // fixed-point fraction bits of y

@@ -17,19 +17,23 @@ namespace tbj {
 // ---------------------------------------------------------------------------
 // Partition function
 
-// LDS copy of a CdfModel's knots (at most kLdsKnots segments) and guide.
+// LDS form of a CdfModel: per segment its first key, the model value at that
+// key and the slope as floats (in partitions), and the range of partitions
+// the segment maps to; plus the guide cells.  Evaluating in float keeps the
+// map monotone inside a segment (conversions, products and sums round
+// monotonically) and the clamp to the segment's partition range keeps it
+// monotone across segments; R and S go through the same code, so equal keys
+// always land in the same partition.
 struct LdsModel {
     uint32_t x[kLdsKnots + 1];
-    uint32_t y[kLdsKnots + 1];
-    uint64_t s[kLdsKnots];
+    float base[kLdsKnots];
+    float slope[kLdsKnots];
+    uint32_t plo[kLdsKnots], phi[kLdsKnots];
     uint16_t guide[kGuideCells + 1];
 };
 
-// Monotone model value of a key, in partitions: find the segment through the
-// guide cell (usually one or two steps), then interpolate in fixed point.
-__device__ __forceinline__ uint32_t model_eval(const LdsModel* m, uint32_t nseg, uint32_t frac,
-                                               uint32_t kmin, uint32_t gshift, uint32_t ncells,
-                                               uint32_t key) {
+__device__ __forceinline__ uint32_t model_eval(const LdsModel* m, uint32_t nseg, uint32_t kmin,
+                                               uint32_t gshift, uint32_t ncells, uint32_t key) {
     if (key < m->x[0]) return 0;
     uint32_t c = (key - kmin) >> gshift;
     if (c >= ncells) c = ncells - 1;
@@ -40,24 +44,24 @@ __device__ __forceinline__ uint32_t model_eval(const LdsModel* m, uint32_t nseg,
         if (m->x[mid] <= key) lo = mid;
         else hi = mid;
     }
-    const uint64_t d = (uint64_t)(key - m->x[lo]);
-    const uint64_t sl = m->s[lo];
-    // (d * sl) >> 32 with a 128-bit product; the result stays below y[lo+1]-y[lo]
-    const uint64_t lo64 = d * sl;
-    const uint64_t hi64 = __umul64hi(d, sl);
-    const uint64_t add = (hi64 << 32) | (lo64 >> 32);
-    uint64_t v = (uint64_t)m->y[lo] + add;
-    const uint64_t vmax = (uint64_t)m->y[lo + 1] - (m->y[lo + 1] > m->y[lo] ? 1 : 0);
-    if (v > vmax) v = vmax;
-    return (uint32_t)(v >> frac);
+    const float v = fmaf((float)(key - m->x[lo]), m->slope[lo], m->base[lo]);
+    uint32_t p = v <= 0.0f ? 0u : (uint32_t)v;
+    p = max(p, m->plo[lo]);
+    return min(p, m->phi[lo]);
 }
 
 __device__ __forceinline__ void load_model(LdsModel* lm, const CdfModel* m) {
     const uint32_t n = m->nseg;
+    const float unit = 1.0f / (float)(1u << (m->frac > 31 ? 31 : m->frac));
     for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
         lm->x[i] = i < n ? (uint32_t)m->x[i] : 0xFFFFFFFFu;
-        lm->y[i] = m->y[i];
-        if (i < n) lm->s[i] = m->slope[i];
+        if (i < n) {
+            const uint32_t y0 = m->y[i], y1 = m->y[i + 1];
+            lm->base[i] = (float)y0 * unit;
+            lm->slope[i] = (float)((double)(y1 - y0) * unit / (double)(m->x[i + 1] - m->x[i]));
+            lm->plo[i] = y0 >> m->frac;
+            lm->phi[i] = (y1 > y0 ? y1 - 1 : y0) >> m->frac;
+        }
     }
     for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->guide[i] = m->guide[i];
 }
@@ -65,7 +69,7 @@ __device__ __forceinline__ void load_model(LdsModel* lm, const CdfModel* m) {
 struct BinFn {
     BinSpec bs;
     const LdsModel* lm;
-    uint32_t nseg, frac, kmin, kmax, gshift, ncells;
+    uint32_t nseg, kmin, kmax, gshift, ncells;
 
     __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
         uint32_t full;
@@ -73,7 +77,7 @@ struct BinFn {
             full = bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
         } else {
             if (bs.drop && (key < kmin || key > kmax)) return TBJ_DROP;
-            full = model_eval(lm, nseg, frac, kmin, gshift, ncells, key);
+            full = model_eval(lm, nseg, kmin, gshift, ncells, key);
         }
         return (full >> bs.pass_shift) & bs.pass_mask;
     }
@@ -83,11 +87,10 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     BinFn f;
     f.bs = bs;
     f.lm = lm;
-    f.nseg = f.frac = f.kmin = f.kmax = f.gshift = f.ncells = 0;
+    f.nseg = f.kmin = f.kmax = f.gshift = f.ncells = 0;
     if (bs.kind == kBinLearned) {
         load_model(lm, bs.model);
         f.nseg = bs.model->nseg;
-        f.frac = bs.model->frac;
         f.kmin = bs.model->kmin;
         f.kmax = bs.model->kmax;
         f.gshift = bs.model->gshift;
@@ -126,48 +129,54 @@ __device__ __forceinline__ uint32_t xcd_slot(uint32_t tis, uint32_t ntiles, uint
     return before + (tis - r0g) / kNumXcd;
 }
 
+// Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
+// ... below tile_end and *ntiles_dev, loading the partition function (the
+// learned model) into LDS once.
 __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ in,
                                                      const TileDesc* __restrict__ tiles,
                                                      const SegDesc* __restrict__ segs,
                                                      const uint32_t* __restrict__ ntiles_dev,
-                                                     uint32_t tile_base, uint32_t* __restrict__ hist,
-                                                     BinSpec bs, uint32_t nbins,
-                                                     uint32_t* __restrict__ fine, BinSpec bs2,
-                                                     uint32_t nbins2) {
-    const uint32_t t = blockIdx.x + tile_base;
-    if (t >= *ntiles_dev) return;
+                                                     uint32_t tile_base, uint32_t tile_end,
+                                                     uint32_t* __restrict__ hist, BinSpec bs,
+                                                     uint32_t nbins, uint32_t* __restrict__ fine,
+                                                     BinSpec bs2, uint32_t nbins2) {
     __shared__ uint32_t cnt[kMaxBins];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
     BinFn bin = make_binfn(bs, &lm);
     BinFn bin2 = bin;
     bin2.bs = bs2;
-    __syncthreads();
-    const TileDesc td = tiles[t];
-    const SegDesc sd = segs[td.seg];
-    uint32_t key[kPartItems];
+    const uint32_t end = min(tile_end, *ntiles_dev);
+    for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
+        __syncthreads();
+        const TileDesc td = tiles[t];
+        const SegDesc sd = segs[td.seg];
+        uint32_t key[kPartItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        const uint32_t j = k * kPartBlock + threadIdx.x;
-        key[k] = j < td.len ? in[td.in_off + j].x : 0;
-    }
+        for (uint32_t k = 0; k < kPartItems; k++) {
+            const uint32_t j = k * kPartBlock + threadIdx.x;
+            key[k] = j < td.len ? in[td.in_off + j].x : 0;
+        }
 #pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        const uint32_t j = k * kPartBlock + threadIdx.x;
-        if (j < td.len) {
-            const uint32_t b = bin(key[k]);
-            if (b != TBJ_DROP) {
-                atomicAdd(&cnt[b], 1u);
-                if (fine && sampled16(td.in_off + j))
-                    atomicAdd(&fine[((uint64_t)td.seg * nbins + b) * nbins2 + bin2(key[k])], 1u);
+        for (uint32_t k = 0; k < kPartItems; k++) {
+            const uint32_t j = k * kPartBlock + threadIdx.x;
+            if (j < td.len) {
+                const uint32_t b = bin(key[k]);
+                if (b != TBJ_DROP) {
+                    atomicAdd(&cnt[b], 1u);
+                    if (fine && sampled16(td.in_off + j))
+                        atomicAdd(&fine[((uint64_t)td.seg * nbins + b) * nbins2 + bin2(key[k])],
+                                  1u);
+                }
             }
         }
+        __syncthreads();
+        const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
+        for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
+            hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
+        __syncthreads();
     }
-    __syncthreads();
-    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, blockIdx.x);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-        hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
 }
 
 // Exact key range of a relation: minmax[0] = min, minmax[1] = max (must start
@@ -275,11 +284,9 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
                                                         const TileDesc* __restrict__ tiles,
                                                         const SegDesc* __restrict__ segs,
                                                         const uint32_t* __restrict__ ntiles_dev,
-                                                        uint32_t tile_base,
+                                                        uint32_t tile_base, uint32_t tile_end,
                                                         const uint32_t* __restrict__ scanned,
                                                         BinSpec bs, uint32_t nbins) {
-    const uint32_t t = blockIdx.x + tile_base;
-    if (t >= *ntiles_dev) return;
     __shared__ uint2 stage[kTile];
     __shared__ uint16_t sbin[kTile];
     __shared__ uint32_t loff[kMaxBins];
@@ -287,45 +294,49 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
     BinFn bin = make_binfn(bs, &lm);
-    __syncthreads();
-    const TileDesc td = tiles[t];
-    uint2 v[kPartItems];
-    uint32_t b[kPartItems];
-    uint32_t rank[kPartItems];
+    const uint32_t end = min(tile_end, *ntiles_dev);
+    for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
+        for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
+        __syncthreads();
+        const TileDesc td = tiles[t];
+        uint2 v[kPartItems];
+        uint32_t b[kPartItems];
+        uint32_t rank[kPartItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        uint32_t j = k * kPartBlock + threadIdx.x;
-        b[k] = TBJ_DROP;
-        if (j < td.len) v[k] = in[td.in_off + j];
-    }
-#pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        uint32_t j = k * kPartBlock + threadIdx.x;
-        if (j < td.len) {
-            b[k] = bin(v[k].x);
-            if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
+        for (uint32_t k = 0; k < kPartItems; k++) {
+            const uint32_t j = k * kPartBlock + threadIdx.x;
+            b[k] = TBJ_DROP;
+            if (j < td.len) v[k] = in[td.in_off + j];
         }
-    }
-    __syncthreads();
-    const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
-    const SegDesc sd = segs[td.seg];
-    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, blockIdx.x);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-        goff[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis] - loff[i];
 #pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        if (b[k] != TBJ_DROP) {
-            uint32_t slot = loff[b[k]] + rank[k];
-            stage[slot] = v[k];
-            sbin[slot] = (uint16_t)b[k];
+        for (uint32_t k = 0; k < kPartItems; k++) {
+            const uint32_t j = k * kPartBlock + threadIdx.x;
+            if (j < td.len) {
+                b[k] = bin(v[k].x);
+                if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
+            }
         }
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < nvalid; j += kPartBlock) {
-        uint32_t bb = sbin[j];
-        out[goff[bb] + j] = stage[j];
+        __syncthreads();
+        const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
+        const SegDesc sd = segs[td.seg];
+        const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
+        for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
+            goff[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis] - loff[i];
+#pragma unroll
+        for (uint32_t k = 0; k < kPartItems; k++) {
+            if (b[k] != TBJ_DROP) {
+                const uint32_t slot = loff[b[k]] + rank[k];
+                stage[slot] = v[k];
+                sbin[slot] = (uint16_t)b[k];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nvalid; j += kPartBlock) {
+            const uint32_t bb = sbin[j];
+            out[goff[bb] + j] = stage[j];
+        }
+        __syncthreads();
     }
 }
 
@@ -681,23 +692,43 @@ static inline size_t model_lds(const BinSpec& bs) {
     return bs.kind == kBinLearned ? sizeof(LdsModel) : 0;
 }
 
-void launch_hist(hipStream_t st, uint32_t grid, const uint2* in, const TileDesc* tiles,
+// Persistent grid: a multiple of the XCD count so that a workgroup's tiles
+// t = blockIdx.x + k * grid stay congruent mod 8 (one XCD per slot group).
+static uint32_t persistent_grid(uint32_t ntiles) {
+    static uint32_t cap = 0;
+    if (!cap) {
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        cap = (uint32_t)(cus > 0 ? cus : 256) * 2;
+        cap = (cap + 7) / 8 * 8;
+    }
+    return ntiles < cap ? ntiles : cap;
+}
+
+void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
                  uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine,
                  const BinSpec* bs2, uint32_t nbins2) {
-    if (!grid) return;
+    if (!ntiles) return;
     const BinSpec b2 = bs2 ? *bs2 : bs;
-    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles, segs,
-                       ntiles_dev, tile_base, hist, bs, nbins, fine, b2, nbins2);
+    // the learned model is loaded into LDS once per workgroup: persistent
+    // grid; radix: one tile per workgroup
+    const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
+    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st,
+                       in, tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins,
+                       fine, b2, nbins2);
 }
 
-void launch_scatter(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
+void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
                     uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
                     uint32_t nbins) {
-    if (!grid) return;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, out, tiles,
-                       segs, ntiles_dev, tile_base, scanned, bs, nbins);
+    if (!ntiles) return;
+    const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kPartBlock), model_lds(bs),
+                       st, in, out, tiles, segs, ntiles_dev, tile_base, tile_base + ntiles,
+                       scanned, bs, nbins);
 }
 
 uint32_t scan_partials(uint64_t n) { return (uint32_t)((n + kScanChunk - 1) / kScanChunk) + 1; }

@@ -136,9 +136,14 @@ int tbj_join_host(int algo,
                   const tbj_options* opt, tbj_result* res);
 
 /*
- * 64-bit keys, 16-byte tuples, sort-merge join (algo TBJ_PSM or TBJ_RSM).
- * pairs come out ordered by (key, R.payload, S.payload) when the learned
- * range partitioner is used or algo is TBJ_PSM.
+ * 64-bit keys, 16-byte tuples, sort-merge join (algo TBJ_PSM or TBJ_RSM;
+ * other algos return TBJ_ERR_UNSUPPORTED).  No reference counterpart: the
+ * reference's tuples are 8 bytes (Include/data-types.h:32-47).
+ * TBJ_PSM (or partitioner TBJ_PART_LEARNED) range-partitions by the learned
+ * key CDF: materialised pairs come out in key order.  TBJ_RSM with the radix
+ * partitioner: key order inside each partition of the low key bits.  Either
+ * way the pairs of one key are contiguous; their order inside that block is
+ * unspecified.  Limits: nR + nS < 2^32 - 2^20 (TBJ_ERR_TOO_LARGE).
  */
 int tbj_join64(int algo,
                const tbj_tuple64* R, uint64_t nR,

@@ -233,9 +233,21 @@ int tbj_join_host(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S,
 int tbj_join64(int algo, const tbj_tuple64* R, uint64_t nR, const tbj_tuple64* S, uint64_t nS,
                tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options* opt, tbj_result* res,
                void* stream) {
-    (void)algo; (void)R; (void)nR; (void)S; (void)nS; (void)pairs; (void)pairs_cap;
-    (void)opt; (void)res; (void)stream;
-    return TBJ_ERR_UNSUPPORTED;
+    if (!res || algo < 0 || algo >= TBJ_ALGO_COUNT) return TBJ_ERR_INVALID;
+    if ((nR && !R) || (nS && !S)) return TBJ_ERR_INVALID;
+    if (algo != TBJ_PSM && algo != TBJ_RSM) return TBJ_ERR_UNSUPPORTED;
+    tbj_options o;
+    if (opt) o = *opt;
+    else tbj_default_options(&o);
+    if (o.materialize && pairs_cap && !pairs) return TBJ_ERR_INVALID;
+    std::lock_guard<std::mutex> g(g_mu);
+    if (!g_ctx.ready) {
+        int s = init_locked(0);
+        if (s != TBJ_OK) return s;
+    }
+    memset(res, 0, sizeof(*res));
+    g_ctx.ws.cap = o.workspace_cap;
+    return run_sm64(algo, R, nR, S, nS, pairs, pairs_cap, o, res, (hipStream_t)stream);
 }
 
 }  // extern "C"

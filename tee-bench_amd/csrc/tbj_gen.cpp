@@ -257,38 +257,33 @@ static inline uint64_t splitmix64(uint64_t& x) {
     return z ^ (z >> 31);
 }
 
+// splitmix64's output is a bijection of its counter, so keys drawn from
+// distinct counters are distinct.  Tuple i takes counter seed + (i+1)*gamma;
+// the (never observed) zero output is replaced by the output of a counter
+// past the last tuple's, which keeps every key distinct and nonzero.
 int tbj_gen_unique64(uint64_t seed, uint64_t n, tbj_tuple64* out, int nthreads) {
     if (!out && n) return TBJ_ERR_INVALID;
-    // draw, sort a copy, redraw duplicates until all keys are distinct
-    uint64_t x = seed;
-    for (uint64_t i = 0; i < n; i++) {
-        uint64_t k;
-        do k = splitmix64(x);
-        while (k == 0);
-        out[i].key = k;
-        out[i].payload = i;
-    }
-    for (int round = 0; round < 64; round++) {
-        std::vector<std::pair<uint64_t, uint64_t>> v(n);
-        parallel_for((n + (1 << 22) - 1) / (1 << 22), nthreads, [&](size_t c) {
-            uint64_t a = c << 22, b = std::min<uint64_t>(n, a + (1 << 22));
-            for (uint64_t i = a; i < b; i++) v[i] = {out[i].key, i};
-        });
-        std::sort(v.begin(), v.end());
-        bool dup = false;
-        for (uint64_t i = 1; i < n; i++)
-            if (v[i].first == v[i - 1].first) {
-                uint64_t k;
-                do k = splitmix64(x);
-                while (k == 0);
-                out[v[i].second].key = k;
-                dup = true;
+    const uint64_t gamma = 0x9E3779B97F4A7C15ull, chunk = 1 << 20;
+    std::atomic<uint64_t> extra{0};
+    parallel_for((n + chunk - 1) / chunk, nthreads, [&](size_t c) {
+        const uint64_t a = c * chunk, b = std::min<uint64_t>(n, a + chunk);
+        for (uint64_t i = a; i < b; i++) {
+            uint64_t x = seed + i * gamma;
+            uint64_t k = splitmix64(x);
+            while (k == 0) {
+                x = seed + (n + extra.fetch_add(1)) * gamma;
+                k = splitmix64(x);
             }
-        if (!dup) return TBJ_OK;
-    }
-    return TBJ_ERR_INVALID;
+            out[i].key = k;
+            out[i].payload = i;
+        }
+    });
+    return TBJ_OK;
 }
 
+// S keys are R's keys at Zipf(z) ranks (rank 1 = R[0]): inverse-CDF lookup
+// of a uniform double.  Blocks of 2^16 tuples own independent splitmix64
+// streams, so the output does not depend on the thread count.
 int tbj_gen_zipf_fk64(uint64_t seed, uint64_t n, const tbj_tuple64* R, uint64_t nR, double z,
                       tbj_tuple64* out, int nthreads) {
     if ((!out && n) || !R || nR == 0) return TBJ_ERR_INVALID;
@@ -303,12 +298,15 @@ int tbj_gen_zipf_fk64(uint64_t seed, uint64_t n, const tbj_tuple64* R, uint64_t 
         s += cdf[i];
         cdf[i] = s;
     }
-    for (uint64_t i = 0; i < nR; i++) cdf[i] /= s;
-    const int T = clamp_threads(nthreads);
-    const uint64_t per = (n + T - 1) / T;
-    parallel_for(T, T, [&](size_t t) {
-        uint64_t x = seed ^ (0xD1B54A32D192ED03ull * (t + 1));
-        const uint64_t a = std::min<uint64_t>(n, t * per), b = std::min<uint64_t>(n, a + per);
+    const double inv = 1.0 / s;
+    parallel_for((nR + chunk - 1) / chunk, nthreads, [&](size_t c) {
+        uint64_t a = c * chunk, b = std::min<uint64_t>(nR, a + chunk);
+        for (uint64_t i = a; i < b; i++) cdf[i] *= inv;
+    });
+    const uint64_t block = 1 << 16;
+    parallel_for((n + block - 1) / block, nthreads, [&](size_t blk) {
+        uint64_t x = seed ^ (0xD1B54A32D192ED03ull * (blk + 1));
+        const uint64_t a = blk * block, b = std::min<uint64_t>(n, a + block);
         for (uint64_t i = a; i < b; i++) {
             const double u = (double)(splitmix64(x) >> 11) * (1.0 / 9007199254740992.0);
             uint64_t pos = (uint64_t)(std::lower_bound(cdf.begin(), cdf.end(), u) - cdf.begin());

@@ -80,6 +80,9 @@ int run_radix_join(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S
                    hipStream_t st);
 int run_pht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
             uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
+int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR, const tbj_tuple64* S, uint64_t nS,
+             tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
+             hipStream_t st);
 int run_cht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
             uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
 

@@ -1,0 +1,123 @@
+// 64-bit-key sort-merge join (16-byte tuples): shapes, device types and the
+// host-side launchers of its kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "tbj_part.h"
+
+namespace tbj {
+
+// Partition passes: a workgroup of kSm64Block threads stages kSm64Tile
+// tuples (32 KiB) in LDS at a time and owns a histogram tile of kSm64Super
+// such stages, so the per-tile histograms stay small next to the data.
+constexpr int kSm64Block = 512;
+constexpr int kSm64Items = 4;
+constexpr int kSm64Tile = kSm64Block * kSm64Items;  // 2048 tuples
+constexpr int kSm64Super = 4;
+constexpr int kSm64Span = kSm64Tile * kSm64Super;   // 8192 tuples per histogram tile
+constexpr int kSm64PassBits = 10;
+constexpr int kSm64MaxBins = 1 << kSm64PassBits;
+// Join: R chunks of at most kSm64Cap tuples are sorted in LDS; a task is one
+// R chunk against at most kSm64SChunk S tuples of the same partition.
+constexpr int kSm64Cap = 2048;
+constexpr int kSm64SItems = 8;
+constexpr int kSm64SChunk = kSm64Block * kSm64SItems;  // 4096
+// Learned model
+constexpr int kSm64Knots = 256;
+constexpr int kSm64Guide = 2048;
+constexpr int kSm64FitMax = 16384;
+// Merge passes for R partitions larger than one chunk
+constexpr int kSm64MergeItems = 8;
+constexpr int kSm64MergeTile = kSm64Block * kSm64MergeItems;
+
+// Piecewise-linear CDF of R's keys: segment i covers keys [x[i], x[i+1])
+// (the last one up to kmax and beyond) and maps key k to partition
+// clamp(base[i] + (k - x[i]) * slope[i], plo[i], phi[i]).  Knot i sits at
+// sample rank r_i with value 2^bits * (r_i + 8) / (m + 15); keys below x[0]
+// continue segment 0's line down to partition 0 and keys above kmax
+// continue the last segment up to the last partition, so the ~n/m keys
+// outside the sample's range spread over the end partitions instead of
+// piling into the first and the last.  The clamps make the map monotone: partitions are key ranges
+// in key order.
+struct Model64 {
+    uint32_t nseg, gshift, ncells, pad;
+    uint64_t kmin, kmax;  // sample key range (cells of the guide table)
+    uint64_t x[kSm64Knots + 1];
+    double base[kSm64Knots];
+    double slope[kSm64Knots];
+    uint32_t plo[kSm64Knots], phi[kSm64Knots];
+    // guide[c]: segment holding key kmin + (c << gshift); guide[ncells] = nseg - 1
+    uint16_t guide[kSm64Guide + 1];
+};
+
+struct Bin64 {
+    uint32_t kind;        // kBinRadix or kBinLearned
+    uint32_t bits;        // total partition bits
+    uint32_t pass_shift;
+    uint32_t pass_mask;
+    const Model64* model;
+    // non-null: keys outside [range[0], range[1]] are dropped (S against R's range)
+    const unsigned long long* range;
+};
+
+struct SmTask {
+    uint32_t r_off, r_len;  // R chunk (at most kSm64Cap tuples)
+    uint32_t s_off, s_len;  // S range of the same partition
+    uint32_t rchunk;        // index of the R chunk
+    uint32_t first;         // 1 for the chunk's first S range (writes the sorted chunk)
+    uint32_t pad[2];
+};
+
+// One kSm64MergeTile slice of the output of a merge pass over an R partition.
+struct MergeBlock {
+    uint32_t seg_off, seg_len, out_begin, pad;
+};
+
+size_t sm64_model_lds(const Bin64& b);
+void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint32_t nseg,
+              uint32_t bits, uint64_t* sample, Model64* model);
+void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
+                     uint32_t nbins, uint32_t* ntiles_dev);
+// mm (optional): receives atomic min / max of the keys of the tiles.
+void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
+               const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
+               uint32_t* hist, const Bin64& b, uint32_t nbins, unsigned long long* mm);
+void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
+                  const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
+                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins);
+void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
+                   uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
+                   uint32_t max_tiles2);
+void sm64_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs, uint32_t outer,
+                 uint32_t inner, uint32_t passes, uint32_t nR, uint32_t nS, uint2* bounds);
+void sm64_task_counts(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t* rcnt,
+                      uint32_t* jcnt, uint32_t* max_r);
+void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const uint32_t* roff,
+                     const uint32_t* joff, SmTask* rtasks, SmTask* jtasks);
+// Sorts every R chunk by key in place (bucket sort in LDS).
+void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
+                 uint32_t nrtasks);
+void sm64_merge(hipStream_t st, const ulonglong2* src, ulonglong2* dst, const MergeBlock* blocks,
+                uint32_t nblocks, uint64_t width);
+void sm64_fill_u64(hipStream_t st, unsigned long long* p, uint64_t n, unsigned long long v);
+// Count.  presorted: R chunks are already sorted in Rs; otherwise each task
+// sorts its chunk in LDS and, materialising, the chunk's first task writes
+// the sorted chunk to Rout (same offsets).
+void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const ulonglong2* Rs,
+                ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
+                const uint32_t* ntasks_dev, unsigned long long* runs,
+                unsigned long long* chunk_vol, unsigned long long* counters);
+// Exclusive scan of a[0, n) (u64) in place by one workgroup; *total = sum.
+void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* total);
+// runs[g] (S count of the key run starting at sorted R position g) becomes
+// the run's first output slot.
+void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
+                      const SmTask* rtasks, uint32_t nrtasks,
+                      const unsigned long long* chunk_off, const unsigned long long* total,
+                      unsigned long long* runs);
+void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
+               const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
+               ulonglong2* pairs, uint64_t cap);
+
+}  // namespace tbj

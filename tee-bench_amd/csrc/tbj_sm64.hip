@@ -1,0 +1,1059 @@
+// 64-bit keys, 16-byte tuples {u64 key, u64 payload}: partitioned sort-merge
+// join whose materialised output is ordered by key.
+//
+// The reference hard-wires 8-byte tuples (Include/data-types.h:32-47), so
+// this path has no reference counterpart; it follows the structure of RSM
+// (partition both inputs, then sort-merge each partition pair:
+// Joins/radix_sortmerge_join.c:74-105) and the sort-then-merge of PSM
+// (Joins/psm/parallel_sortmerge_join.h:18-111), at 16-byte tuples:
+//   1. partition R and S in one or two passes (radix on the low key bits, or
+//      the learned range model fitted on an R sample, whose partitions are
+//      key ranges in key order);
+//   2. count: each task sorts its R chunk (at most kSm64Cap tuples) by key
+//      in LDS (bucket sort; the chunk's first task also writes the sorted
+//      chunk out), then every S tuple finds its R key run by binary search;
+//      per run the matching S tuples are counted (one global counter per
+//      sorted R position) and per chunk, the pairs.  R partitions above one
+//      chunk are instead sorted beforehand (chunks, then merge passes);
+//   3. a scan of the per-chunk pair counts gives each chunk's output offset,
+//      and one pass per chunk turns the run counts into run output offsets
+//      (scan of count x run length in key order);
+//   4. emit: each S tuple claims a slot block of run-length pairs in its run
+//      and writes them.
+// So pairs of one key are contiguous, and with the learned partitioner the
+// whole output is in key order (radix: in key order inside each partition).
+#include "tbj_device.h"
+#include "tbj_kernels.h"
+#include "tbj_sm64.h"
+
+namespace tbj {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Increments ctr[idx] for each active lane with idx != kNone and returns
+// the value before the lane's own increment (its rank).  Lanes sharing the
+// idx of the wave's first such lane are served by one atomic; every active
+// lane of the wave must call it.
+__device__ __forceinline__ uint32_t agg_inc(uint32_t* ctr, uint32_t idx) {
+    const uint64_t any = __ballot(idx != kNone);
+    if (any == 0) return 0;
+    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
+    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
+    const uint64_t same = __ballot(idx == lead);
+    const uint32_t lane = lane_id();
+    uint32_t r = 0;
+    if (idx == lead) {
+        if (lane == fl) r = atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
+    } else if (idx != kNone) {
+        r = atomicAdd(&ctr[idx], 1u);
+    }
+    const uint32_t b = __shfl(r, fl, TBJ_WAVE);
+    return idx == lead ? b + (uint32_t)__popcll(same & ((1ull << lane) - 1)) : r;
+}
+
+// agg_inc without the rank.
+__device__ __forceinline__ void agg_add(uint32_t* ctr, uint32_t idx) {
+    const uint64_t any = __ballot(idx != kNone);
+    if (any == 0) return;
+    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
+    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
+    const uint64_t same = __ballot(idx == lead);
+    if (idx == lead) {
+        if (lane_id() == fl) atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
+    } else if (idx != kNone) {
+        atomicAdd(&ctr[idx], 1u);
+    }
+}
+
+__device__ __forceinline__ bool tuple_less(const ulonglong2& a, const ulonglong2& b) {
+    return a.x < b.x || (a.x == b.x && a.y < b.y);
+}
+
+// Exclusive prefix sum of 64-bit values across a block (BLOCK / 64 + 1 words
+// of scratch); two barriers, every thread must call it.
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_exclusive_scan_u64(
+    unsigned long long v, unsigned long long* scratch, unsigned long long* total) {
+    constexpr int NW = BLOCK / TBJ_WAVE;
+    const uint32_t lane = lane_id(), wid = wave_id();
+    unsigned long long inc = v;
+#pragma unroll
+    for (int d = 1; d < TBJ_WAVE; d <<= 1) {
+        const unsigned long long y = __shfl_up(inc, d, TBJ_WAVE);
+        if (lane >= (uint32_t)d) inc += y;
+    }
+    if (lane == TBJ_WAVE - 1) scratch[wid] = inc;
+    __syncthreads();
+    if (wid == 0) {
+        const unsigned long long w = lane < NW ? scratch[lane] : 0;
+        unsigned long long wi = w;
+#pragma unroll
+        for (int d = 1; d < NW; d <<= 1) {
+            const unsigned long long y = __shfl_up(wi, d, TBJ_WAVE);
+            if (lane >= (uint32_t)d) wi += y;
+        }
+        if (lane < NW) scratch[lane] = wi - w;
+        if (lane == NW - 1) scratch[NW] = wi;
+    }
+    __syncthreads();
+    *total = scratch[NW];
+    return scratch[wid] + inc - v;
+}
+
+// ---------------------------------------------------------------------------
+// Partition function
+
+struct LdsModel64 {
+    uint64_t x[kSm64Knots + 1];
+    double base[kSm64Knots];
+    double slope[kSm64Knots];
+    uint32_t plo[kSm64Knots], phi[kSm64Knots];
+    uint16_t guide[kSm64Guide + 1];
+};
+
+struct Bin64Fn {
+    uint32_t kind, pass_shift, pass_mask, radix_mask;
+    const LdsModel64* lm;
+    uint32_t nseg, gshift, ncells;
+    uint64_t kmin, lo, hi;
+    bool drop;
+
+    __device__ __forceinline__ uint32_t full(uint64_t key) const {
+        if (kind == kBinRadix) return (uint32_t)key & radix_mask;
+        if (key < lm->x[0]) {  // below the sample: segment 0's line, down to 0
+            const double v = lm->base[0] - (double)(lm->x[0] - key) * lm->slope[0];
+            return min(v <= 0.0 ? 0u : (uint32_t)v, lm->phi[0]);
+        }
+        uint64_t c = (key - kmin) >> gshift;
+        if (c >= ncells) c = ncells - 1;
+        uint32_t a = lm->guide[c], b = (uint32_t)lm->guide[c + 1] + 1;
+        if (b > nseg) b = nseg;
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (lm->x[mid] <= key) a = mid;
+            else b = mid;
+        }
+        const double v = fma((double)(key - lm->x[a]), lm->slope[a], lm->base[a]);
+        uint32_t p = v <= 0.0 ? 0u : (v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v);
+        p = max(p, lm->plo[a]);
+        return min(p, lm->phi[a]);
+    }
+    __device__ __forceinline__ uint32_t operator()(uint64_t key) const {
+        if (drop && (key < lo || key > hi)) return kNone;
+        return (full(key) >> pass_shift) & pass_mask;
+    }
+};
+
+// Loads the model into LDS (no barrier: the caller's next barrier covers it).
+__device__ __forceinline__ Bin64Fn make_bin64(const Bin64& b, LdsModel64* lm) {
+    Bin64Fn f;
+    f.kind = b.kind;
+    f.pass_shift = b.pass_shift;
+    f.pass_mask = b.pass_mask;
+    f.radix_mask = b.bits >= 32 ? 0xFFFFFFFFu : (1u << b.bits) - 1;
+    f.lm = lm;
+    f.nseg = f.gshift = f.ncells = 0;
+    f.kmin = 0;
+    f.drop = b.range != nullptr;
+    f.lo = f.drop ? b.range[0] : 0;
+    f.hi = f.drop ? b.range[1] : ~0ull;
+    if (b.kind == kBinLearned) {
+        const Model64* m = b.model;
+        for (uint32_t i = threadIdx.x; i <= m->nseg; i += blockDim.x) {
+            lm->x[i] = m->x[i];
+            if (i < m->nseg) {
+                lm->base[i] = m->base[i];
+                lm->slope[i] = m->slope[i];
+                lm->plo[i] = m->plo[i];
+                lm->phi[i] = m->phi[i];
+            }
+        }
+        for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->guide[i] = m->guide[i];
+        f.nseg = m->nseg;
+        f.gshift = m->gshift;
+        f.ncells = m->ncells;
+        f.kmin = m->kmin;
+    }
+    return f;
+}
+
+// ---------------------------------------------------------------------------
+// Learned model: strided sample of R, then one workgroup sorts it and places
+// equal-mass knots.
+
+__global__ void k_sample64(const ulonglong2* __restrict__ in, uint64_t n, uint32_t m,
+                           uint64_t* __restrict__ sample) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t stride = n / m;  // m <= n
+    uint64_t pos = (uint64_t)i * stride + ((uint64_t)i * 2654435761u) % stride;
+    if (pos >= n) pos = n - 1;
+    sample[i] = in[pos].x;
+}
+
+__global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sample, uint32_t m,
+                                                uint32_t nseg_req, uint32_t bits,
+                                                Model64* __restrict__ model) {
+    __shared__ uint64_t a[kSm64FitMax];
+    __shared__ uint64_t knots[kSm64Knots];
+    __shared__ uint32_t krank[kSm64Knots];
+    __shared__ uint32_t nk_s;
+    for (uint32_t i = threadIdx.x; i < m; i += 1024) a[i] = sample[i];
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += 1024) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const bool up = (i & k) == 0;
+                    const uint64_t x = a[i], y = a[ixj];
+                    if ((x > y) == up) {
+                        a[i] = y;
+                        a[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint32_t nk = 1;
+        knots[0] = a[0];
+        krank[0] = 0;
+        for (uint32_t j = 1; j < nseg_req; j++) {
+            const uint32_t idx = (uint32_t)((uint64_t)j * m / nseg_req);
+            if (a[idx] > knots[nk - 1]) {
+                knots[nk] = a[idx];
+                krank[nk] = idx;
+                nk++;
+            }
+        }
+        nk_s = nk;
+    }
+    __syncthreads();
+    const uint32_t nseg = nk_s;
+    const uint64_t kmin = a[0], kmax = a[m - 1];
+    const double parts = (double)(1ull << bits);
+    const uint32_t last = (uint32_t)((1ull << bits) - 1);
+    for (uint32_t i = threadIdx.x; i < nseg; i += 1024) {
+        // knot at sample rank r: 2^bits * (r + 8) / (m + 15).  The R keys
+        // beyond the sample's extremes number about n / (m + 1) per side,
+        // with an exponential tail; each side keeps 8 times that room.  The
+        // last segment ends at the sample maximum and continues beyond it.
+        const double y0 = parts * (krank[i] + 8.0) / (m + 15.0);
+        const double y1 = parts * ((i + 1 < nseg ? krank[i + 1] : m - 1) + 8.0) / (m + 15.0);
+        const uint64_t x0 = knots[i];
+        const uint64_t x1 = i + 1 < nseg ? knots[i + 1] : kmax;
+        const double dx = x1 > x0 ? (double)(x1 - x0) : 1.0;
+        model->x[i] = x0;
+        model->base[i] = y0;
+        model->slope[i] = (y1 - y0) / dx;
+        const uint32_t lo = i == 0 ? 0u : min((uint32_t)floor(y0), last);
+        uint32_t hi = (uint32_t)ceil(y1);
+        hi = hi > 0 ? hi - 1 : 0;
+        model->plo[i] = lo;
+        model->phi[i] = i + 1 < nseg ? min(max(hi, lo), last) : last;
+    }
+    if (threadIdx.x == 0) model->x[nseg] = kmax;
+    uint32_t gshift = 0;
+    while (((kmax - kmin) >> gshift) >= (uint64_t)kSm64Guide) gshift++;
+    const uint32_t ncells = (uint32_t)(((kmax - kmin) >> gshift) + 1);
+    for (uint32_t c = threadIdx.x; c <= ncells; c += 1024) {
+        uint32_t g = nseg - 1;
+        if (c < ncells) {
+            const uint64_t key = kmin + ((uint64_t)c << gshift);
+            uint32_t lo = 0, hi = nseg;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (knots[mid] <= key) lo = mid;
+                else hi = mid;
+            }
+            g = lo;
+        }
+        model->guide[c] = (uint16_t)g;
+    }
+    if (threadIdx.x == 0) {
+        model->nseg = nseg;
+        model->kmin = kmin;
+        model->kmax = kmax;
+        model->gshift = gshift;
+        model->ncells = ncells;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Partition passes: per histogram tile, a bin-major count; one device scan;
+// then each tile scatters its stages through LDS.  First pass: segment 0 is
+// R and segment 1 is S; second pass: one segment per (relation, first-pass
+// bin).
+
+__global__ __launch_bounds__(kSm64Block) void k_hist64(
+    const ulonglong2* __restrict__ in, const TileDesc* __restrict__ tiles,
+    const SegDesc* __restrict__ segs, const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base,
+    uint32_t* __restrict__ hist, Bin64 b, uint32_t nbins, unsigned long long* __restrict__ mm) {
+    __shared__ uint32_t cnt[kSm64MaxBins];
+    __shared__ unsigned long long bmm[2];
+    extern __shared__ __align__(16) uint8_t dyn_lds[];
+    const uint32_t t = tile_base + blockIdx.x;
+    if (t >= *ntiles_dev) return;
+    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) cnt[i] = 0;
+    if (threadIdx.x == 0) {
+        bmm[0] = ~0ull;
+        bmm[1] = 0;
+    }
+    __syncthreads();
+    const TileDesc td = tiles[t];
+    const uint64_t* keys = reinterpret_cast<const uint64_t*>(in + td.in_off);
+    uint64_t mn = ~0ull, mx = 0;
+    for (uint32_t j = threadIdx.x; j < td.len; j += kSm64Block) {
+        const uint64_t k = keys[2 * (uint64_t)j];
+        mn = min(mn, k);
+        mx = max(mx, k);
+        agg_add(cnt, bin(k));
+    }
+    if (mm) {  // tile min / max: waves -> LDS -> one global atomic pair
+        for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+            mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, TBJ_WAVE));
+            mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, TBJ_WAVE));
+        }
+        if (lane_id() == 0) {
+            atomicMin(&bmm[0], (unsigned long long)mn);
+            atomicMax(&bmm[1], (unsigned long long)mx);
+        }
+    }
+    __syncthreads();
+    if (mm && threadIdx.x == 0) {
+        atomicMin(&mm[0], bmm[0]);
+        atomicMax(&mm[1], bmm[1]);
+    }
+    const SegDesc sd = segs[td.seg];
+    const uint32_t tis = t - sd.first_tile;
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
+        hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
+}
+
+__global__ __launch_bounds__(kSm64Block) void k_scatter64(
+    const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out,
+    const TileDesc* __restrict__ tiles, const SegDesc* __restrict__ segs,
+    const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base,
+    const uint32_t* __restrict__ scanned, Bin64 b, uint32_t nbins) {
+    __shared__ ulonglong2 stage[kSm64Tile];
+    __shared__ uint16_t sbin[kSm64Tile];
+    __shared__ uint32_t loff[kSm64MaxBins];
+    __shared__ uint32_t goff[kSm64MaxBins];
+    __shared__ uint32_t next[kSm64MaxBins];
+    __shared__ uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
+    extern __shared__ __align__(16) uint8_t dyn_lds[];
+    const uint32_t t = tile_base + blockIdx.x;
+    if (t >= *ntiles_dev) return;
+    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
+    const TileDesc td = tiles[t];
+    const SegDesc sd = segs[td.seg];
+    const uint32_t tis = t - sd.first_tile;
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
+        next[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis];
+    for (uint32_t s0 = 0; s0 < td.len; s0 += kSm64Tile) {
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) loff[i] = 0;
+        __syncthreads();
+        ulonglong2 v[kSm64Items];
+        uint32_t bb[kSm64Items], rank[kSm64Items];
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64Items; k++) {
+            const uint32_t j = s0 + k * kSm64Block + threadIdx.x;
+            bb[k] = kNone;
+            if (j < td.len) {
+                v[k] = in[td.in_off + j];
+                bb[k] = bin(v[k].x);
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64Items; k++) rank[k] = agg_inc(loff, bb[k]);
+        __syncthreads();
+        const uint32_t nvalid = block_scan_lds<kSm64Block>(loff, nbins, scratch);
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) {
+            const uint32_t end = i + 1 < nbins ? loff[i + 1] : nvalid;
+            goff[i] = next[i] - loff[i];
+            next[i] += end - loff[i];
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64Items; k++) {
+            if (bb[k] != kNone) {
+                const uint32_t slot = loff[bb[k]] + rank[k];
+                stage[slot] = v[k];
+                sbin[slot] = (uint16_t)bb[k];
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < nvalid; j += kSm64Block) out[goff[sbin[j]] + j] = stage[j];
+        __syncthreads();
+    }
+}
+
+__global__ void k_make_tiles64(TileDesc* __restrict__ tiles, SegDesc* __restrict__ segs,
+                               uint32_t nR, uint32_t nS, uint32_t nbins,
+                               uint32_t* __restrict__ ntiles_dev) {
+    const uint32_t tR = (nR + kSm64Span - 1) / kSm64Span, tS = (nS + kSm64Span - 1) / kSm64Span;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < tR + tS) {
+        const bool r = i < tR;
+        const uint32_t k = r ? i : i - tR, n = r ? nR : nS;
+        TileDesc d;
+        d.in_off = k * kSm64Span;
+        d.len = min((uint32_t)kSm64Span, n - k * kSm64Span);
+        d.seg = r ? 0 : 1;
+        d.pad = 0;
+        tiles[i] = d;
+    }
+    if (i == 0) {
+        SegDesc a{}, c{};
+        a.n = nR;
+        a.ntiles = tR;
+        c.hist_base = (uint64_t)nbins * tR;
+        c.n = nS;
+        c.ntiles = tS;
+        c.first_tile = tR;
+        segs[0] = a;
+        segs[1] = c;
+        *ntiles_dev = tR + tS;
+    }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_make_segments64(
+    const uint32_t* __restrict__ scanned1, const SegDesc* __restrict__ segs1, uint32_t nbins1,
+    uint32_t nbins2, SegDesc* __restrict__ segs2, uint32_t* __restrict__ ntiles2_dev) {
+    __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t carry_s;
+    const uint32_t nseg2 = 2 * nbins1;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nseg2; base += kScanBlock) {
+        const uint32_t s2 = base + threadIdx.x;
+        uint32_t st = 0, n = 0, nt = 0;
+        if (s2 < nseg2) {
+            const SegDesc sd = segs1[s2 / nbins1];
+            const uint32_t bb = s2 % nbins1;
+            st = scanned1[sd.hist_base + (uint64_t)bb * sd.ntiles];
+            n = scanned1[sd.hist_base + (uint64_t)(bb + 1) * sd.ntiles] - st;
+            nt = (n + kSm64Span - 1) / kSm64Span;
+        }
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<kScanBlock>(nt, scratch, &total);
+        if (s2 < nseg2) {
+            SegDesc d;
+            d.in_off = st;
+            d.n = n;
+            d.ntiles = nt;
+            d.first_tile = carry_s + ex;
+            d.hist_base = (uint64_t)d.first_tile * nbins2;
+            segs2[s2] = d;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) carry_s += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *ntiles2_dev = carry_s;
+}
+
+__global__ void k_make_tiles64_2(const SegDesc* __restrict__ segs2, uint32_t nseg2,
+                                 const uint32_t* __restrict__ ntiles2_dev,
+                                 TileDesc* __restrict__ tiles2) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *ntiles2_dev) return;
+    // last segment whose first tile is <= t; step back over empty segments
+    // sharing that first tile (the segment owning tile t has tiles)
+    uint32_t lo = 0, hi = nseg2;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (segs2[mid].first_tile <= t) lo = mid;
+        else hi = mid;
+    }
+    while (lo > 0 && (segs2[lo].ntiles == 0 || t >= segs2[lo].first_tile + segs2[lo].ntiles)) lo--;
+    const SegDesc sd = segs2[lo];
+    const uint32_t k = t - sd.first_tile;
+    TileDesc td;
+    td.in_off = sd.in_off + k * kSm64Span;
+    td.len = min((uint32_t)kSm64Span, sd.n - k * kSm64Span);
+    td.seg = lo;
+    td.pad = 0;
+    tiles2[t] = td;
+}
+
+// Final partition bounds: bounds[rel * nparts + p] = {start, end}.
+__global__ void k_bounds64(const uint32_t* __restrict__ scanned, const SegDesc* __restrict__ segs,
+                           uint32_t outer, uint32_t inner, uint32_t passes, uint32_t nR,
+                           uint32_t nS, uint2* __restrict__ bounds) {
+    const uint32_t nparts = outer * inner;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * nparts) return;
+    const uint32_t rel = i / nparts, p = i % nparts;
+    if (passes == 0) {
+        bounds[i] = rel ? make_uint2(nR, nR + nS) : make_uint2(0, nR);
+        return;
+    }
+    const uint32_t seg = passes == 1 ? rel : rel * outer + p / inner;
+    const uint32_t bin = passes == 1 ? p : p % inner;
+    const SegDesc sd = segs[seg];
+    bounds[i] = make_uint2(scanned[sd.hist_base + (uint64_t)bin * sd.ntiles],
+                           scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ntiles]);
+}
+
+// Per partition: R chunks and join tasks (R chunk x S chunk); word nparts is
+// zero (scan sentinel); max_r receives the largest R partition.
+__global__ void k_task_counts64(const uint2* __restrict__ bounds, uint32_t nparts,
+                                uint32_t* __restrict__ rcnt, uint32_t* __restrict__ jcnt,
+                                uint32_t* __restrict__ max_r) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t nr = 0;
+    if (p < nparts) {
+        const uint2 r = bounds[p], s = bounds[nparts + p];
+        nr = r.y - r.x;
+        const uint32_t ns = s.y - s.x;
+        const uint32_t rc = (nr + kSm64Cap - 1) / kSm64Cap;
+        rcnt[p] = rc;
+        jcnt[p] = ns ? rc * ((ns + kSm64SChunk - 1) / kSm64SChunk) : 0;
+    } else if (p == nparts) {
+        rcnt[p] = 0;
+        jcnt[p] = 0;
+    }
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) nr = max(nr, (uint32_t)__shfl_xor(nr, d, TBJ_WAVE));
+    if (lane_id() == 0 && nr) atomicMax(max_r, nr);
+}
+
+__global__ void k_fill_tasks64(const uint2* __restrict__ bounds, uint32_t nparts,
+                               const uint32_t* __restrict__ roff, const uint32_t* __restrict__ joff,
+                               SmTask* __restrict__ rtasks, SmTask* __restrict__ jtasks) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparts) return;
+    const uint2 r = bounds[p], s = bounds[nparts + p];
+    uint32_t ro = roff[p], jo = joff[p];
+    for (uint32_t a = r.x; a < r.y; a += kSm64Cap, ro++) {
+        SmTask t{};
+        t.r_off = a;
+        t.r_len = min((uint32_t)kSm64Cap, r.y - a);
+        t.rchunk = ro;
+        rtasks[ro] = t;
+        for (uint32_t c = s.x; c < s.y; c += kSm64SChunk) {
+            SmTask j = t;
+            j.first = c == s.x;
+            j.s_off = c;
+            j.s_len = min((uint32_t)kSm64SChunk, s.y - c);
+            jtasks[jo++] = j;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R chunk sort in LDS.  Buckets: n equal-width key ranges between the
+// chunk's min and max key (a monotone map, so bucket order is key order);
+// each bucket is then insertion-sorted by one thread.  A bucket above
+// kBucketMax tuples (clustered keys) sends the whole chunk to a bitonic sort
+// instead; a chunk of one key is sorted as it stands.
+
+constexpr uint32_t kSortItems = kSm64Cap / kSm64Block;
+constexpr uint32_t kBucketMax = 32;
+
+struct SortLds {
+    ulonglong2 a[kSm64Cap];
+    uint32_t cnt[kSm64Cap];
+    uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
+    unsigned long long mm[2];
+    uint32_t big;
+};
+
+// v: item k of thread t is tuple k * kSm64Block + t (valid below n).  Leaves
+// the chunk sorted by key in L.a[0, n); every thread must call it; ends with
+// a barrier.
+__device__ void chunk_sort(const ulonglong2 (&v)[kSortItems], uint32_t n, SortLds& L) {
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        L.mm[0] = ~0ull;
+        L.mm[1] = 0;
+        L.big = 0;
+    }
+    for (uint32_t i = tid; i < n; i += kSm64Block) L.cnt[i] = 0;
+    uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        if (k * kSm64Block + tid < n) {
+            mn = min(mn, (uint64_t)v[k].x);
+            mx = max(mx, (uint64_t)v[k].x);
+        }
+    }
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+        mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, TBJ_WAVE));
+        mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, TBJ_WAVE));
+    }
+    __syncthreads();
+    if (lane_id() == 0) {
+        atomicMin(&L.mm[0], (unsigned long long)mn);
+        atomicMax(&L.mm[1], (unsigned long long)mx);
+    }
+    __syncthreads();
+    const uint64_t lo = L.mm[0], span = L.mm[1] - L.mm[0];
+    if (span == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < kSortItems; k++)
+            if (k * kSm64Block + tid < n) L.a[k * kSm64Block + tid] = v[k];
+        __syncthreads();
+        return;
+    }
+    const double scale = (double)n / ((double)span + 1.0);
+    uint32_t bk[kSortItems], rk[kSortItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        if (k * kSm64Block + tid < n) {
+            bk[k] = min(n - 1, (uint32_t)((double)(v[k].x - lo) * scale));
+            rk[k] = atomicAdd(&L.cnt[bk[k]], 1u);
+        }
+    }
+    __syncthreads();
+    block_scan_lds<kSm64Block>(L.cnt, n, L.scratch);
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++)
+        if (k * kSm64Block + tid < n) L.a[L.cnt[bk[k]] + rk[k]] = v[k];
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kSm64Block) {
+        const uint32_t s0 = L.cnt[i], s1 = i + 1 < n ? L.cnt[i + 1] : n;
+        if (s1 - s0 > kBucketMax) {
+            L.big = 1;
+        } else {
+            for (uint32_t x = s0 + 1; x < s1; x++) {
+                const ulonglong2 t = L.a[x];
+                uint32_t y = x;
+                while (y > s0 && L.a[y - 1].x > t.x) {
+                    L.a[y] = L.a[y - 1];
+                    y--;
+                }
+                L.a[y] = t;
+            }
+        }
+    }
+    __syncthreads();
+    if (L.big) {
+        const uint32_t np2 = next_pow2_u32(n);
+        for (uint32_t j = n + tid; j < np2; j += kSm64Block) L.a[j] = make_ulonglong2(~0ull, ~0ull);
+        __syncthreads();
+        for (uint32_t k = 2; k <= np2; k <<= 1) {
+            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+                for (uint32_t i = tid; i < np2; i += kSm64Block) {
+                    const uint32_t ixj = i ^ j;
+                    if (ixj > i) {
+                        const bool up = (i & k) == 0;
+                        const ulonglong2 x = L.a[i], y = L.a[ixj];
+                        if (tuple_less(y, x) == up) {
+                            L.a[i] = y;
+                            L.a[ixj] = x;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void load_chunk(const ulonglong2* __restrict__ src, uint32_t n,
+                                           ulonglong2 (&v)[kSortItems]) {
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        const uint32_t j = k * kSm64Block + threadIdx.x;
+        if (j < n) v[k] = src[j];
+    }
+}
+
+// Sorts every R chunk in place (partitions above one chunk, before merging).
+__global__ __launch_bounds__(kSm64Block) void k_sort_r64(ulonglong2* __restrict__ Rb,
+                                                         const SmTask* __restrict__ rtasks,
+                                                         uint32_t nrtasks) {
+    __shared__ SortLds L;
+    for (uint32_t t = blockIdx.x; t < nrtasks; t += gridDim.x) {
+        const SmTask tk = rtasks[t];
+        ulonglong2 v[kSortItems];
+        load_chunk(Rb + tk.r_off, tk.r_len, v);
+        chunk_sort(v, tk.r_len, L);
+        for (uint32_t j = threadIdx.x; j < tk.r_len; j += kSm64Block) Rb[tk.r_off + j] = L.a[j];
+        __syncthreads();
+    }
+}
+
+// One merge pass over R partitions longer than a chunk: sorted runs of
+// `width` tuples are merged pairwise (merge path: each thread finds how
+// many of its first output's predecessors come from the left run, then
+// merges kSm64MergeItems outputs).  A run pair whose right run is empty is
+// copied.
+__global__ __launch_bounds__(kSm64Block) void k_merge64(const ulonglong2* __restrict__ src,
+                                                        ulonglong2* __restrict__ dst,
+                                                        const MergeBlock* __restrict__ blocks,
+                                                        uint64_t width) {
+    const MergeBlock mb = blocks[blockIdx.x];
+    const uint64_t o = (uint64_t)mb.out_begin + (uint64_t)threadIdx.x * kSm64MergeItems;
+    const uint64_t len = mb.seg_len;
+    if (o >= len) return;
+    const ulonglong2* s = src + mb.seg_off;
+    ulonglong2* d = dst + mb.seg_off;
+    const uint64_t q0 = o / (2 * width) * (2 * width);
+    const uint64_t na = min(width, len - q0);
+    const uint64_t b0 = q0 + na;
+    const uint64_t nb = b0 < len ? min(width, len - b0) : 0;
+    const ulonglong2* A = s + q0;
+    const ulonglong2* B = s + b0;
+    const uint64_t diag = o - q0;
+    uint64_t lo = diag > nb ? diag - nb : 0, hi = min(diag, na);
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (!tuple_less(B[diag - 1 - mid], A[mid])) lo = mid + 1;
+        else hi = mid;
+    }
+    uint64_t i = lo, j = diag - lo;
+    for (uint32_t e = 0; e < kSm64MergeItems && i + j < na + nb; e++) {
+        const bool take_a = j >= nb || (i < na && !tuple_less(B[j], A[i]));
+        d[q0 + i + j] = take_a ? A[i] : B[j];
+        if (take_a) i++;
+        else j++;
+    }
+}
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// First sorted position with key >= k in keys[0, n).
+__device__ __forceinline__ uint32_t lower64(const uint64_t* keys, uint32_t n, uint64_t k) {
+    uint32_t L = 0, H = n;
+    while (L < H) {
+        const uint32_t m = (L + H) >> 1;
+        if (keys[m] < k) L = m + 1;
+        else H = m;
+    }
+    return L;
+}
+
+__device__ __forceinline__ uint32_t lower_tuple(const ulonglong2* a, uint32_t n, uint64_t k) {
+    uint32_t L = 0, H = n;
+    while (L < H) {
+        const uint32_t m = (L + H) >> 1;
+        if (a[m].x < k) L = m + 1;
+        else H = m;
+    }
+    return L;
+}
+
+// Length of the key run starting at sorted position j.
+__device__ __forceinline__ uint32_t run_len(const uint64_t* keys, uint32_t n, uint32_t j) {
+    uint32_t len = 1;
+    while (j + len < n && keys[j + len] == keys[j]) len++;
+    return len;
+}
+
+// Count.  counters[0] += matches, counters[1] += sum of (R.payload +
+// S.payload).  MAT: runs[g] += S tuples matching the key run starting at
+// sorted R position g; chunk_vol[rchunk] += pairs of the task.
+template <bool MAT, bool PRESORTED>
+__global__ __launch_bounds__(kSm64Block) void k_count64(
+    const ulonglong2* __restrict__ Rs, ulonglong2* __restrict__ Rout,
+    const ulonglong2* __restrict__ Sb, const SmTask* __restrict__ tasks,
+    const uint32_t* __restrict__ ntasks_dev, unsigned long long* __restrict__ runs,
+    unsigned long long* __restrict__ chunk_vol, unsigned long long* __restrict__ counters) {
+    __shared__ SortLds L;
+    __shared__ unsigned long long tm;
+    __shared__ unsigned long long red_m[kSm64Block / TBJ_WAVE], red_c[kSm64Block / TBJ_WAVE];
+    uint64_t matches = 0, cks = 0;
+    const uint32_t ntasks = *ntasks_dev;
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const SmTask tk = tasks[t];
+        const uint32_t n = tk.r_len;
+        if (PRESORTED) {
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.a[j] = Rs[tk.r_off + j];
+            __syncthreads();
+        } else {
+            ulonglong2 v[kSortItems];
+            load_chunk(Rs + tk.r_off, n, v);
+            chunk_sort(v, n, L);
+            if (MAT && tk.first)
+                for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) Rout[tk.r_off + j] = L.a[j];
+        }
+        if (MAT)
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.cnt[j] = 0;
+        if (threadIdx.x == 0) tm = 0;
+        __syncthreads();
+        uint64_t mt = 0;
+        for (uint32_t j = threadIdx.x; j < tk.s_len; j += kSm64Block) {
+            const ulonglong2 s = Sb[tk.s_off + j];
+            const uint32_t lo = lower_tuple(L.a, n, s.x);
+            const bool hit = lo < n && L.a[lo].x == s.x;
+            if (hit) {
+                for (uint32_t e = lo; e < n && L.a[e].x == s.x; e++) {
+                    mt++;
+                    cks += L.a[e].y + s.y;
+                }
+            }
+            if (MAT) agg_add(L.cnt, hit ? lo : kNone);
+        }
+        matches += mt;
+        if (MAT) {
+            mt = wave_reduce_u64(mt);
+            if (lane_id() == 0 && mt) atomicAdd(&tm, (unsigned long long)mt);
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block)
+                if (L.cnt[j]) atomicAdd(&runs[tk.r_off + j], (unsigned long long)L.cnt[j]);
+            if (threadIdx.x == 0 && tm) atomicAdd(&chunk_vol[tk.rchunk], tm);
+        }
+        __syncthreads();
+    }
+    matches = wave_reduce_u64(matches);
+    cks = wave_reduce_u64(cks);
+    if (lane_id() == 0) {
+        red_m[wave_id()] = matches;
+        red_c[wave_id()] = cks;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0, c = 0;
+        for (int w = 0; w < kSm64Block / TBJ_WAVE; w++) {
+            m += red_m[w];
+            c += red_c[w];
+        }
+        if (m) atomicAdd(&counters[0], m);
+        if (c) atomicAdd(&counters[1], c);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_u64(unsigned long long* __restrict__ a, uint32_t n,
+                                                   unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long scratch[1024 / TBJ_WAVE + 1];
+    const uint32_t per = (n + 1023) / 1024;
+    const uint32_t beg = min(n, threadIdx.x * per), end = min(n, beg + per);
+    unsigned long long s = 0;
+    for (uint32_t i = beg; i < end; i++) s += a[i];
+    unsigned long long tot;
+    unsigned long long run = block_exclusive_scan_u64<1024>(s, scratch, &tot);
+    for (uint32_t i = beg; i < end; i++) {
+        const unsigned long long c = a[i];
+        a[i] = run;
+        run += c;
+    }
+    if (threadIdx.x == 0) *total = tot;
+}
+
+// Per chunk with output: runs[g] = chunk_off + sum of count x run length over
+// the runs before g, for the run starts g with a nonzero count.
+__global__ __launch_bounds__(kSm64Block) void k_run_offsets64(
+    const ulonglong2* __restrict__ Rsorted, const SmTask* __restrict__ rtasks, uint32_t nrtasks,
+    const unsigned long long* __restrict__ chunk_off, const unsigned long long* __restrict__ total,
+    unsigned long long* __restrict__ runs) {
+    __shared__ uint64_t rk[kSm64Cap];
+    __shared__ unsigned long long scratch[kSm64Block / TBJ_WAVE + 1];
+    constexpr uint32_t kPer = kSm64Cap / kSm64Block;
+    for (uint32_t c = blockIdx.x; c < nrtasks; c += gridDim.x) {
+        const unsigned long long base = chunk_off[c];
+        const unsigned long long end = c + 1 < nrtasks ? chunk_off[c + 1] : *total;
+        if (end == base) continue;  // no output: the chunk may be unsorted
+        const SmTask tk = rtasks[c];
+        const uint32_t n = tk.r_len;
+        for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) rk[j] = Rsorted[tk.r_off + j].x;
+        __syncthreads();
+        uint32_t cnt[kPer];
+        unsigned long long vol[kPer], vsum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            const uint32_t j = threadIdx.x * kPer + q;
+            cnt[q] = j < n ? (uint32_t)runs[tk.r_off + j] : 0;
+            vol[q] = cnt[q] ? (unsigned long long)cnt[q] * run_len(rk, n, j) : 0;
+            vsum += vol[q];
+        }
+        unsigned long long tot;
+        unsigned long long run = base + block_exclusive_scan_u64<kSm64Block>(vsum, scratch, &tot);
+#pragma unroll
+        for (uint32_t q = 0; q < kPer; q++) {
+            if (cnt[q]) runs[tk.r_off + threadIdx.x * kPer + q] = run;
+            run += vol[q];
+        }
+        __syncthreads();
+    }
+}
+
+// Emit.  runs[g] holds the next free output slot of the run starting at g;
+// the S tuples of a task claim run-length blocks from it.
+__global__ __launch_bounds__(kSm64Block) void k_emit64(
+    const ulonglong2* __restrict__ Rsorted, const ulonglong2* __restrict__ Sb,
+    const SmTask* __restrict__ tasks, const uint32_t* __restrict__ ntasks_dev,
+    unsigned long long* __restrict__ runs, ulonglong2* __restrict__ pairs, uint64_t cap) {
+    __shared__ uint64_t rk[kSm64Cap];
+    __shared__ uint32_t lcnt[kSm64Cap];
+    __shared__ unsigned long long lbase[kSm64Cap];
+    const uint32_t ntasks = *ntasks_dev;
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const SmTask tk = tasks[t];
+        const uint32_t n = tk.r_len;
+        const ulonglong2* Rc = Rsorted + tk.r_off;
+        for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
+            rk[j] = Rc[j].x;
+            lcnt[j] = 0;
+        }
+        __syncthreads();
+        uint64_t sp[kSm64SItems];
+        uint32_t slo[kSm64SItems], srank[kSm64SItems];
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64SItems; k++) {
+            const uint32_t j = k * kSm64Block + threadIdx.x;
+            slo[k] = kNone;
+            sp[k] = 0;
+            if (j < tk.s_len) {
+                const ulonglong2 s = Sb[tk.s_off + j];
+                const uint32_t lo = lower64(rk, n, s.x);
+                if (lo < n && rk[lo] == s.x) slo[k] = lo;
+                sp[k] = s.y;
+            }
+            srank[k] = agg_inc(lcnt, slo[k]);
+        }
+        __syncthreads();
+        for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
+            const uint32_t c = lcnt[j];
+            if (c)
+                lbase[j] = atomicAdd(&runs[tk.r_off + j],
+                                     (unsigned long long)c * run_len(rk, n, j));
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64SItems; k++) {
+            if (slo[k] != kNone) {
+                const uint32_t lo = slo[k];
+                const uint32_t len = run_len(rk, n, lo);
+                unsigned long long o = lbase[lo] + (unsigned long long)srank[k] * len;
+                for (uint32_t e = 0; e < len; e++, o++)
+                    if (o < cap) pairs[o] = make_ulonglong2(Rc[lo + e].y, sp[k]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+size_t sm64_model_lds(const Bin64& b) { return b.kind == kBinLearned ? sizeof(LdsModel64) : 0; }
+
+void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint32_t nseg,
+              uint32_t bits, uint64_t* sample, Model64* model) {
+    hipLaunchKernelGGL(k_sample64, dim3((m + 255) / 256), dim3(256), 0, st, R, nR, m, sample);
+    hipLaunchKernelGGL(k_fit64, dim3(1), dim3(1024), 0, st, sample, m, nseg, bits, model);
+}
+
+void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
+                     uint32_t nbins, uint32_t* ntiles_dev) {
+    const uint32_t t = (nR + kSm64Span - 1) / kSm64Span + (nS + kSm64Span - 1) / kSm64Span;
+    hipLaunchKernelGGL(k_make_tiles64, dim3(t ? (t + 255) / 256 : 1), dim3(256), 0, st, tiles,
+                       segs, nR, nS, nbins, ntiles_dev);
+}
+
+void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
+               const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
+               uint32_t* hist, const Bin64& b, uint32_t nbins, unsigned long long* mm) {
+    if (!grid) return;
+    hipLaunchKernelGGL(k_hist64, dim3(grid), dim3(kSm64Block), sm64_model_lds(b), st, in, tiles,
+                       segs, ntiles_dev, tile_base, hist, b, nbins, mm);
+}
+
+void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
+                  const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
+                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins) {
+    if (!grid) return;
+    hipLaunchKernelGGL(k_scatter64, dim3(grid), dim3(kSm64Block), sm64_model_lds(b), st, in, out,
+                       tiles, segs, ntiles_dev, tile_base, scanned, b, nbins);
+}
+
+void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
+                   uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
+                   uint32_t max_tiles2) {
+    hipLaunchKernelGGL(k_make_segments64, dim3(1), dim3(kScanBlock), 0, st, scanned1, segs1,
+                       nbins1, nbins2, segs2, ntiles2_dev);
+    hipLaunchKernelGGL(k_make_tiles64_2, dim3((max_tiles2 + 255) / 256), dim3(256), 0, st, segs2,
+                       2 * nbins1, ntiles2_dev, tiles2);
+}
+
+void sm64_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs, uint32_t outer,
+                 uint32_t inner, uint32_t passes, uint32_t nR, uint32_t nS, uint2* bounds) {
+    const uint32_t n = 2 * outer * inner;
+    hipLaunchKernelGGL(k_bounds64, dim3((n + 255) / 256), dim3(256), 0, st, scanned, segs, outer,
+                       inner, passes, nR, nS, bounds);
+}
+
+void sm64_task_counts(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t* rcnt,
+                      uint32_t* jcnt, uint32_t* max_r) {
+    hipLaunchKernelGGL(k_task_counts64, dim3((nparts + 1 + 255) / 256), dim3(256), 0, st, bounds,
+                       nparts, rcnt, jcnt, max_r);
+}
+
+void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const uint32_t* roff,
+                     const uint32_t* joff, SmTask* rtasks, SmTask* jtasks) {
+    hipLaunchKernelGGL(k_fill_tasks64, dim3((nparts + 255) / 256), dim3(256), 0, st, bounds,
+                       nparts, roff, joff, rtasks, jtasks);
+}
+
+void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
+                 uint32_t nrtasks) {
+    if (nrtasks && grid)
+        hipLaunchKernelGGL(k_sort_r64, dim3(grid), dim3(kSm64Block), 0, st, Rb, rtasks, nrtasks);
+}
+
+void sm64_merge(hipStream_t st, const ulonglong2* src, ulonglong2* dst, const MergeBlock* blocks,
+                uint32_t nblocks, uint64_t width) {
+    if (nblocks)
+        hipLaunchKernelGGL(k_merge64, dim3(nblocks), dim3(kSm64Block), 0, st, src, dst, blocks,
+                           width);
+}
+
+void sm64_fill_u64(hipStream_t st, unsigned long long* p, uint64_t n, unsigned long long v) {
+    if (!n) return;
+    uint64_t g = (n + 255) / 256;
+    if (g > 8192) g = 8192;
+    hipLaunchKernelGGL(k_fill_u64, dim3((uint32_t)g), dim3(256), 0, st, p, n, v);
+}
+
+void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const ulonglong2* Rs,
+                ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
+                const uint32_t* ntasks_dev, unsigned long long* runs,
+                unsigned long long* chunk_vol, unsigned long long* counters) {
+    const dim3 g(grid), b(kSm64Block);
+    if (mat && presorted)
+        hipLaunchKernelGGL((k_count64<true, true>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
+                           runs, chunk_vol, counters);
+    else if (mat)
+        hipLaunchKernelGGL((k_count64<true, false>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
+                           runs, chunk_vol, counters);
+    else if (presorted)
+        hipLaunchKernelGGL((k_count64<false, true>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
+                           runs, chunk_vol, counters);
+    else
+        hipLaunchKernelGGL((k_count64<false, false>), g, b, 0, st, Rs, Rout, Sb, tasks,
+                           ntasks_dev, runs, chunk_vol, counters);
+}
+
+void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* total) {
+    hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, st, a, n, total);
+}
+
+void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
+                      const SmTask* rtasks, uint32_t nrtasks,
+                      const unsigned long long* chunk_off, const unsigned long long* total,
+                      unsigned long long* runs) {
+    if (nrtasks)
+        hipLaunchKernelGGL(k_run_offsets64, dim3(grid), dim3(kSm64Block), 0, st, Rsorted, rtasks,
+                           nrtasks, chunk_off, total, runs);
+}
+
+void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
+               const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
+               ulonglong2* pairs, uint64_t cap) {
+    hipLaunchKernelGGL(k_emit64, dim3(grid), dim3(kSm64Block), 0, st, Rsorted, Sb, tasks,
+                       ntasks_dev, runs, pairs, cap);
+}
+
+}  // namespace tbj

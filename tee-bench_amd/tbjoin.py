@@ -154,11 +154,34 @@ def join(algo, R, S, pairs=None, **opts):
 
 
 def join64_ptr(algo, r_ptr, nR, s_ptr, nS, pairs_ptr=0, pairs_cap=0, options=None, stream=0):
+    """64-bit-key sort-merge join (PSM / RSM) of 16-byte tuples at raw HBM pointers."""
     o = options if options is not None else make_options()
     res = Result()
     check(lib().tbj_join64(_algo(algo), r_ptr, nR, s_ptr, nS, pairs_ptr or None, pairs_cap,
                            ctypes.byref(o), ctypes.byref(res), stream or None))
     return res
+
+
+def join64(algo, R, S, pairs=None, **opts):
+    """Joins device tensors of 16-byte tuples (int64/uint64 [n, 2]: key, payload).
+
+    pairs: optional device tensor (int64 [cap, 2]) for the (R.payload,
+    S.payload) matches; materialises when given.  PSM output is in key order.
+    """
+    for t in (R, S):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("relations must be contiguous device tensors")
+    nR = R.numel() * R.element_size() // 16
+    nS = S.numel() * S.element_size() // 16
+    cap = 0
+    if pairs is not None:
+        cap = pairs.numel() * pairs.element_size() // 16
+        opts.setdefault("materialize", True)
+    o = make_options(**opts)
+    import torch
+    stream = torch.cuda.current_stream(R.device).cuda_stream
+    return join64_ptr(algo, R.data_ptr(), nR, S.data_ptr(), nS,
+                      pairs.data_ptr() if pairs is not None else 0, cap, o, stream)
 
 
 # ---------------------------------------------------------------------------

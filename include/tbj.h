@@ -151,6 +151,50 @@ int tbj_join64(int algo,
                tbj_pair64* pairs, uint64_t pairs_cap,
                const tbj_options* opt, tbj_result* res, void* stream);
 
+/*
+ * Batches of joins (no reference counterpart: the reference runs one join
+ * per process, App/native.cpp:121-149).  All relations live in one device
+ * arena; item i names R_i = arena[r_off, r_off + nR) and S_i = arena[s_off,
+ * s_off + nS).  counts (device, n entries) receives each join's match count.
+ * Count-only RHO semantics (algo TBJ_RHO; others: TBJ_ERR_UNSUPPORTED).
+ * The batch is processed in waves of at most 2^31 partitioned tuples; each
+ * item's R and S must lie within one window of 2^32 - 2^20 tuples.
+ *
+ * A plan fixes the launch sequence for the batch's sizes; tbj_batch_run
+ * launches it (no host synchronisation inside) and, after
+ * tbj_batch_capture, replays it as one HIP graph.  The arena contents may
+ * change between runs; the sizes and offsets may not.
+ */
+typedef struct tbj_batch_item { uint64_t r_off, nR, s_off, nS; } tbj_batch_item;
+typedef struct tbj_batch_plan tbj_batch_plan;
+
+typedef struct tbj_batch_tuning {
+    uint32_t target_r;          /* average R tuples per partition; 0 = 2048 */
+    uint32_t one_pass_bits;     /* most partition bits done in one pass; 0 = 8 */
+    uint32_t s_chunk;           /* S tuples per join task; 0 = 8192 */
+    uint32_t wave_tuples;       /* partitioned tuples per wave; 0 = 2^31 */
+} tbj_batch_tuning;
+
+typedef struct tbj_batch_result {
+    uint64_t matches;           /* sum over the batch */
+    uint64_t tuples;            /* sum of nR + nS */
+    uint32_t waves, groups;     /* waves and (wave, radix plan) groups */
+    float ms;                   /* device time of the run */
+} tbj_batch_result;
+
+int tbj_batch_create(int algo, const tbj_tuple* arena, const tbj_batch_item* items, uint32_t n,
+                     uint64_t* counts, const tbj_batch_tuning* tuning, tbj_batch_plan** out);
+/* stream: hipStream_t (NULL = the default stream); synchronous. */
+int tbj_batch_run(tbj_batch_plan* plan, void* stream, tbj_batch_result* res);
+/* Records the plan's launches into a HIP graph (on stream, or on a stream of
+ * the library's own when stream is NULL); later runs replay the graph. */
+int tbj_batch_capture(tbj_batch_plan* plan, void* stream);
+void tbj_batch_destroy(tbj_batch_plan* plan);
+/* Times candidate tunings (target_r, one_pass_bits, s_chunk) on the batch
+ * itself (one warm and `reps` timed runs each) and returns the fastest. */
+int tbj_batch_autotune(int algo, const tbj_tuple* arena, const tbj_batch_item* items, uint32_t n,
+                       uint64_t* counts, int reps, tbj_batch_tuning* best, float* best_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,14 +46,45 @@ int tbj_gen_zipf(uint32_t seed, uint64_t n, uint32_t alphabet, double z, tbj_tup
  * cannot combine -z with -l, native.cpp:107-118). */
 void tbj_remap_odd(tbj_tuple* rel, uint64_t n, uint32_t offset);
 
-/* 16-byte tuples: n unique nonzero random 64-bit keys (splitmix64 stream
- * with rejection of duplicates), payload = row index. */
+/* 16-byte tuples: n unique nonzero random 64-bit keys, payload = row index.
+ * Key i is splitmix64 of counter seed + (i+1)*0x9E3779B97F4A7C15; the
+ * finaliser is a bijection of the counter, so distinct counters give
+ * distinct keys. */
 int tbj_gen_unique64(uint64_t seed, uint64_t n, tbj_tuple64* out, int nthreads);
 /* n probe tuples whose keys are Zipf(z) ranks over R's keys (rank 1 = R[0]),
  * payload = row index.  Ranks are drawn by inverse-CDF lookup over a
  * double-precision Zipf table with a splitmix64 uniform stream. */
 int tbj_gen_zipf_fk64(uint64_t seed, uint64_t n, const tbj_tuple64* R, uint64_t nR, double z,
                       tbj_tuple64* out, int nthreads);
+
+/*
+ * Batches of joins generated on the device (no reference counterpart: the
+ * batch workload has no reference path).  Join i of a batch takes the seed
+ * s_i = splitmix64(base_seed ^ i) and from its splitmix64 stream draws
+ *   u ~ U[0,1): |R_i| = floor(2^(10 + 12u));
+ *   k ~ U{1..16}: |S_i| = k * |R_i|;
+ *   z ~ 0.1 * U{0..15} (so z = 0, a uniform foreign key, has weight 1/16).
+ * R_i: keys 1..|R_i| in a pseudo-random order (a Feistel permutation of the
+ * row index, seeded by s_i).  S_i: keys in 1..|R_i|, uniform for z = 0 and
+ * Zipf(z) ranks otherwise (rejection-inversion sampling, Hormann and
+ * Derflinger, "Rejection-inversion to generate variates from monotone
+ * discrete distributions", 1996).  Payload = row index within the relation.
+ * Every S key is an R key, so join i has exactly |S_i| matches.
+ */
+typedef struct tbj_batch_spec {
+    uint64_t seed;      /* s_i */
+    uint64_t nR, nS;
+    double z;
+    uint64_t r_off, s_off;  /* tuple offsets in the arena */
+} tbj_batch_spec;
+
+/* Fills specs[0..n) for joins of base_seed (offsets: R_0, S_0, R_1, S_1, ...
+ * contiguous from tuple 0); returns the total tuple count. */
+uint64_t tbj_gen_batch_specs(uint64_t base_seed, uint32_t n, tbj_batch_spec* specs);
+/* Writes every join's R and S into the device arena (tuples at the spec
+ * offsets) on the given hipStream_t (NULL = default); asynchronous. */
+int tbj_gen_batch_device(tbj_tuple* arena, const tbj_batch_spec* specs, uint32_t n,
+                         void* stream);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,9 @@ static int init_locked(int device) {
     g_ctx.num_cus = prop.multiProcessorCount;
     for (auto& e : g_ctx.ev)
         if (!e) TBJ_HIP(hipEventCreate(&e));
+    // default priority; used to capture HIP graphs when the caller's stream
+    // is the legacy default stream, which cannot be captured
+    if (!g_ctx.own_stream) TBJ_HIP(hipStreamCreateWithFlags(&g_ctx.own_stream, hipStreamNonBlocking));
     g_ctx.ready = true;
     return TBJ_OK;
 }

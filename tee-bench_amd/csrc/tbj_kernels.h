@@ -53,6 +53,9 @@ void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
 void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1,
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
                           TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2);
+// Tiles of kTile tuples over a segment list (segment first_tile / ntiles set).
+void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
+                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles);
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
 void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nseg,
                 uint32_t bits, const uint32_t* minmax, CdfModel* model);

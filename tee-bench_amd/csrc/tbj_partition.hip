@@ -753,6 +753,13 @@ void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDes
                            nseg1 * nbins1, ntiles2_dev, tiles2);
 }
 
+void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
+                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles) {
+    if (max_tiles)
+        hipLaunchKernelGGL(k_make_tiles2, dim3((max_tiles + 255) / 256), dim3(256), 0, st, segs,
+                           nseg, ntiles_dev, tiles);
+}
+
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample) {
     hipLaunchKernelGGL(k_sample_keys, dim3((m + 255) / 256), dim3(256), 0, st, in, n, m, sample);
 }

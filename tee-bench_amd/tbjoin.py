@@ -275,3 +275,131 @@ def gen_zipf_fk64(seed, n, R, z, nthreads=0):
     check(_gen_lib().tbj_gen_zipf_fk64(seed, n, R.ctypes.data, R.shape[0], z, out.ctypes.data,
                                        nthreads))
     return out
+
+
+# ---------------------------------------------------------------------------
+# Batches of joins (include/tbj.h: tbj_batch_*; include/tbj_gen.h:
+# tbj_gen_batch_*).  All relations of a batch live in one device arena.
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("r_off", ctypes.c_uint64), ("nR", ctypes.c_uint64),
+                ("s_off", ctypes.c_uint64), ("nS", ctypes.c_uint64)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("target_r", ctypes.c_uint32), ("one_pass_bits", ctypes.c_uint32),
+                ("s_chunk", ctypes.c_uint32), ("wave_tuples", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class BatchResult(ctypes.Structure):
+    _fields_ = [("matches", ctypes.c_uint64), ("tuples", ctypes.c_uint64),
+                ("waves", ctypes.c_uint32), ("groups", ctypes.c_uint32), ("ms", ctypes.c_float)]
+
+
+class BatchSpec(ctypes.Structure):
+    _fields_ = [("seed", ctypes.c_uint64), ("nR", ctypes.c_uint64), ("nS", ctypes.c_uint64),
+                ("z", ctypes.c_double), ("r_off", ctypes.c_uint64), ("s_off", ctypes.c_uint64)]
+
+
+_batch_ready = False
+
+
+def _batch_lib():
+    global _batch_ready
+    L = lib()
+    if not _batch_ready:
+        vp, u32, ci = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+        L.tbj_batch_create.argtypes = [ci, vp, ctypes.POINTER(BatchItem), u32, vp,
+                                       ctypes.POINTER(BatchTuning), ctypes.POINTER(vp)]
+        L.tbj_batch_run.argtypes = [vp, vp, ctypes.POINTER(BatchResult)]
+        L.tbj_batch_capture.argtypes = [vp, vp]
+        L.tbj_batch_destroy.argtypes = [vp]
+        L.tbj_batch_destroy.restype = None
+        L.tbj_batch_autotune.argtypes = [ci, vp, ctypes.POINTER(BatchItem), u32, vp, ci,
+                                         ctypes.POINTER(BatchTuning),
+                                         ctypes.POINTER(ctypes.c_float)]
+        L.tbj_gen_batch_specs.argtypes = [ctypes.c_uint64, u32, ctypes.POINTER(BatchSpec)]
+        L.tbj_gen_batch_specs.restype = ctypes.c_uint64
+        L.tbj_gen_batch_device.argtypes = [vp, ctypes.POINTER(BatchSpec), u32, vp]
+        for f in ("tbj_batch_create", "tbj_batch_run", "tbj_batch_capture",
+                  "tbj_batch_autotune", "tbj_gen_batch_device"):
+            getattr(L, f).restype = ctypes.c_int
+        _batch_ready = True
+    return L
+
+
+def batch_items(specs_or_tuples):
+    """BatchItem array from BatchSpecs or (r_off, nR, s_off, nS) tuples."""
+    arr = (BatchItem * len(specs_or_tuples))()
+    for i, x in enumerate(specs_or_tuples):
+        if isinstance(x, BatchSpec):
+            arr[i] = BatchItem(x.r_off, x.nR, x.s_off, x.nS)
+        else:
+            arr[i] = BatchItem(*x)
+    return arr
+
+
+class Batch:
+    """A planned batch of count-only RHO joins over one device arena.
+
+    arena: device tensor holding the tuples (8 bytes each); items: BatchItem
+    array; counts: device int64 tensor with one entry per join.
+    """
+
+    def __init__(self, arena, items, counts, tuning=None, algo="RHO"):
+        L = _batch_lib()
+        self.arena, self.items, self.counts = arena, items, counts
+        self._plan = ctypes.c_void_p()
+        t = ctypes.byref(tuning) if tuning is not None else None
+        check(L.tbj_batch_create(_algo(algo), arena.data_ptr(), items, len(items),
+                                 counts.data_ptr(), t, ctypes.byref(self._plan)))
+
+    def _stream(self):
+        import torch
+        return torch.cuda.current_stream(self.arena.device).cuda_stream
+
+    def capture(self):
+        check(_batch_lib().tbj_batch_capture(self._plan, self._stream()))
+
+    def run(self):
+        res = BatchResult()
+        check(_batch_lib().tbj_batch_run(self._plan, self._stream(), ctypes.byref(res)))
+        return res
+
+    def close(self):
+        if self._plan:
+            _batch_lib().tbj_batch_destroy(self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_autotune(arena, items, counts, reps=2, wave_tuples=0, algo="RHO"):
+    """Times candidate tunings on the batch; returns (BatchTuning, ms)."""
+    best = BatchTuning(0, 0, 0, wave_tuples)
+    ms = ctypes.c_float()
+    check(_batch_lib().tbj_batch_autotune(_algo(algo), arena.data_ptr(), items, len(items),
+                                          counts.data_ptr(), reps, ctypes.byref(best),
+                                          ctypes.byref(ms)))
+    return best, ms.value
+
+
+def gen_batch_specs(base_seed, n):
+    """Sizes, seeds and arena offsets of a generated batch: (specs, total tuples)."""
+    specs = (BatchSpec * n)()
+    total = _batch_lib().tbj_gen_batch_specs(base_seed, n, specs)
+    return specs, total
+
+
+def gen_batch_device(arena, specs):
+    """Writes the batch's relations into the device arena tensor."""
+    import torch
+    stream = torch.cuda.current_stream(arena.device).cuda_stream
+    check(_batch_lib().tbj_gen_batch_device(arena.data_ptr(), specs, len(specs), stream))

@@ -152,3 +152,22 @@ def test_cli_unknown_algorithm_exits_success():
 def test_cli_bad_selectivity_exits_failure():
     p = subprocess.run([APP, "-a", "RHO", "-l", "200"], capture_output=True, text=True)
     assert p.returncode != 0 and "Selectivity should be between 0 and 100" in p.stdout
+
+
+def test_batch_specs_follow_the_batch_recipe(tb):
+    specs, total = tb.gen_batch_specs(0xBA7C4, 4096)
+    nR = np.array([s.nR for s in specs])
+    nS = np.array([s.nS for s in specs])
+    z = np.array([s.z for s in specs])
+    assert nR.min() >= 1024 and nR.max() < (1 << 22)
+    k = nS // nR
+    assert np.all(nS == k * nR) and k.min() == 1 and k.max() == 16
+    assert set(np.round(z * 10).astype(int)) == set(range(16))
+    # log-uniform sizes: log2 |R| roughly uniform on [10, 22)
+    hist = np.histogram(np.log2(nR), bins=12, range=(10, 22))[0]
+    assert hist.min() > 250 and hist.max() < 450
+    off = np.array([[s.r_off, s.s_off] for s in specs])
+    ends = off[:, 1] + nS
+    assert off[0, 0] == 0 and np.all(off[1:, 0] == ends[:-1]) and ends[-1] == total
+    again, _ = tb.gen_batch_specs(0xBA7C4, 4096)
+    assert all(a.seed == b.seed and a.nS == b.nS for a, b in zip(specs, again))

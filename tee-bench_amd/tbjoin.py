@@ -66,6 +66,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # torch ships its own HIP runtime: load it first so that libtbj.so
+        # binds to that same runtime instead of a second copy from /opt/rocm
+        # (two runtimes in one process hide the GPU from one of them)
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         vp, u64 = ctypes.c_void_p, ctypes.c_uint64
         L.tbj_init.argtypes = [ctypes.c_int]

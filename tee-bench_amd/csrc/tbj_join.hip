@@ -78,26 +78,30 @@ __global__ void k_task_counts(const uint2* __restrict__ bounds, uint32_t nparts,
     }
 }
 
-// Larger partitions first (a partition's tasks are contiguous; the task
-// counter hands them out in order).
+// One thread per task: task t lies in the partition p with task_off[p] <= t
+// < task_off[p + 1] (binary search), as R chunk li / nsc and S chunk li %
+// nsc of it (li = t - task_off[p], nsc = S chunks of p).  A partition's
+// tasks are contiguous in partition order.
 __global__ void k_fill_tasks(const uint2* __restrict__ bounds, uint32_t nparts, uint32_t cap,
                              uint32_t chunk, const uint32_t* __restrict__ task_off,
-                             JoinTask* __restrict__ tasks) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nparts) return;
-    const uint2 r = bounds[p], s = bounds[nparts + p];
-    const uint32_t n = tasks_of(r, s, cap, chunk);
-    uint32_t o = task_off[p];
-    for (uint32_t ro = r.x; ro < r.y && n; ro += cap) {
-        for (uint32_t so = s.x; so < s.y; so += chunk) {
-            JoinTask t;
-            t.r_off = ro;
-            t.r_len = min(cap, r.y - ro);
-            t.s_off = so;
-            t.s_len = min(chunk, s.y - so);
-            tasks[o++] = t;
-        }
+                             uint32_t ntasks, JoinTask* __restrict__ tasks) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntasks) return;
+    uint32_t lo = 0, hi = nparts;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (task_off[mid] <= t) lo = mid;
+        else hi = mid;
     }
+    const uint2 r = bounds[lo], s = bounds[nparts + lo];
+    const uint32_t nsc = (s.y - s.x + chunk - 1) / chunk;
+    const uint32_t li = t - task_off[lo];
+    JoinTask tk;
+    tk.r_off = r.x + (li / nsc) * cap;
+    tk.r_len = min(cap, r.y - tk.r_off);
+    tk.s_off = s.x + (li % nsc) * chunk;
+    tk.s_len = min(chunk, s.y - tk.s_off);
+    tasks[t] = tk;
 }
 
 // ---------------------------------------------------------------------------
@@ -473,9 +477,11 @@ void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, ui
 }
 
 void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
-                       uint32_t s_chunk, const uint32_t* task_off, JoinTask* tasks) {
-    hipLaunchKernelGGL(k_fill_tasks, dim3((nparts + 255) / 256), dim3(256), 0, st, bounds, nparts,
-                       cap, s_chunk, task_off, tasks);
+                       uint32_t s_chunk, const uint32_t* task_off, uint32_t ntasks,
+                       JoinTask* tasks) {
+    if (ntasks)
+        hipLaunchKernelGGL(k_fill_tasks, dim3((ntasks + 255) / 256), dim3(256), 0, st, bounds,
+                           nparts, cap, s_chunk, task_off, ntasks, tasks);
 }
 
 template <int MODE, bool MAT>

@@ -88,7 +88,8 @@ void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, ui
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
                         uint32_t* ntasks_dev, uint32_t* stats);
 void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
-                       uint32_t s_chunk, const uint32_t* task_off, JoinTask* tasks);
+                       uint32_t s_chunk, const uint32_t* task_off, uint32_t ntasks,
+                       JoinTask* tasks);
 void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, const uint2* Rb,
                  const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
                  uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,

@@ -26,29 +26,28 @@ struct JoinTask {
     uint32_t r_off, r_len, s_off, s_len;
 };
 
-// Piecewise-linear key-CDF model (learned range partitioner).  Knot i covers
-// keys [x[i], x[i+1]); its cumulative value y[i] is in units of
-// 2^-frac partitions.  Keys below x[0] map to partition 0 and keys at or
-// above x[nseg] to the last one (the model is clamped, so it stays
-// monotone).  kmin/kmax are R's exact key bounds: keys outside are dropped
-// from S because they cannot match.
+// Piecewise-linear key-CDF model (learned range partitioner).  Knot i
+// (key x[i]) has cumulative value y[i] in units of 2^-frac partitions; the
+// knots sit at equal-mass sample quantiles between R's exact minimum kmin
+// and maximum kmax (x[nseg] = kmax + 1, y[nseg] = 2^32).  The partition
+// function reads the model through an equal-width cell table: cells[c] is
+// the model at key kmin + (c << cshift) in units of 1/256 partition, and a
+// key interpolates linearly (integer arithmetic) between the values at its
+// cell's two ends, so the map is monotone: partitions are key ranges in key
+// order.  Keys outside [kmin, kmax] cannot match R and are dropped from S.
 constexpr int kMaxKnots = 4096;
-constexpr int kGuideCells = 2048;  // equal-width cells over [kmin, kmax]
+constexpr int kCells = 4096;
 struct CdfModel {
     uint32_t nseg;
     uint32_t frac;        // fixed-point fraction bits of y
     uint32_t kmin, kmax;  // exact R key range (inclusive)
     uint32_t drop_outside;
-    uint32_t gshift;      // cell of a key = (key - kmin) >> gshift
-    uint32_t ncells;
+    uint32_t cshift;      // cell of a key = (key - kmin) >> cshift
+    uint32_t ncells;      // cells over [kmin, kmax]; cells[ncells] = 256 << bits
     uint32_t pad;
-    // guide[c] = segment holding the first key of cell c; guide[ncells] =
-    // nseg - 1.  A key of cell c lies in a segment of [guide[c], guide[c+1]].
-    uint16_t guide[kGuideCells + 1];
-    uint16_t pad2[7];
-    uint64_t x[kMaxKnots + 1];      // knot keys (x[nseg] may be 2^32)
-    uint32_t y[kMaxKnots + 1];      // cumulative partition position
-    uint64_t slope[kMaxKnots];      // (y[i+1]-y[i]) * 2^32 / (x[i+1]-x[i])
+    uint64_t x[kMaxKnots + 1];  // knot keys (x[nseg] may be 2^32)
+    uint32_t y[kMaxKnots + 1];  // cumulative partition position
+    uint32_t cells[kCells + 1];
 };
 
 enum BinKind : uint32_t { kBinRadix = 0, kBinLearned = 1 };

@@ -17,59 +17,23 @@ namespace tbj {
 // ---------------------------------------------------------------------------
 // Partition function
 
-// LDS form of a CdfModel: per segment its first key, the model value at that
-// key and the slope as floats (in partitions), and the range of partitions
-// the segment maps to; plus the guide cells.  Evaluating in float keeps the
-// map monotone inside a segment (conversions, products and sums round
-// monotonically) and the clamp to the segment's partition range keeps it
-// monotone across segments; R and S go through the same code, so equal keys
-// always land in the same partition.
+// LDS copy of a CdfModel's cell table.  Within cell c a key at offset d
+// from the cell start maps to cells[c] + (d * (cells[c+1] - cells[c]) >>
+// cshift) (1/256 partitions): d < 2^cshift keeps the value below
+// cells[c+1], so the map is monotone inside and across cells, and R and S
+// go through the same code, so equal keys always land in the same partition.
 struct LdsModel {
-    uint32_t x[kLdsKnots + 1];
-    float base[kLdsKnots];
-    float slope[kLdsKnots];
-    uint32_t plo[kLdsKnots], phi[kLdsKnots];
-    uint16_t guide[kGuideCells + 1];
+    uint32_t cells[kCells + 1];
 };
 
-__device__ __forceinline__ uint32_t model_eval(const LdsModel* m, uint32_t nseg, uint32_t kmin,
-                                               uint32_t gshift, uint32_t ncells, uint32_t key) {
-    if (key < m->x[0]) return 0;
-    uint32_t c = (key - kmin) >> gshift;
-    if (c >= ncells) c = ncells - 1;
-    uint32_t lo = m->guide[c], hi = (uint32_t)m->guide[c + 1] + 1;  // answer in [lo, hi)
-    if (hi > nseg) hi = nseg;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (m->x[mid] <= key) lo = mid;
-        else hi = mid;
-    }
-    const float v = fmaf((float)(key - m->x[lo]), m->slope[lo], m->base[lo]);
-    uint32_t p = v <= 0.0f ? 0u : (uint32_t)v;
-    p = max(p, m->plo[lo]);
-    return min(p, m->phi[lo]);
-}
-
 __device__ __forceinline__ void load_model(LdsModel* lm, const CdfModel* m) {
-    const uint32_t n = m->nseg;
-    const float unit = 1.0f / (float)(1u << (m->frac > 31 ? 31 : m->frac));
-    for (uint32_t i = threadIdx.x; i <= n; i += blockDim.x) {
-        lm->x[i] = i < n ? (uint32_t)m->x[i] : 0xFFFFFFFFu;
-        if (i < n) {
-            const uint32_t y0 = m->y[i], y1 = m->y[i + 1];
-            lm->base[i] = (float)y0 * unit;
-            lm->slope[i] = (float)((double)(y1 - y0) * unit / (double)(m->x[i + 1] - m->x[i]));
-            lm->plo[i] = y0 >> m->frac;
-            lm->phi[i] = (y1 > y0 ? y1 - 1 : y0) >> m->frac;
-        }
-    }
-    for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->guide[i] = m->guide[i];
+    for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->cells[i] = m->cells[i];
 }
 
 struct BinFn {
     BinSpec bs;
     const LdsModel* lm;
-    uint32_t nseg, kmin, kmax, gshift, ncells;
+    uint32_t kmin, kmax, cshift, ncells, last;
 
     __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
         uint32_t full;
@@ -77,24 +41,36 @@ struct BinFn {
             full = bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
         } else {
             if (bs.drop && (key < kmin || key > kmax)) return TBJ_DROP;
-            full = model_eval(lm, nseg, kmin, gshift, ncells, key);
+            if (key < kmin) {
+                full = 0;
+            } else {
+                const uint32_t d = key - kmin, c = d >> cshift;
+                if (c >= ncells) {
+                    full = last;
+                } else {
+                    const uint32_t y0 = lm->cells[c], y1 = lm->cells[c + 1];
+                    const uint32_t off = d & ((1u << cshift) - 1);
+                    full = (y0 + (uint32_t)(((uint64_t)off * (y1 - y0)) >> cshift)) >> 8;
+                }
+            }
         }
         return (full >> bs.pass_shift) & bs.pass_mask;
     }
 };
 
+// Loads the cell table into LDS (no barrier: the caller's next barrier covers it).
 __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     BinFn f;
     f.bs = bs;
     f.lm = lm;
-    f.nseg = f.kmin = f.kmax = f.gshift = f.ncells = 0;
+    f.kmin = f.kmax = f.cshift = f.ncells = f.last = 0;
     if (bs.kind == kBinLearned) {
         load_model(lm, bs.model);
-        f.nseg = bs.model->nseg;
         f.kmin = bs.model->kmin;
         f.kmax = bs.model->kmax;
-        f.gshift = bs.model->gshift;
+        f.cshift = bs.model->cshift;
         f.ncells = bs.model->ncells;
+        f.last = bs.bits >= 32 ? 0xFFFFFFFFu : (1u << bs.bits) - 1;
     }
     return f;
 }
@@ -599,28 +575,28 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
         model->y[i] = (uint32_t)yi;
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nseg; i += kFitBlock) {
-        const uint64_t dx = model->x[i + 1] - model->x[i];
-        const uint64_t dy = (uint64_t)model->y[i + 1] - model->y[i];
-        model->slope[i] = dx ? ((dy << 32) / dx) : 0;
-    }
-    // guide cells: equal-width over [kmin, kmax]
-    uint32_t gshift = 0;
-    while ((((uint64_t)kmax - kmin) >> gshift) + 1 > (uint64_t)kGuideCells) gshift++;
-    const uint32_t ncells = (uint32_t)((((uint64_t)kmax - kmin) >> gshift) + 1);
+    // cell table: the model at equal-width cell starts, in 1/256 partitions
+    uint32_t cshift = 0;
+    while ((((uint64_t)kmax - kmin) >> cshift) >= (uint64_t)kCells) cshift++;
+    const uint32_t ncells = (uint32_t)((((uint64_t)kmax - kmin) >> cshift) + 1);
+    const double to_cells = 1.0 / (double)(1ull << (frac - 8));  // 2^-frac -> 2^-8 units
     for (uint32_t c = threadIdx.x; c <= ncells; c += kFitBlock) {
-        uint32_t g = nseg - 1;
-        if (c < ncells) {
-            const uint64_t key = (uint64_t)kmin + ((uint64_t)c << gshift);
-            uint32_t lo = 0, hi = nseg;  // largest i with knots[i] <= key
+        const uint64_t key = (uint64_t)kmin + ((uint64_t)c << cshift);
+        double y;
+        if (c == ncells || key > kmax) {
+            y = 4294967296.0;
+        } else {
+            uint32_t lo = 0, hi = nseg;  // largest i with x[i] <= key
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (knots[mid] <= key) lo = mid;
+                if (model->x[mid] <= key) lo = mid;
                 else hi = mid;
             }
-            g = lo;
+            const double dx = (double)(model->x[lo + 1] - model->x[lo]);
+            const double dy = (double)model->y[lo + 1] - (double)model->y[lo];
+            y = (double)model->y[lo] + (double)(key - model->x[lo]) * dy / dx;
         }
-        model->guide[c] = (uint16_t)g;
+        model->cells[c] = (uint32_t)(y * to_cells);
     }
     if (threadIdx.x == 0) {
         model->nseg = nseg;
@@ -628,7 +604,7 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
         model->kmin = kmin;
         model->kmax = kmax;
         model->drop_outside = 1;
-        model->gshift = gshift;
+        model->cshift = cshift;
         model->ncells = ncells;
     }
 }

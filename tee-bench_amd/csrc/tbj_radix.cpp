@@ -293,7 +293,7 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
     res->max_r_partition = h.stats[0];
     JoinTask* tasks = nullptr;
     TBJ_TRY(W.get("tasks", sizeof(JoinTask) * std::max<uint32_t>(1, h.ntasks), (void**)&tasks));
-    launch_fill_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, tasks);
+    launch_fill_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, h.ntasks, tasks);
     TBJ_TRY(pt.mark(kPhTasks));
     const uint32_t hshift = learned ? 0 : plan.B;
     const uint32_t occ = (uint32_t)join_occupancy(J.mode, J.mat);

@@ -148,11 +148,66 @@ __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hm
     }
 }
 
-// One task per loop trip: every R and S tuple of the task is loaded into
-// registers first (one memory phase), the R range is built into LDS, the S
-// tuples are probed, and with MAT the block claims one output range for
-// the whole task.  S ranges are at most kJoinBlock * join_s_items(MAT)
-// tuples (fewer with MAT, whose per-item output slots need registers).
+// Counts the matches of N S tuples at once (sv[k0 .. k0 + N), item k at S
+// position s0 + k * kJoinBlock + threadIdx.x, valid below s_len): the N
+// chains (or ranges) are walked in lockstep, so each step issues N
+// independent LDS reads instead of one dependent read per step.  ck[q] =
+// matches of item q, fp[q] = R.payload of its (last) match.
+template <int MODE, int N, int SI>
+__device__ __forceinline__ void lds_count_batch(const uint2 (&sv)[SI], int k0, uint32_t s0,
+                                                uint32_t s_len, uint32_t hshift, uint32_t hmask,
+                                                const uint32_t* rkey, const uint32_t* rpay,
+                                                const uint32_t* head, const uint16_t* nxt,
+                                                uint32_t (&ck)[N], uint32_t (&fp)[N],
+                                                uint64_t& cks) {
+    uint32_t e[N], en[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        const bool ok = s0 + (k0 + q) * kJoinBlock + threadIdx.x < s_len;
+        const uint32_t h = (sv[k0 + q].x >> hshift) & hmask;
+        ck[q] = 0;
+        fp[q] = 0;
+        if (MODE == kModeChain) {
+            e[q] = ok ? head[h] : 0;  // 1-based, 0 = end of chain
+            en[q] = 0;
+        } else {
+            e[q] = ok ? head[h] : 0;
+            en[q] = ok ? head[h + 1] : 0;
+        }
+    }
+    for (;;) {
+        bool any = false;
+        uint32_t rk[N], nx[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const bool live = MODE == kModeChain ? e[q] != 0 : e[q] < en[q];
+            const uint32_t i = MODE == kModeChain ? (live ? e[q] - 1 : 0) : (live ? e[q] : 0);
+            rk[q] = rkey[i];
+            nx[q] = MODE == kModeChain ? nxt[i] : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const bool live = MODE == kModeChain ? e[q] != 0 : e[q] < en[q];
+            if (live) {
+                const uint32_t i = MODE == kModeChain ? e[q] - 1 : e[q];
+                if (rk[q] == sv[k0 + q].x) {
+                    fp[q] = rpay[i];
+                    ck[q]++;
+                    cks += (uint64_t)fp[q] + sv[k0 + q].y;
+                }
+                e[q] = MODE == kModeChain ? nx[q] : e[q] + 1;
+                any |= MODE == kModeChain ? e[q] != 0 : e[q] < en[q];
+            }
+        }
+        if (!any) break;
+    }
+}
+
+// One task per loop trip: the R range and the first S sub-chunk are loaded
+// into registers together (one memory phase), the R range is built into LDS
+// once, and the task's S range is probed in sub-chunks of SITEMS *
+// kJoinBlock tuples (fewer with MAT, whose per-item output slots need
+// registers); with MAT the block claims one output range per sub-chunk.
 template <int MODE, bool MAT, int SITEMS = MAT ? kJoinSItems / 2 : kJoinSItems>
 __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join(const uint2* __restrict__ Rb,
                                                      const uint2* __restrict__ Sb,
@@ -176,16 +231,26 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     constexpr int HALF = SITEMS / NCLAIM;       // items per output claim
     __shared__ uint32_t wtot[kJoinBlock / TBJ_WAVE][HALF];
 
+    __shared__ JoinTask tk_s;
+
     const uint32_t ntasks = *ntasks_dev;
     uint64_t matches = 0, cks = 0;
-    if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+    if (threadIdx.x == 0) {
+        task_s = atomicAdd(task_ctr, 1u);
+        if (task_s < ntasks) tk_s = tasks[task_s];
+    }
     for (uint32_t i = threadIdx.x; i < NHEAD; i += kJoinBlock) head[i] = 0;
     __syncthreads();
     for (;;) {
         const uint32_t t = task_s;
         if (t >= ntasks) break;
-        const JoinTask tk = tasks[t];
+        const JoinTask tk = tk_s;
         const uint32_t n = tk.r_len;
+        // the next task's index is claimed now and its descriptor read after
+        // the build, so neither round trip sits between two tasks
+        uint32_t next_t = 0;
+        JoinTask nd{};
+        if (threadIdx.x == 0) next_t = atomicAdd(task_ctr, 1u);
         // ---- loads: R and S of the task in flight together
         uint2 rv[RITEMS], sv[SITEMS];
 #pragma unroll
@@ -240,79 +305,95 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
             __syncthreads();
         }
 
-        // ---- probe
-        if (!MAT) {
-            uint32_t c = 0, fdummy = 0;
-            uint64_t o = 0;
+        if (threadIdx.x == 0 && next_t < ntasks) nd = tasks[next_t];
+        // ---- probe the S range in sub-chunks of SITEMS * kJoinBlock tuples
+        // against the one build (sub-chunk 0 was loaded with R)
+        for (uint32_t s0 = 0; s0 < tk.s_len; s0 += SITEMS * kJoinBlock) {
+            if (s0) {
 #pragma unroll
-            for (int k = 0; k < SITEMS; k++) {
-                const uint32_t j = k * kJoinBlock + threadIdx.x;
-                if (j < tk.s_len)
-                    lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c, cks,
-                                           fdummy, pairs, cap, o);
+                for (int k = 0; k < SITEMS; k++) {
+                    const uint32_t j = s0 + k * kJoinBlock + threadIdx.x;
+                    if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
+                }
             }
-            matches += c;
-        } else {
-            // NCLAIM groups of HALF items: count (remembering the first
-            // match), claim one output range per group, write.  Slots go in
-            // item-major order (all threads' item 0, then item 1, ...) so that
-            // a wave's stores of one item are contiguous.
-            const uint32_t lane = lane_id(), w = wave_id();
+            // ---- probe
+            if (!MAT) {
+                constexpr int G = MODE == kModeChain ? 8 : 4;
 #pragma unroll
-            for (int hh = 0; hh < NCLAIM; hh++) {
-                uint32_t ck[HALF], fp[HALF];
+                for (int k0 = 0; k0 < SITEMS; k0 += G) {
+                    uint32_t ck[G], fp[G];
+                    lds_count_batch<MODE, G>(sv, k0, s0, tk.s_len, hshift, hmask, rkey, rpay, head,
+                                             nxt, ck, fp, cks);
 #pragma unroll
-                for (int q = 0; q < HALF; q++) {
-                    const int k = hh * HALF + q;
-                    const uint32_t j = k * kJoinBlock + threadIdx.x;
-                    uint32_t ci = 0;
-                    uint64_t od = 0;
-                    fp[q] = 0;
-                    if (j < tk.s_len)
-                        lds_probe<MODE, false>(sv[k], hshift, hmask, rkey, rpay, head, nxt, ci, cks,
-                                               fp[q], pairs, cap, od);
-                    ck[q] = ci;
-                    matches += ci;
-                    const uint32_t incl = wave_inclusive_scan(ci);
-                    if (lane == TBJ_WAVE - 1) wtot[w][q] = incl;
+                    for (int q = 0; q < G; q++) matches += ck[q];
                 }
-                __syncthreads();
-                if (threadIdx.x == 0) {
-                    uint32_t tot = 0;
-                    for (int q = 0; q < HALF; q++)
-                        for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) tot += wtot[ww][q];
-                    obase_s = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
-                }
-                __syncthreads();
-                uint64_t item_base = obase_s;
+            } else {
+                // NCLAIM groups of HALF items: count (remembering the first
+                // match), claim one output range per group, write.  Slots go in
+                // item-major order (all threads' item 0, then item 1, ...) so that
+                // a wave's stores of one item are contiguous.
+                const uint32_t lane = lane_id(), w = wave_id();
 #pragma unroll
-                for (int q = 0; q < HALF; q++) {
-                    const int k = hh * HALF + q;
-                    uint32_t before = 0, all = 0;
-                    for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) {
-                        const uint32_t v = wtot[ww][q];
-                        before += (uint32_t)ww < w ? v : 0;
-                        all += v;
+                for (int hh = 0; hh < NCLAIM; hh++) {
+                    uint32_t ck[HALF], fp[HALF];
+#pragma unroll
+                    for (int g0 = 0; g0 < HALF; g0 += 4) {
+                        uint32_t c4[4], f4[4];
+                        lds_count_batch<MODE, 4>(sv, hh * HALF + g0, s0, tk.s_len, hshift, hmask,
+                                                 rkey, rpay, head, nxt, c4, f4, cks);
+#pragma unroll
+                        for (int q = 0; q < 4; q++) {
+                            ck[g0 + q] = c4[q];
+                            fp[g0 + q] = f4[q];
+                        }
                     }
-                    const uint32_t incl = wave_inclusive_scan(ck[q]);
-                    uint64_t ok = item_base + before + incl - ck[q];
-                    if (ck[q] == 1) {
-                        if (ok < cap) pairs[ok] = make_uint2(fp[q], sv[k].y);
-                    } else if (ck[q] > 1) {
-                        uint32_t c2 = 0, f2 = 0;
-                        uint64_t k2 = 0;
-                        lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
-                                              f2, pairs, cap, ok);
+#pragma unroll
+                    for (int q = 0; q < HALF; q++) {
+                        matches += ck[q];
+                        const uint32_t incl = wave_inclusive_scan(ck[q]);
+                        if (lane == TBJ_WAVE - 1) wtot[w][q] = incl;
                     }
-                    item_base += all;
+                    __syncthreads();
+                    if (threadIdx.x == 0) {
+                        uint32_t tot = 0;
+                        for (int q = 0; q < HALF; q++)
+                            for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) tot += wtot[ww][q];
+                        obase_s = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
+                    }
+                    __syncthreads();
+                    uint64_t item_base = obase_s;
+#pragma unroll
+                    for (int q = 0; q < HALF; q++) {
+                        const int k = hh * HALF + q;
+                        uint32_t before = 0, all = 0;
+                        for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) {
+                            const uint32_t v = wtot[ww][q];
+                            before += (uint32_t)ww < w ? v : 0;
+                            all += v;
+                        }
+                        const uint32_t incl = wave_inclusive_scan(ck[q]);
+                        uint64_t ok = item_base + before + incl - ck[q];
+                        if (ck[q] == 1) {
+                            if (ok < cap) pairs[ok] = make_uint2(fp[q], sv[k].y);
+                        } else if (ck[q] > 1) {
+                            uint32_t c2 = 0, f2 = 0;
+                            uint64_t k2 = 0;
+                            lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
+                                                  f2, pairs, cap, ok);
+                        }
+                        item_base += all;
+                    }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
         }
         __syncthreads();
         // clear the heads for the next task and fetch its index
         for (uint32_t i = threadIdx.x; i <= nb && i < (uint32_t)NHEAD; i += kJoinBlock) head[i] = 0;
-        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+        if (threadIdx.x == 0) {
+            task_s = next_t;
+            tk_s = nd;
+        }
         __syncthreads();
     }
     // block reduction of the counters

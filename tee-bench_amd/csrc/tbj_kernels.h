@@ -28,9 +28,11 @@ constexpr int kJoinCap = 4096;   // max R tuples of one build (LDS hash table)
 constexpr int kJoinProbeItems = 4;
 constexpr int kJoinSItems = 16;  // S tuples per thread of one task
 constexpr int kJoinSChunk = kJoinBlock * kJoinSItems;  // 8192
-// S tuples per join task: kJoinSChunk counting, half of it materialising
-// (the per-item output slots need the registers)
+// S tuples probed per sub-chunk of a join task: kJoinSChunk counting, half
+// of it materialising (the per-item output slots need the registers)
 inline int join_s_chunk(bool materialize) { return materialize ? kJoinSChunk / 2 : kJoinSChunk; }
+// S tuples per join task (the R range is built once per task)
+constexpr int kJoinSTaskChunks = 4;
 
 enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 

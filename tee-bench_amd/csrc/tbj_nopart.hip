@@ -59,10 +59,12 @@ __global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, 
 }
 
 // Block-aggregated output: every round each thread handles kProbeItems S
-// tuples; matches are counted, the block claims one output range, and the
-// matches are walked again to write them.
+// tuples; all their buckets are loaded before any is used (independent
+// loads in flight together), matches are counted, the block claims one
+// output range, and the buckets still in registers are walked again to
+// write the matches.
 constexpr int kProbeBlock = 256;
-constexpr int kProbeItems = 4;
+constexpr int kProbeItems = 8;
 
 template <bool MAT>
 __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
@@ -77,6 +79,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
     for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < nS;
          base += (uint64_t)gridDim.x * per_round) {
         uint2 sv[kProbeItems];
+        PhtBucket bk[kProbeItems];
         uint32_t c = 0;
 #pragma unroll
         for (int k = 0; k < kProbeItems; k++) {
@@ -86,15 +89,25 @@ __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
 #pragma unroll
         for (int k = 0; k < kProbeItems; k++) {
             const uint64_t j = base + k * kProbeBlock + threadIdx.x;
-            if (j >= nS) continue;
-            const PhtBucket b = tab[sv[k].x & mask];
-            const uint32_t n = min(b.count, 2u);
-            for (uint32_t q = 0; q < n; q++)
-                if (b.slot[q].x == sv[k].x) {
-                    c++;
-                    cks += (uint64_t)b.slot[q].y + sv[k].y;
-                }
-            for (uint32_t e = b.ovf; e; e = pool[e - 1].next)
+            if (j < nS) {
+                bk[k] = tab[sv[k].x & mask];
+            } else {
+                bk[k].count = 0;
+                bk[k].ovf = 0;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            // slots by constant index, so the buckets stay in registers
+            if (bk[k].count >= 1 && bk[k].slot[0].x == sv[k].x) {
+                c++;
+                cks += (uint64_t)bk[k].slot[0].y + sv[k].y;
+            }
+            if (bk[k].count >= 2 && bk[k].slot[1].x == sv[k].x) {
+                c++;
+                cks += (uint64_t)bk[k].slot[1].y + sv[k].y;
+            }
+            for (uint32_t e = bk[k].ovf; e; e = pool[e - 1].next)
                 if (pool[e - 1].t.x == sv[k].x) {
                     c++;
                     cks += (uint64_t)pool[e - 1].t.y + sv[k].y;
@@ -110,16 +123,15 @@ __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
             if (c) {
 #pragma unroll
                 for (int k = 0; k < kProbeItems; k++) {
-                    const uint64_t j = base + k * kProbeBlock + threadIdx.x;
-                    if (j >= nS) continue;
-                    const PhtBucket b = tab[sv[k].x & mask];
-                    const uint32_t n = min(b.count, 2u);
-                    for (uint32_t q = 0; q < n; q++)
-                        if (b.slot[q].x == sv[k].x) {
-                            if (o < cap) pairs[o] = make_uint2(b.slot[q].y, sv[k].y);
-                            o++;
-                        }
-                    for (uint32_t e = b.ovf; e; e = pool[e - 1].next)
+                    if (bk[k].count >= 1 && bk[k].slot[0].x == sv[k].x) {
+                        if (o < cap) pairs[o] = make_uint2(bk[k].slot[0].y, sv[k].y);
+                        o++;
+                    }
+                    if (bk[k].count >= 2 && bk[k].slot[1].x == sv[k].x) {
+                        if (o < cap) pairs[o] = make_uint2(bk[k].slot[1].y, sv[k].y);
+                        o++;
+                    }
+                    for (uint32_t e = bk[k].ovf; e; e = pool[e - 1].next)
                         if (pool[e - 1].t.x == sv[k].x) {
                             if (o < cap) pairs[o] = make_uint2(pool[e - 1].t.y, sv[k].y);
                             o++;

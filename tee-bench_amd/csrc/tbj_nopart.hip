@@ -254,48 +254,6 @@ __global__ void k_cht_overflow(const uint2* __restrict__ R, const uint32_t* __re
     ov[h] = t;
 }
 
-// Looks one S tuple up; EMIT=false counts (and sums the checksum), EMIT=true
-// writes the matches from output slot o on.
-template <bool EMIT>
-__device__ __forceinline__ void cht_lookup(uint2 s, const ChtParams& p,
-                                           const uint32_t* __restrict__ bits,
-                                           const uint32_t* __restrict__ cnt,
-                                           const uint32_t* __restrict__ taken,
-                                           const uint2* __restrict__ arr, uint32_t nslots,
-                                           bool any_ov, const uint32_t* __restrict__ ov_used,
-                                           const uint2* __restrict__ ov, uint32_t& c,
-                                           uint64_t& cks, uint2* __restrict__ pairs,
-                                           uint64_t cap, uint64_t& o) {
-    const uint32_t key = s.x;
-    const uint32_t pos = cht_pos(bits, cnt, key & p.bmask);
-#pragma unroll
-    for (uint32_t q = 0; q < 2; q++) {
-        const uint32_t ps = pos + q;
-        if (ps < nslots && (taken[ps >> 5] & (1u << (ps & 31))) && arr[ps].x == key) {
-            if (EMIT) {
-                if (o < cap) pairs[o] = make_uint2(arr[ps].y, s.y);
-                o++;
-            } else {
-                c++;
-                cks += (uint64_t)arr[ps].y + s.y;
-            }
-        }
-    }
-    if (any_ov) {
-        for (uint32_t e = (key * 2654435761u) & p.ov_mask; ov_used[e]; e = (e + 1) & p.ov_mask) {
-            if (ov[e].x == key) {
-                if (EMIT) {
-                    if (o < cap) pairs[o] = make_uint2(ov[e].y, s.y);
-                    o++;
-                } else {
-                    c++;
-                    cks += (uint64_t)ov[e].y + s.y;
-                }
-            }
-        }
-    }
-}
-
 template <bool MAT>
 __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
     const uint2* __restrict__ S, uint64_t nS, ChtParams p, const uint32_t* __restrict__ bits,
@@ -311,9 +269,14 @@ __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
     const uint64_t per_round = (uint64_t)kProbeBlock * kProbeItems;
     for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < nS;
          base += (uint64_t)gridDim.x * per_round) {
+        // every level of the lookup is issued for all items before any is
+        // used: S, then bitmap word + prefix count, then the two dense slots
+        // and their taken bits
         uint2 sv[kProbeItems];
+        uint32_t bw[kProbeItems], cw[kProbeItems], pos[kProbeItems];
+        uint2 a0[kProbeItems], a1[kProbeItems];
+        uint32_t t0[kProbeItems], t1[kProbeItems];
         uint32_t c = 0;
-        uint64_t o = 0;
 #pragma unroll
         for (int k = 0; k < kProbeItems; k++) {
             const uint64_t j = base + k * kProbeBlock + threadIdx.x;
@@ -321,10 +284,40 @@ __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
         }
 #pragma unroll
         for (int k = 0; k < kProbeItems; k++) {
+            const uint32_t w = (sv[k].x & p.bmask) >> 5;
+            bw[k] = bits[w];
+            cw[k] = cnt[w];
+        }
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
+            const uint32_t h = sv[k].x & p.bmask;
+            pos[k] = cw[k] + __popc(bw[k] & ((1u << (h & 31)) - 1u));
+            const uint32_t p0 = min(pos[k], nslots - 1), p1 = min(pos[k] + 1, nslots - 1);
+            a0[k] = arr[p0];
+            a1[k] = arr[p1];
+            t0[k] = taken[p0 >> 5];
+            t1[k] = taken[p1 >> 5];
+        }
+        uint32_t hit[kProbeItems];  // bit 0: slot pos matches, bit 1: slot pos + 1
+#pragma unroll
+        for (int k = 0; k < kProbeItems; k++) {
             const uint64_t j = base + k * kProbeBlock + threadIdx.x;
-            if (j < nS)
-                cht_lookup<false>(sv[k], p, bits, cnt, taken, arr, nslots, any_ov, ov_used, ov, c,
-                                  cks, pairs, cap, o);
+            const uint32_t p0 = pos[k], p1 = pos[k] + 1;
+            const bool ok = j < nS;
+            const bool m0 = ok && p0 < nslots && ((t0[k] >> (p0 & 31)) & 1u) && a0[k].x == sv[k].x;
+            const bool m1 = ok && p1 < nslots && ((t1[k] >> (p1 & 31)) & 1u) && a1[k].x == sv[k].x;
+            hit[k] = (m0 ? 1u : 0u) | (m1 ? 2u : 0u);
+            if (m0) cks += (uint64_t)a0[k].y + sv[k].y;
+            if (m1) cks += (uint64_t)a1[k].y + sv[k].y;
+            c += (uint32_t)m0 + (uint32_t)m1;
+            if (any_ov && ok) {
+                for (uint32_t e = (sv[k].x * 2654435761u) & p.ov_mask; ov_used[e];
+                     e = (e + 1) & p.ov_mask)
+                    if (ov[e].x == sv[k].x) {
+                        c++;
+                        cks += (uint64_t)ov[e].y + sv[k].y;
+                    }
+            }
         }
         matches += c;
         if (MAT) {
@@ -333,15 +326,27 @@ __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
             if (threadIdx.x == 0)
                 obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
             __syncthreads();
-            o = obase_s + ex;
+            uint64_t o = obase_s + ex;
             if (c) {
-                uint64_t dummy = 0;
 #pragma unroll
                 for (int k = 0; k < kProbeItems; k++) {
                     const uint64_t j = base + k * kProbeBlock + threadIdx.x;
-                    if (j < nS)
-                        cht_lookup<true>(sv[k], p, bits, cnt, taken, arr, nslots, any_ov, ov_used,
-                                         ov, c, dummy, pairs, cap, o);
+                    if (hit[k] & 1u) {
+                        if (o < cap) pairs[o] = make_uint2(a0[k].y, sv[k].y);
+                        o++;
+                    }
+                    if (hit[k] & 2u) {
+                        if (o < cap) pairs[o] = make_uint2(a1[k].y, sv[k].y);
+                        o++;
+                    }
+                    if (any_ov && j < nS) {
+                        for (uint32_t e = (sv[k].x * 2654435761u) & p.ov_mask; ov_used[e];
+                             e = (e + 1) & p.ov_mask)
+                            if (ov[e].x == sv[k].x) {
+                                if (o < cap) pairs[o] = make_uint2(ov[e].y, sv[k].y);
+                                o++;
+                            }
+                    }
                 }
             }
             __syncthreads();

@@ -66,7 +66,8 @@ struct SmTask {
     uint32_t s_off, s_len;  // S range of the same partition
     uint32_t rchunk;        // index of the R chunk
     uint32_t first;         // 1 for the chunk's first S range (writes the sorted chunk)
-    uint32_t pad[2];
+    uint32_t multi;         // 1 if other tasks share this R chunk (more than one S range)
+    uint32_t pad;
 };
 
 // One kSm64MergeTile slice of the output of a merge pass over an R partition.
@@ -93,8 +94,10 @@ void sm64_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs, u
                  uint32_t inner, uint32_t passes, uint32_t nR, uint32_t nS, uint2* bounds);
 void sm64_task_counts(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t* rcnt,
                       uint32_t* jcnt, uint32_t* max_r);
+// One thread per task (nrtasks R chunks, njtasks join tasks).
 void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const uint32_t* roff,
-                     const uint32_t* joff, SmTask* rtasks, SmTask* jtasks);
+                     const uint32_t* joff, uint32_t nrtasks, uint32_t njtasks, SmTask* rtasks,
+                     SmTask* jtasks);
 // Sorts every R chunk by key in place (bucket sort in LDS).
 void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
                  uint32_t nrtasks);
@@ -108,8 +111,15 @@ void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const u
                 ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
                 const uint32_t* ntasks_dev, unsigned long long* runs,
                 unsigned long long* chunk_vol, unsigned long long* counters);
-// Exclusive scan of a[0, n) (u64) in place by one workgroup; *total = sum.
-void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* total);
+// Zeroes the run counters of R chunks that several join tasks share (the
+// count kernel writes the others whole).
+void sm64_clear_shared_runs(hipStream_t st, const SmTask* tasks, uint32_t ntasks,
+                            unsigned long long* runs);
+// Exclusive scan of a[0, n) (u64) in place; *total = sum; part needs
+// sm64_scan_partials(n) words.
+uint32_t sm64_scan_partials(uint32_t n);
+void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* part,
+                   unsigned long long* total);
 // runs[g] (S count of the key run starting at sorted R position g) becomes
 // the run's first output slot.
 void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,

@@ -22,6 +22,8 @@
 //      and writes them.
 // So pairs of one key are contiguous, and with the learned partitioner the
 // whole output is in key order (radix: in key order inside each partition).
+#include <algorithm>
+
 #include "tbj_device.h"
 #include "tbj_kernels.h"
 #include "tbj_sm64.h"
@@ -308,11 +310,21 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
     const TileDesc td = tiles[t];
     const uint64_t* keys = reinterpret_cast<const uint64_t*>(in + td.in_off);
     uint64_t mn = ~0ull, mx = 0;
-    for (uint32_t j = threadIdx.x; j < td.len; j += kSm64Block) {
-        const uint64_t k = keys[2 * (uint64_t)j];
-        mn = min(mn, k);
-        mx = max(mx, k);
-        agg_add(cnt, bin(k));
+    constexpr int kHistItems = kSm64Span / kSm64Block;
+    uint64_t kv[kHistItems];
+#pragma unroll
+    for (int q = 0; q < kHistItems; q++) {  // all loads in flight before any use
+        const uint32_t j = q * kSm64Block + threadIdx.x;
+        kv[q] = j < td.len ? keys[2 * (uint64_t)j] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < kHistItems; q++) {
+        const uint32_t j = q * kSm64Block + threadIdx.x;
+        if (j < td.len) {
+            mn = min(mn, kv[q]);
+            mx = max(mx, kv[q]);
+        }
+        agg_add(cnt, j < td.len ? bin(kv[q]) : kNone);
     }
     if (mm) {  // tile min / max: waves -> LDS -> one global atomic pair
         for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
@@ -355,19 +367,20 @@ __global__ __launch_bounds__(kSm64Block) void k_scatter64(
     const uint32_t tis = t - sd.first_tile;
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
         next[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis];
+    ulonglong2 v[kSm64Items];
+#pragma unroll
+    for (uint32_t k = 0; k < kSm64Items; k++) {
+        const uint32_t j = k * kSm64Block + threadIdx.x;
+        if (j < td.len) v[k] = in[td.in_off + j];
+    }
     for (uint32_t s0 = 0; s0 < td.len; s0 += kSm64Tile) {
         for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) loff[i] = 0;
         __syncthreads();
-        ulonglong2 v[kSm64Items];
         uint32_t bb[kSm64Items], rank[kSm64Items];
 #pragma unroll
         for (uint32_t k = 0; k < kSm64Items; k++) {
             const uint32_t j = s0 + k * kSm64Block + threadIdx.x;
-            bb[k] = kNone;
-            if (j < td.len) {
-                v[k] = in[td.in_off + j];
-                bb[k] = bin(v[k].x);
-            }
+            bb[k] = j < td.len ? bin(v[k].x) : kNone;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kSm64Items; k++) rank[k] = agg_inc(loff, bb[k]);
@@ -385,6 +398,12 @@ __global__ __launch_bounds__(kSm64Block) void k_scatter64(
                 stage[slot] = v[k];
                 sbin[slot] = (uint16_t)bb[k];
             }
+        }
+        // the next stage's loads go out before this stage is written back
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64Items; k++) {
+            const uint32_t j = s0 + kSm64Tile + k * kSm64Block + threadIdx.x;
+            if (j < td.len) v[k] = in[td.in_off + j];
         }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < nvalid; j += kSm64Block) out[goff[sbin[j]] + j] = stage[j];
@@ -522,26 +541,48 @@ __global__ void k_task_counts64(const uint2* __restrict__ bounds, uint32_t npart
     if (lane_id() == 0 && nr) atomicMax(max_r, nr);
 }
 
+// Task t of a list lies in the partition p with off[p] <= t < off[p + 1].
+__device__ __forceinline__ uint32_t owner_part(const uint32_t* off, uint32_t nparts, uint32_t t) {
+    uint32_t lo = 0, hi = nparts;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// One thread per task: threads [0, nrtasks) write the R chunks, threads
+// [nrtasks, nrtasks + njtasks) the join tasks (R chunk li / nsc, S chunk
+// li % nsc of their partition).
 __global__ void k_fill_tasks64(const uint2* __restrict__ bounds, uint32_t nparts,
                                const uint32_t* __restrict__ roff, const uint32_t* __restrict__ joff,
-                               SmTask* __restrict__ rtasks, SmTask* __restrict__ jtasks) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nparts) return;
-    const uint2 r = bounds[p], s = bounds[nparts + p];
-    uint32_t ro = roff[p], jo = joff[p];
-    for (uint32_t a = r.x; a < r.y; a += kSm64Cap, ro++) {
+                               uint32_t nrtasks, uint32_t njtasks, SmTask* __restrict__ rtasks,
+                               SmTask* __restrict__ jtasks) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrtasks) {
+        const uint32_t p = owner_part(roff, nparts, i);
+        const uint2 r = bounds[p];
         SmTask t{};
-        t.r_off = a;
-        t.r_len = min((uint32_t)kSm64Cap, r.y - a);
-        t.rchunk = ro;
-        rtasks[ro] = t;
-        for (uint32_t c = s.x; c < s.y; c += kSm64SChunk) {
-            SmTask j = t;
-            j.first = c == s.x;
-            j.s_off = c;
-            j.s_len = min((uint32_t)kSm64SChunk, s.y - c);
-            jtasks[jo++] = j;
-        }
+        t.r_off = r.x + (i - roff[p]) * kSm64Cap;
+        t.r_len = min((uint32_t)kSm64Cap, r.y - t.r_off);
+        t.rchunk = i;
+        rtasks[i] = t;
+    } else if (i < nrtasks + njtasks) {
+        const uint32_t jt = i - nrtasks;
+        const uint32_t p = owner_part(joff, nparts, jt);
+        const uint2 r = bounds[p], s = bounds[nparts + p];
+        const uint32_t nsc = (s.y - s.x + kSm64SChunk - 1) / kSm64SChunk;
+        const uint32_t li = jt - joff[p], rc = li / nsc, sc = li % nsc;
+        SmTask t{};
+        t.r_off = r.x + rc * kSm64Cap;
+        t.r_len = min((uint32_t)kSm64Cap, r.y - t.r_off);
+        t.s_off = s.x + sc * kSm64SChunk;
+        t.s_len = min((uint32_t)kSm64SChunk, s.y - t.s_off);
+        t.rchunk = roff[p] + rc;
+        t.first = sc == 0;
+        t.multi = nsc > 1;
+        jtasks[jt] = t;
     }
 }
 
@@ -722,25 +763,28 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n, unsig
         p[i] = v;
 }
 
-// First sorted position with key >= k in keys[0, n).
-__device__ __forceinline__ uint32_t lower64(const uint64_t* keys, uint32_t n, uint64_t k) {
-    uint32_t L = 0, H = n;
-    while (L < H) {
-        const uint32_t m = (L + H) >> 1;
-        if (keys[m] < k) L = m + 1;
-        else H = m;
+// Lower bounds of N keys at once (lo[q] = first position with key >= k[q],
+// n for invalid items): the N binary searches advance in lockstep, so every
+// step issues N independent LDS reads.  key_at(m) reads sorted key m.
+template <int N, typename KeyAt>
+__device__ __forceinline__ void lower_batch(KeyAt key_at, uint32_t n, const uint64_t (&k)[N],
+                                            const bool (&valid)[N], uint32_t (&lo)[N]) {
+    uint32_t hi[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        lo[q] = valid[q] ? 0 : n;
+        hi[q] = n;
     }
-    return L;
-}
-
-__device__ __forceinline__ uint32_t lower_tuple(const ulonglong2* a, uint32_t n, uint64_t k) {
-    uint32_t L = 0, H = n;
-    while (L < H) {
-        const uint32_t m = (L + H) >> 1;
-        if (a[m].x < k) L = m + 1;
-        else H = m;
+    for (uint32_t w = n; w; w >>= 1) {
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            if (lo[q] < hi[q]) {
+                const uint32_t m = (lo[q] + hi[q]) >> 1;
+                if (key_at(m) < k[q]) lo[q] = m + 1;
+                else hi[q] = m;
+            }
+        }
     }
-    return L;
 }
 
 // Length of the key run starting at sorted position j.
@@ -754,7 +798,7 @@ __device__ __forceinline__ uint32_t run_len(const uint64_t* keys, uint32_t n, ui
 // S.payload).  MAT: runs[g] += S tuples matching the key run starting at
 // sorted R position g; chunk_vol[rchunk] += pairs of the task.
 template <bool MAT, bool PRESORTED>
-__global__ __launch_bounds__(kSm64Block) void k_count64(
+__global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6))) void k_count64(
     const ulonglong2* __restrict__ Rs, ulonglong2* __restrict__ Rout,
     const ulonglong2* __restrict__ Sb, const SmTask* __restrict__ tasks,
     const uint32_t* __restrict__ ntasks_dev, unsigned long long* __restrict__ runs,
@@ -782,26 +826,46 @@ __global__ __launch_bounds__(kSm64Block) void k_count64(
         if (threadIdx.x == 0) tm = 0;
         __syncthreads();
         uint64_t mt = 0;
-        for (uint32_t j = threadIdx.x; j < tk.s_len; j += kSm64Block) {
-            const ulonglong2 s = Sb[tk.s_off + j];
-            const uint32_t lo = lower_tuple(L.a, n, s.x);
-            const bool hit = lo < n && L.a[lo].x == s.x;
-            if (hit) {
-                for (uint32_t e = lo; e < n && L.a[e].x == s.x; e++) {
-                    mt++;
-                    cks += L.a[e].y + s.y;
-                }
+        constexpr int G = 4;  // S tuples searched in lockstep per thread
+        for (uint32_t j0 = 0; j0 < tk.s_len; j0 += G * kSm64Block) {
+            uint64_t sk[G], sp[G];
+            bool ok[G];
+            uint32_t lo[G];
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                const uint32_t j = j0 + q * kSm64Block + threadIdx.x;
+                ok[q] = j < tk.s_len;
+                const ulonglong2 sv = ok[q] ? Sb[tk.s_off + j] : make_ulonglong2(0, 0);
+                sk[q] = sv.x;
+                sp[q] = sv.y;
             }
-            if (MAT) agg_add(L.cnt, hit ? lo : kNone);
+            lower_batch<G>([&](uint32_t m) { return (uint64_t)L.a[m].x; }, n, sk, ok, lo);
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                const bool hit = lo[q] < n && L.a[lo[q]].x == sk[q];
+                if (hit) {
+                    for (uint32_t e = lo[q]; e < n && L.a[e].x == sk[q]; e++) {
+                        mt++;
+                        cks += L.a[e].y + sp[q];
+                    }
+                }
+                if (MAT) agg_add(L.cnt, hit ? lo[q] : kNone);
+            }
         }
         matches += mt;
         if (MAT) {
             mt = wave_reduce_u64(mt);
             if (lane_id() == 0 && mt) atomicAdd(&tm, (unsigned long long)mt);
             __syncthreads();
-            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block)
-                if (L.cnt[j]) atomicAdd(&runs[tk.r_off + j], (unsigned long long)L.cnt[j]);
-            if (threadIdx.x == 0 && tm) atomicAdd(&chunk_vol[tk.rchunk], tm);
+            // a chunk probed by this task alone is written plainly
+            if (tk.multi) {
+                for (uint32_t j = threadIdx.x; j < n; j += kSm64Block)
+                    if (L.cnt[j]) atomicAdd(&runs[tk.r_off + j], (unsigned long long)L.cnt[j]);
+                if (threadIdx.x == 0 && tm) atomicAdd(&chunk_vol[tk.rchunk], tm);
+            } else {  // every counter of the chunk, zeros included (no prior clear)
+                for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) runs[tk.r_off + j] = L.cnt[j];
+                if (threadIdx.x == 0) chunk_vol[tk.rchunk] = tm;
+            }
         }
         __syncthreads();
     }
@@ -823,30 +887,80 @@ __global__ __launch_bounds__(kSm64Block) void k_count64(
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_u64(unsigned long long* __restrict__ a, uint32_t n,
-                                                   unsigned long long* __restrict__ total) {
-    __shared__ unsigned long long scratch[1024 / TBJ_WAVE + 1];
-    const uint32_t per = (n + 1023) / 1024;
-    const uint32_t beg = min(n, threadIdx.x * per), end = min(n, beg + per);
+// Device-wide exclusive scan of u64 values in three launches: per-block
+// sums of 8192-value chunks, one scan of those sums (*total = grand total),
+// then each block scans its chunk from its carry.
+constexpr uint32_t kScan64Chunk = 1024 * 8;
+
+__global__ __launch_bounds__(1024) void k_scan64_reduce(const unsigned long long* __restrict__ a,
+                                                        uint32_t n,
+                                                        unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long red[1024 / TBJ_WAVE];
+    const uint64_t base = (uint64_t)blockIdx.x * kScan64Chunk;
     unsigned long long s = 0;
-    for (uint32_t i = beg; i < end; i++) s += a[i];
-    unsigned long long tot;
-    unsigned long long run = block_exclusive_scan_u64<1024>(s, scratch, &tot);
-    for (uint32_t i = beg; i < end; i++) {
-        const unsigned long long c = a[i];
-        a[i] = run;
-        run += c;
+    for (uint32_t i = threadIdx.x; i < kScan64Chunk; i += 1024)
+        if (base + i < n) s += a[base + i];
+    s = wave_reduce_u64(s);
+    if (lane_id() == 0) red[wave_id()] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < 1024 / TBJ_WAVE; w++) t += red[w];
+        part[blockIdx.x] = t;
     }
-    if (threadIdx.x == 0) *total = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan64_partials(unsigned long long* __restrict__ part,
+                                                          uint32_t nb,
+                                                          unsigned long long* __restrict__ total) {
+    __shared__ unsigned long long scratch[1024 / TBJ_WAVE + 1];
+    unsigned long long carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += 1024) {
+        const uint32_t i = b0 + threadIdx.x;
+        const unsigned long long v = i < nb ? part[i] : 0;
+        unsigned long long tot;
+        const unsigned long long ex = block_exclusive_scan_u64<1024>(v, scratch, &tot);
+        if (i < nb) part[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ __launch_bounds__(1024) void k_scan64_down(unsigned long long* __restrict__ a,
+                                                      uint32_t n,
+                                                      const unsigned long long* __restrict__ part) {
+    __shared__ unsigned long long scratch[1024 / TBJ_WAVE + 1];
+    __shared__ unsigned long long tile[kScan64Chunk];
+    const uint64_t base = (uint64_t)blockIdx.x * kScan64Chunk;
+    for (uint32_t i = threadIdx.x; i < kScan64Chunk; i += 1024)
+        tile[i] = base + i < n ? a[base + i] : 0;
+    __syncthreads();
+    unsigned long long v[8], s = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        v[q] = tile[threadIdx.x * 8 + q];
+        s += v[q];
+    }
+    unsigned long long tot;
+    unsigned long long run = part[blockIdx.x] + block_exclusive_scan_u64<1024>(s, scratch, &tot);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        tile[threadIdx.x * 8 + q] = run;
+        run += v[q];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kScan64Chunk; i += 1024)
+        if (base + i < n) a[base + i] = tile[i];
 }
 
 // Per chunk with output: runs[g] = chunk_off + sum of count x run length over
-// the runs before g, for the run starts g with a nonzero count.
+// the runs before g, for the run starts g with a nonzero count.  Keys are
+// read only where a run starts (count > 0), to measure its length.
 __global__ __launch_bounds__(kSm64Block) void k_run_offsets64(
     const ulonglong2* __restrict__ Rsorted, const SmTask* __restrict__ rtasks, uint32_t nrtasks,
     const unsigned long long* __restrict__ chunk_off, const unsigned long long* __restrict__ total,
     unsigned long long* __restrict__ runs) {
-    __shared__ uint64_t rk[kSm64Cap];
     __shared__ unsigned long long scratch[kSm64Block / TBJ_WAVE + 1];
     constexpr uint32_t kPer = kSm64Cap / kSm64Block;
     for (uint32_t c = blockIdx.x; c < nrtasks; c += gridDim.x) {
@@ -855,15 +969,20 @@ __global__ __launch_bounds__(kSm64Block) void k_run_offsets64(
         if (end == base) continue;  // no output: the chunk may be unsorted
         const SmTask tk = rtasks[c];
         const uint32_t n = tk.r_len;
-        for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) rk[j] = Rsorted[tk.r_off + j].x;
-        __syncthreads();
+        const ulonglong2* Rc = Rsorted + tk.r_off;
         uint32_t cnt[kPer];
         unsigned long long vol[kPer], vsum = 0;
 #pragma unroll
         for (uint32_t q = 0; q < kPer; q++) {
             const uint32_t j = threadIdx.x * kPer + q;
             cnt[q] = j < n ? (uint32_t)runs[tk.r_off + j] : 0;
-            vol[q] = cnt[q] ? (unsigned long long)cnt[q] * run_len(rk, n, j) : 0;
+            vol[q] = 0;
+            if (cnt[q]) {
+                const uint64_t k = Rc[j].x;
+                uint32_t len = 1;
+                while (j + len < n && Rc[j + len].x == k) len++;
+                vol[q] = (unsigned long long)cnt[q] * len;
+            }
             vsum += vol[q];
         }
         unsigned long long tot;
@@ -874,6 +993,17 @@ __global__ __launch_bounds__(kSm64Block) void k_run_offsets64(
             run += vol[q];
         }
         __syncthreads();
+    }
+}
+
+// Clears the run counters of the R chunks that several tasks share (the
+// chunk's first task does it, before any task adds to them).
+__global__ void k_clear_shared_runs(const SmTask* __restrict__ tasks, uint32_t ntasks,
+                                    unsigned long long* __restrict__ runs) {
+    for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
+        const SmTask tk = tasks[t];
+        if (!(tk.multi && tk.first)) continue;
+        for (uint32_t j = threadIdx.x; j < tk.r_len; j += blockDim.x) runs[tk.r_off + j] = 0;
     }
 }
 
@@ -898,25 +1028,32 @@ __global__ __launch_bounds__(kSm64Block) void k_emit64(
         __syncthreads();
         uint64_t sp[kSm64SItems];
         uint32_t slo[kSm64SItems], srank[kSm64SItems];
+        {
+            uint64_t sk[kSm64SItems];
+            bool ok[kSm64SItems];
 #pragma unroll
-        for (uint32_t k = 0; k < kSm64SItems; k++) {
-            const uint32_t j = k * kSm64Block + threadIdx.x;
-            slo[k] = kNone;
-            sp[k] = 0;
-            if (j < tk.s_len) {
-                const ulonglong2 s = Sb[tk.s_off + j];
-                const uint32_t lo = lower64(rk, n, s.x);
-                if (lo < n && rk[lo] == s.x) slo[k] = lo;
+            for (uint32_t k = 0; k < kSm64SItems; k++) {
+                const uint32_t j = k * kSm64Block + threadIdx.x;
+                ok[k] = j < tk.s_len;
+                const ulonglong2 s = ok[k] ? Sb[tk.s_off + j] : make_ulonglong2(0, 0);
+                sk[k] = s.x;
                 sp[k] = s.y;
             }
-            srank[k] = agg_inc(lcnt, slo[k]);
+            lower_batch<kSm64SItems>([&](uint32_t m) { return rk[m]; }, n, sk, ok, slo);
+#pragma unroll
+            for (uint32_t k = 0; k < kSm64SItems; k++) {
+                if (!(slo[k] < n && rk[slo[k]] == sk[k])) slo[k] = kNone;
+                srank[k] = agg_inc(lcnt, slo[k]);
+            }
         }
         __syncthreads();
+        // a chunk probed by this task alone owns its runs' slots outright
         for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
             const uint32_t c = lcnt[j];
             if (c)
-                lbase[j] = atomicAdd(&runs[tk.r_off + j],
-                                     (unsigned long long)c * run_len(rk, n, j));
+                lbase[j] = tk.multi ? atomicAdd(&runs[tk.r_off + j],
+                                                (unsigned long long)c * run_len(rk, n, j))
+                                    : runs[tk.r_off + j];
         }
         __syncthreads();
 #pragma unroll
@@ -992,9 +1129,12 @@ void sm64_task_counts(hipStream_t st, const uint2* bounds, uint32_t nparts, uint
 }
 
 void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const uint32_t* roff,
-                     const uint32_t* joff, SmTask* rtasks, SmTask* jtasks) {
-    hipLaunchKernelGGL(k_fill_tasks64, dim3((nparts + 255) / 256), dim3(256), 0, st, bounds,
-                       nparts, roff, joff, rtasks, jtasks);
+                     const uint32_t* joff, uint32_t nrtasks, uint32_t njtasks, SmTask* rtasks,
+                     SmTask* jtasks) {
+    const uint64_t n = (uint64_t)nrtasks + njtasks;
+    if (n)
+        hipLaunchKernelGGL(k_fill_tasks64, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                           bounds, nparts, roff, joff, nrtasks, njtasks, rtasks, jtasks);
 }
 
 void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
@@ -1036,8 +1176,22 @@ void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const u
                            ntasks_dev, runs, chunk_vol, counters);
 }
 
-void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* total) {
-    hipLaunchKernelGGL(k_scan_u64, dim3(1), dim3(1024), 0, st, a, n, total);
+void sm64_clear_shared_runs(hipStream_t st, const SmTask* tasks, uint32_t ntasks,
+                            unsigned long long* runs) {
+    if (ntasks)
+        hipLaunchKernelGGL(k_clear_shared_runs, dim3(std::min<uint32_t>(ntasks, 4096)), dim3(256),
+                           0, st, tasks, ntasks, runs);
+}
+
+uint32_t sm64_scan_partials(uint32_t n) { return (n + kScan64Chunk - 1) / kScan64Chunk + 1; }
+
+void sm64_scan_u64(hipStream_t st, unsigned long long* a, uint32_t n, unsigned long long* part,
+                   unsigned long long* total) {
+    const uint32_t nb = (n + kScan64Chunk - 1) / kScan64Chunk;
+    if (!nb) return;
+    hipLaunchKernelGGL(k_scan64_reduce, dim3(nb), dim3(1024), 0, st, a, n, part);
+    hipLaunchKernelGGL(k_scan64_partials, dim3(1), dim3(1024), 0, st, part, nb, total);
+    hipLaunchKernelGGL(k_scan64_down, dim3(nb), dim3(1024), 0, st, a, n, part);
 }
 
 void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,

@@ -190,7 +190,7 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
     SmTask *rtasks = nullptr, *jtasks = nullptr;
     TBJ_TRY(W.get("s64_rtasks", sizeof(SmTask) * std::max<uint32_t>(1, h.nrtasks), (void**)&rtasks));
     TBJ_TRY(W.get("s64_jtasks", sizeof(SmTask) * std::max<uint32_t>(1, h.njtasks), (void**)&jtasks));
-    sm64_fill_tasks(st, bounds, nparts, rcnt, jcnt, rtasks, jtasks);
+    sm64_fill_tasks(st, bounds, nparts, rcnt, jcnt, h.nrtasks, h.njtasks, rtasks, jtasks);
 
     // ---- R partitions above one chunk: sort the chunks, then merge passes.
     // Otherwise the count kernel sorts each chunk in LDS and its first task
@@ -235,14 +235,17 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
         TBJ_TRY(W.get("s64_runs", sizeof(unsigned long long) * nR, (void**)&runs));
         TBJ_TRY(W.get("s64_chunks", sizeof(unsigned long long) * std::max<uint32_t>(1, h.nrtasks),
                       (void**)&chunk_vol));
-        sm64_fill_u64(st, runs, nR, 0);
+        sm64_clear_shared_runs(st, jtasks, h.njtasks, runs);
         sm64_fill_u64(st, chunk_vol, h.nrtasks, 0);
     }
     if (h.njtasks)
         sm64_count(st, jgrid, mat, presorted, Pb, Tmp, Pb, jtasks, &sm->njtasks, runs, chunk_vol,
                    sm->counters);
     if (mat && h.njtasks) {
-        sm64_scan_u64(st, chunk_vol, h.nrtasks, &sm->volume);
+        unsigned long long* vpart = nullptr;
+        TBJ_TRY(W.get("s64_vpart", sizeof(unsigned long long) * sm64_scan_partials(h.nrtasks),
+                      (void**)&vpart));
+        sm64_scan_u64(st, chunk_vol, h.nrtasks, vpart, &sm->volume);
         const uint32_t ogrid = std::max<uint32_t>(1, std::min<uint32_t>(h.nrtasks, C.num_cus * 4));
         sm64_run_offsets(st, ogrid, Rsorted, rtasks, h.nrtasks, chunk_vol, &sm->volume, runs);
         sm64_emit(st, jgrid, Rsorted, Pb, jtasks, &sm->njtasks, runs, (ulonglong2*)pairs,

@@ -42,10 +42,46 @@ struct TaskView {
     uint32_t r_off, r_len, s_off, s_len, join;
 };
 
+// Decodes task t (all 64 lanes of one wave): a 64-ary search for the last
+// partition q with task_off[q] <= t (each step one load per lane, so
+// log64(NP) round trips), then the task's R and S chunks inside q.  Lane 0
+// returns the view; join = ~0 past the last task.
+__device__ __forceinline__ TaskView decode_task(uint32_t t, uint32_t ntasks,
+                                                const uint32_t* __restrict__ task_off, uint32_t NP,
+                                                uint32_t P, const uint2* __restrict__ bounds,
+                                                uint32_t s_chunk,
+                                                const uint32_t* __restrict__ join_ids) {
+    TaskView v{};
+    v.join = 0xFFFFFFFFu;
+    if (t >= ntasks) return v;
+    const uint32_t lane = lane_id();
+    uint32_t lo = 0, hi = NP;
+    while (hi - lo > 1) {
+        const uint32_t step = (hi - lo + TBJ_WAVE - 1) / TBJ_WAVE;
+        const uint32_t idx = lo + lane * step;
+        const bool le = idx < hi && task_off[idx] <= t;
+        const uint64_t m = __ballot(le);  // lanes 0..L hold (task_off is non-decreasing)
+        const uint32_t L = 63 - __clzll((unsigned long long)m);
+        const uint32_t nlo = lo + L * step;
+        hi = min(hi, nlo + step);
+        lo = nlo;
+    }
+    const uint2 r = bounds[lo], s = bounds[NP + lo];
+    const uint32_t nsc = (s.y - s.x + s_chunk - 1) / s_chunk;
+    const uint32_t li = t - task_off[lo];
+    v.r_off = r.x + (li / nsc) * kJoinCap;
+    v.r_len = min((uint32_t)kJoinCap, r.y - v.r_off);
+    v.s_off = s.x + (li % nsc) * s_chunk;
+    v.s_len = min(s_chunk, s.y - v.s_off);
+    v.join = join_ids[lo / P];
+    return v;
+}
+
 // Persistent count-only join.  Task t lies in the partition q with
 // task_off[q] <= t < task_off[q + 1]; inside it, task li covers R chunk
 // li / nsc and S chunk li % nsc (nsc = S chunks of the partition).
-// counts[join_ids[q / P]] += matches of the task.
+// counts[join_ids[q / P]] += matches of the task.  Wave 0 claims and decodes
+// the next task while the current one runs.
 __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_bjoin(
     const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, const uint2* __restrict__ bounds,
     uint32_t NP, uint32_t P, const uint32_t* __restrict__ task_off,
@@ -59,34 +95,20 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ TaskView tv;
     __shared__ unsigned long long tm;
     const uint32_t ntasks = *ntasks_dev;
+    const bool w0 = threadIdx.x < TBJ_WAVE;
     for (uint32_t i = threadIdx.x; i < kJoinCap; i += kJoinBlock) head[i] = 0;
-    if (threadIdx.x == 0) tv.r_len = atomicAdd(task_ctr, 1u);  // task index until decoded
-    __syncthreads();
-    for (;;) {
+    if (w0) {
+        uint32_t t = 0;
+        if (threadIdx.x == 0) t = atomicAdd(task_ctr, 1u);
+        t = __shfl(t, 0, TBJ_WAVE);
+        const TaskView v = decode_task(t, ntasks, task_off, NP, P, bounds, s_chunk, join_ids);
         if (threadIdx.x == 0) {
-            const uint32_t t = tv.r_len;
-            TaskView v{};
-            v.join = 0xFFFFFFFFu;
-            if (t < ntasks) {
-                uint32_t lo = 0, hi = NP;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (task_off[mid] <= t) lo = mid;
-                    else hi = mid;
-                }
-                const uint2 r = bounds[lo], s = bounds[NP + lo];
-                const uint32_t nsc = (s.y - s.x + s_chunk - 1) / s_chunk;
-                const uint32_t li = t - task_off[lo];
-                v.r_off = r.x + (li / nsc) * kJoinCap;
-                v.r_len = min((uint32_t)kJoinCap, r.y - v.r_off);
-                v.s_off = s.x + (li % nsc) * s_chunk;
-                v.s_len = min(s_chunk, s.y - v.s_off);
-                v.join = join_ids[lo / P];
-            }
             tv = v;
             tm = 0;
         }
-        __syncthreads();
+    }
+    __syncthreads();
+    for (;;) {
         const TaskView tk = tv;
         if (tk.join == 0xFFFFFFFFu) break;
         const uint32_t n = tk.r_len;
@@ -111,15 +133,44 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
             }
         }
         __syncthreads();
+        // chains of G S tuples walked in lockstep: G independent LDS reads per step
         uint32_t c = 0;
+        constexpr int G = 8;
 #pragma unroll
-        for (int k = 0; k < SITEMS; k++) {
-            const uint32_t j = k * kJoinBlock + threadIdx.x;
-            if (j < tk.s_len) {
-                const uint32_t key = sv[k].x;
-                for (uint32_t e = head[(key >> hshift) & hmask]; e; e = nxt[e - 1])
-                    c += rkey[e - 1] == key;
+        for (int k0 = 0; k0 < SITEMS; k0 += G) {
+            uint32_t e[G];
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+                const uint32_t j = (k0 + q) * kJoinBlock + threadIdx.x;
+                e[q] = j < tk.s_len ? head[(sv[k0 + q].x >> hshift) & hmask] : 0;
             }
+            for (;;) {
+                uint32_t rk[G], nx[G];
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < G; q++) {
+                    const uint32_t i = e[q] ? e[q] - 1 : 0;
+                    rk[q] = rkey[i];
+                    nx[q] = nxt[i];
+                }
+#pragma unroll
+                for (int q = 0; q < G; q++) {
+                    if (e[q]) {
+                        c += rk[q] == sv[k0 + q].x;
+                        e[q] = nx[q];
+                        any |= e[q] != 0;
+                    }
+                }
+                if (!any) break;
+            }
+        }
+        // the next task, decoded by wave 0 while the other waves finish probing
+        TaskView nv{};
+        if (w0) {
+            uint32_t t = 0;
+            if (threadIdx.x == 0) t = atomicAdd(task_ctr, 1u);
+            t = __shfl(t, 0, TBJ_WAVE);
+            nv = decode_task(t, ntasks, task_off, NP, P, bounds, s_chunk, join_ids);
         }
         const uint64_t wc = wave_reduce_u64(c);
         if (lane_id() == 0 && wc) atomicAdd(&tm, (unsigned long long)wc);
@@ -127,7 +178,8 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
         for (uint32_t i = threadIdx.x; i <= hmask; i += kJoinBlock) head[i] = 0;
         if (threadIdx.x == 0) {
             if (tm) atomicAdd(&counts[tk.join], tm);
-            tv.r_len = atomicAdd(task_ctr, 1u);
+            tv = nv;
+            tm = 0;
         }
         __syncthreads();
     }

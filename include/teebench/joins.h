@@ -34,11 +34,18 @@ result_t* CHT(relation_t* relR, relation_t* relS, int nthreads);
 result_t* RSM(relation_t* relR, relation_t* relS, int nthreads);
 result_t* PSM(relation_t* relR, relation_t* relS, int nthreads);
 
-/* Runtime switches of this implementation (no reference counterpart). */
+/* Runtime switches of this implementation (no reference counterpart; C
+ * linkage so that FFI callers find them by name). */
+#ifdef __cplusplus
+extern "C" {
+#endif
 void teebench_set_materialize(int on);
 /* 0 = radix (default), 1 = learned CDF range partitioner with radix fallback. */
 void teebench_set_partitioner(int learned);
 void teebench_free_result(result_t* res);
+#ifdef __cplusplus
+}
+#endif
 
 /* The selector table of the plain-CPU harness (native.cpp:34-46) with the
  * GPU implementations; terminated by an entry with an empty name. */

@@ -11,14 +11,38 @@ namespace tbj {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (TBJ_WAVE - 1); }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / TBJ_WAVE; }
 
-// Inclusive prefix sum across the 64 lanes of a wave.
+// Inclusive prefix sum across the 64 lanes of a wave, in DPP moves (no LDS
+// crossbar round trips): shifts by 1, 2, 4, 8 inside each row of 16 lanes
+// (lanes shifted in from outside the row read 0), then row 0's total is
+// broadcast into row 1 and row 2's into row 3 (row_bcast:15, rows 1 and 3
+// written), then the total of rows 0-1 into rows 2 and 3 (row_bcast:31).
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < TBJ_WAVE; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, TBJ_WAVE);
-        if (lane >= (uint32_t)d) x += y;
-    }
+    constexpr int kRowShr = 0x110, kBcast15 = 0x142, kBcast31 = 0x143;
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 1, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 2, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 4, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 8, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kBcast15, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kBcast31, 0xc, 0xf, false);
+    return x;
+}
+
+// One DPP step of a 64-bit value (both halves moved alike).
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t x) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)x, CTRL, ROWS, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(x >> 32), CTRL, ROWS, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// 64-bit inclusive prefix sum across a wave, the same DPP steps as above.
+__device__ __forceinline__ uint64_t wave_inclusive_scan_u64(uint64_t x) {
+    x += dpp_u64<0x111, 0xf>(x);
+    x += dpp_u64<0x112, 0xf>(x);
+    x += dpp_u64<0x114, 0xf>(x);
+    x += dpp_u64<0x118, 0xf>(x);
+    x += dpp_u64<0x142, 0xa>(x);
+    x += dpp_u64<0x143, 0xc>(x);
     return x;
 }
 

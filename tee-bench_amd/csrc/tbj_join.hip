@@ -203,20 +203,97 @@ __device__ __forceinline__ void lds_count_batch(const uint2 (&sv)[SI], int k0, u
     }
 }
 
-// One task per loop trip: the R range and the first S sub-chunk are loaded
-// into registers together (one memory phase), the R range is built into LDS
-// once, and the task's S range is probed in sub-chunks of SITEMS *
-// kJoinBlock tuples (fewer with MAT, whose per-item output slots need
-// registers); with MAT the block claims one output range per sub-chunk.
-template <int MODE, bool MAT, int SITEMS = MAT ? kJoinSItems / 2 : kJoinSItems>
-__global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join(const uint2* __restrict__ Rb,
-                                                     const uint2* __restrict__ Sb,
-                                                     const JoinTask* __restrict__ tasks,
-                                                     const uint32_t* __restrict__ ntasks_dev,
-                                                     uint32_t* __restrict__ task_ctr,
-                                                     uint32_t hshift, uint2* __restrict__ pairs,
-                                                     uint64_t cap,
-                                                     unsigned long long* __restrict__ counters) {
+// Loads tuples first + k * kJoinBlock + threadIdx.x (k < N) of a range of
+// len >= 1 tuples; positions past the end re-read the last tuple, so every
+// lane issues the same loads and no load waits on a branch.
+template <int N>
+__device__ __forceinline__ void load_range(uint2 (&v)[N], const uint2* __restrict__ base,
+                                           uint32_t off, uint32_t len, uint32_t first) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const uint32_t j = first + k * kJoinBlock + threadIdx.x;
+        v[k] = base[off + min(j, len - 1)];
+    }
+}
+
+// Builds the LDS table of an R range of n tuples (rv[k] = tuple k *
+// kJoinBlock + threadIdx.x) into zeroed heads.  kModeChain: chains through
+// nxt (1-based, 0 ends a chain); kModeHist: R reordered by bucket, head[b]
+// = first tuple of bucket b.  The caller's barrier completes the table.
+template <int MODE, int RITEMS>
+__device__ __forceinline__ void lds_build(const uint2 (&rv)[RITEMS], uint32_t n, uint32_t hshift,
+                                          uint32_t hmask, uint32_t nb, uint32_t* rkey,
+                                          uint32_t* rpay, uint32_t* head, uint16_t* nxt,
+                                          uint32_t* scratch) {
+    if (MODE == kModeChain) {
+#pragma unroll
+        for (int k = 0; k < RITEMS; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < n) {
+                rkey[j] = rv[k].x;
+                rpay[j] = rv[k].y;
+                nxt[j] = (uint16_t)atomicExch(&head[(rv[k].x >> hshift) & hmask], j + 1);
+            }
+        }
+    } else {
+        uint32_t rk[RITEMS];
+#pragma unroll
+        for (int k = 0; k < RITEMS; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < n) rk[k] = atomicAdd(&head[(rv[k].x >> hshift) & hmask], 1u);
+        }
+        __syncthreads();
+        block_scan_lds<kJoinBlock>(head, nb + 1, scratch);
+#pragma unroll
+        for (int k = 0; k < RITEMS; k++) {
+            const uint32_t j = k * kJoinBlock + threadIdx.x;
+            if (j < n) {
+                const uint32_t pos = head[(rv[k].x >> hshift) & hmask] + rk[k];
+                rkey[pos] = rv[k].x;
+                rpay[pos] = rv[k].y;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t table_buckets(int mode, uint32_t n) {
+    if (mode == kModeChain) return next_pow2_u32(n);
+    const uint32_t nb = next_pow2_u32(n) >> 1;
+    return nb ? nb : 1;
+}
+
+// Block reduction of (matches, checksum) into counters[0..1].
+__device__ __forceinline__ void flush_counters(uint64_t matches, uint64_t cks,
+                                               unsigned long long* red_m,
+                                               unsigned long long* red_c,
+                                               unsigned long long* __restrict__ counters) {
+    matches = wave_reduce_u64(matches);
+    cks = wave_reduce_u64(cks);
+    if (lane_id() == 0) {
+        red_m[wave_id()] = matches;
+        red_c[wave_id()] = cks;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0, c = 0;
+        for (int w = 0; w < kJoinBlock / TBJ_WAVE; w++) {
+            m += red_m[w];
+            c += red_c[w];
+        }
+        if (m) atomicAdd(&counters[0], m);
+        if (c) atomicAdd(&counters[1], c);
+    }
+}
+
+// Counting join, one task per loop trip: the R range and the first S
+// sub-chunk are loaded together (one memory phase), the R range is built into
+// LDS once, and the task's S range is probed in sub-chunks of SITEMS *
+// kJoinBlock tuples.
+template <int MODE, int SITEMS = kJoinSItems>
+__global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join(
+    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, const JoinTask* __restrict__ tasks,
+    const uint32_t* __restrict__ ntasks_dev, uint32_t* __restrict__ task_ctr, uint32_t hshift,
+    unsigned long long* __restrict__ counters) {
     constexpr int RITEMS = kJoinCap / kJoinBlock;
     constexpr int NHEAD = MODE == kModeChain ? kJoinCap : kJoinCap / 2 + 1;
     __shared__ uint32_t rkey[kJoinCap];
@@ -225,12 +302,7 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ uint16_t nxt[MODE == kModeChain ? kJoinCap : 1];
     __shared__ uint32_t scratch[kJoinBlock / TBJ_WAVE + 1];
     __shared__ uint32_t task_s;
-    __shared__ unsigned long long obase_s;
     __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
-    constexpr int NCLAIM = SITEMS > 8 ? 2 : 1;  // output claims per task
-    constexpr int HALF = SITEMS / NCLAIM;       // items per output claim
-    __shared__ uint32_t wtot[kJoinBlock / TBJ_WAVE][HALF];
-
     __shared__ JoinTask tk_s;
 
     const uint32_t ntasks = *ntasks_dev;
@@ -251,140 +323,25 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
         uint32_t next_t = 0;
         JoinTask nd{};
         if (threadIdx.x == 0) next_t = atomicAdd(task_ctr, 1u);
-        // ---- loads: R and S of the task in flight together
         uint2 rv[RITEMS], sv[SITEMS];
-#pragma unroll
-        for (int k = 0; k < RITEMS; k++) {
-            const uint32_t j = k * kJoinBlock + threadIdx.x;
-            if (j < n) rv[k] = Rb[tk.r_off + j];
-        }
-#pragma unroll
-        for (int k = 0; k < SITEMS; k++) {
-            const uint32_t j = k * kJoinBlock + threadIdx.x;
-            if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
-        }
-        uint32_t nb;
-        if (MODE == kModeChain) {
-            nb = next_pow2_u32(n);
-        } else {
-            nb = next_pow2_u32(n) >> 1;
-            if (nb == 0) nb = 1;
-        }
+        load_range(rv, Rb, tk.r_off, n, 0);
+        load_range(sv, Sb, tk.s_off, tk.s_len, 0);
+        const uint32_t nb = table_buckets(MODE, n);
         const uint32_t hmask = nb - 1;
-
-        // ---- build (heads were cleared at the end of the previous task)
-        if (MODE == kModeChain) {
-#pragma unroll
-            for (int k = 0; k < RITEMS; k++) {
-                const uint32_t j = k * kJoinBlock + threadIdx.x;
-                if (j < n) {
-                    rkey[j] = rv[k].x;
-                    rpay[j] = rv[k].y;
-                    nxt[j] = (uint16_t)atomicExch(&head[(rv[k].x >> hshift) & hmask], j + 1);
-                }
-            }
-            __syncthreads();
-        } else {
-            uint32_t rk[RITEMS];
-#pragma unroll
-            for (int k = 0; k < RITEMS; k++) {
-                const uint32_t j = k * kJoinBlock + threadIdx.x;
-                if (j < n) rk[k] = atomicAdd(&head[(rv[k].x >> hshift) & hmask], 1u);
-            }
-            __syncthreads();
-            block_scan_lds<kJoinBlock>(head, nb + 1, scratch);
-#pragma unroll
-            for (int k = 0; k < RITEMS; k++) {
-                const uint32_t j = k * kJoinBlock + threadIdx.x;
-                if (j < n) {
-                    const uint32_t pos = head[(rv[k].x >> hshift) & hmask] + rk[k];
-                    rkey[pos] = rv[k].x;
-                    rpay[pos] = rv[k].y;
-                }
-            }
-            __syncthreads();
-        }
+        lds_build<MODE>(rv, n, hshift, hmask, nb, rkey, rpay, head, nxt, scratch);
+        __syncthreads();
 
         if (threadIdx.x == 0 && next_t < ntasks) nd = tasks[next_t];
-        // ---- probe the S range in sub-chunks of SITEMS * kJoinBlock tuples
-        // against the one build (sub-chunk 0 was loaded with R)
         for (uint32_t s0 = 0; s0 < tk.s_len; s0 += SITEMS * kJoinBlock) {
-            if (s0) {
+            if (s0) load_range(sv, Sb, tk.s_off, tk.s_len, s0);
+            constexpr int G = MODE == kModeChain ? 8 : 4;
 #pragma unroll
-                for (int k = 0; k < SITEMS; k++) {
-                    const uint32_t j = s0 + k * kJoinBlock + threadIdx.x;
-                    if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
-                }
-            }
-            // ---- probe
-            if (!MAT) {
-                constexpr int G = MODE == kModeChain ? 8 : 4;
+            for (int k0 = 0; k0 < SITEMS; k0 += G) {
+                uint32_t ck[G], fp[G];
+                lds_count_batch<MODE, G>(sv, k0, s0, tk.s_len, hshift, hmask, rkey, rpay, head,
+                                         nxt, ck, fp, cks);
 #pragma unroll
-                for (int k0 = 0; k0 < SITEMS; k0 += G) {
-                    uint32_t ck[G], fp[G];
-                    lds_count_batch<MODE, G>(sv, k0, s0, tk.s_len, hshift, hmask, rkey, rpay, head,
-                                             nxt, ck, fp, cks);
-#pragma unroll
-                    for (int q = 0; q < G; q++) matches += ck[q];
-                }
-            } else {
-                // NCLAIM groups of HALF items: count (remembering the first
-                // match), claim one output range per group, write.  Slots go in
-                // item-major order (all threads' item 0, then item 1, ...) so that
-                // a wave's stores of one item are contiguous.
-                const uint32_t lane = lane_id(), w = wave_id();
-#pragma unroll
-                for (int hh = 0; hh < NCLAIM; hh++) {
-                    uint32_t ck[HALF], fp[HALF];
-#pragma unroll
-                    for (int g0 = 0; g0 < HALF; g0 += 4) {
-                        uint32_t c4[4], f4[4];
-                        lds_count_batch<MODE, 4>(sv, hh * HALF + g0, s0, tk.s_len, hshift, hmask,
-                                                 rkey, rpay, head, nxt, c4, f4, cks);
-#pragma unroll
-                        for (int q = 0; q < 4; q++) {
-                            ck[g0 + q] = c4[q];
-                            fp[g0 + q] = f4[q];
-                        }
-                    }
-#pragma unroll
-                    for (int q = 0; q < HALF; q++) {
-                        matches += ck[q];
-                        const uint32_t incl = wave_inclusive_scan(ck[q]);
-                        if (lane == TBJ_WAVE - 1) wtot[w][q] = incl;
-                    }
-                    __syncthreads();
-                    if (threadIdx.x == 0) {
-                        uint32_t tot = 0;
-                        for (int q = 0; q < HALF; q++)
-                            for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) tot += wtot[ww][q];
-                        obase_s = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
-                    }
-                    __syncthreads();
-                    uint64_t item_base = obase_s;
-#pragma unroll
-                    for (int q = 0; q < HALF; q++) {
-                        const int k = hh * HALF + q;
-                        uint32_t before = 0, all = 0;
-                        for (int ww = 0; ww < kJoinBlock / TBJ_WAVE; ww++) {
-                            const uint32_t v = wtot[ww][q];
-                            before += (uint32_t)ww < w ? v : 0;
-                            all += v;
-                        }
-                        const uint32_t incl = wave_inclusive_scan(ck[q]);
-                        uint64_t ok = item_base + before + incl - ck[q];
-                        if (ck[q] == 1) {
-                            if (ok < cap) pairs[ok] = make_uint2(fp[q], sv[k].y);
-                        } else if (ck[q] > 1) {
-                            uint32_t c2 = 0, f2 = 0;
-                            uint64_t k2 = 0;
-                            lds_probe<MODE, true>(sv[k], hshift, hmask, rkey, rpay, head, nxt, c2, k2,
-                                                  f2, pairs, cap, ok);
-                        }
-                        item_base += all;
-                    }
-                    __syncthreads();
-                }
+                for (int q = 0; q < G; q++) matches += ck[q];
             }
         }
         __syncthreads();
@@ -396,23 +353,151 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
         }
         __syncthreads();
     }
-    // block reduction of the counters
-    matches = wave_reduce_u64(matches);
-    cks = wave_reduce_u64(cks);
-    if (lane_id() == 0) {
-        red_m[wave_id()] = matches;
-        red_c[wave_id()] = cks;
-    }
-    __syncthreads();
+    flush_counters(matches, cks, red_m, red_c, counters);
+}
+
+// Materialising join, software-pipelined: each S sub-chunk (SI * kJoinBlock
+// tuples) is counted, its output range claimed with one atomic, and its
+// pairs written; the loads of the next sub-chunk -- or, after a task's last
+// sub-chunk, of the next task's R range and first sub-chunk -- are issued
+// before the claim is awaited, so they travel while the claim returns and the
+// pairs are written.  Output slots go in item-major order (all threads' item
+// 0, then item 1, ...) so that a wave's stores of one item are contiguous.
+template <int MODE>
+__global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join_mat(
+    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, const JoinTask* __restrict__ tasks,
+    const uint32_t* __restrict__ ntasks_dev, uint32_t* __restrict__ task_ctr, uint32_t hshift,
+    uint2* __restrict__ pairs, uint64_t cap, unsigned long long* __restrict__ counters) {
+    constexpr int SI = kJoinSItems / 2;
+    constexpr int SUB = SI * kJoinBlock;
+    constexpr int RITEMS = kJoinCap / kJoinBlock;
+    constexpr int NHEAD = MODE == kModeChain ? kJoinCap : kJoinCap / 2 + 1;
+    constexpr int NW = kJoinBlock / TBJ_WAVE;
+    __shared__ uint32_t rkey[kJoinCap];
+    __shared__ uint32_t rpay[kJoinCap];
+    __shared__ uint32_t head[NHEAD];
+    __shared__ uint16_t nxt[MODE == kModeChain ? kJoinCap : 1];
+    __shared__ uint32_t scratch[NW + 1];
+    // per-wave match totals of each item, and each wave's output offset for
+    // each item; double-buffered by sub-chunk parity (a wave may count the
+    // next sub-chunk while others still write this one)
+    __shared__ uint32_t wtot[2][NW][SI];
+    __shared__ uint32_t woff[2][NW][SI];
+    __shared__ uint32_t first_s, next_s;
+    __shared__ JoinTask tk_s;
+    __shared__ unsigned long long obase_s;
+    __shared__ unsigned long long red_m[NW], red_c[NW];
+
+    const uint32_t ntasks = *ntasks_dev;
+    const uint32_t lane = lane_id(), w = wave_id();
+    uint64_t matches = 0, cks = 0;
     if (threadIdx.x == 0) {
-        unsigned long long m = 0, c = 0;
-        for (int w = 0; w < kJoinBlock / TBJ_WAVE; w++) {
-            m += red_m[w];
-            c += red_c[w];
-        }
-        if (m) atomicAdd(&counters[0], m);
-        if (c) atomicAdd(&counters[1], c);
+        const uint32_t t = atomicAdd(task_ctr, 1u);
+        first_s = t;
+        if (t < ntasks) tk_s = tasks[t];
     }
+    for (uint32_t i = threadIdx.x; i < NHEAD; i += kJoinBlock) head[i] = 0;
+    __syncthreads();
+    bool live = first_s < ntasks;
+    JoinTask tk = tk_s;
+    uint2 rv[RITEMS], sv[SI];
+    if (live) {
+        load_range(rv, Rb, tk.r_off, tk.r_len, 0);
+        load_range(sv, Sb, tk.s_off, tk.s_len, 0);
+    }
+    uint32_t ph = 0;
+    while (live) {
+        const uint32_t n = tk.r_len;
+        const uint32_t nb = table_buckets(MODE, n);
+        const uint32_t hmask = nb - 1;
+        // the next task is popped during the build; every thread then reads
+        // its descriptor (one broadcast load per wave), which travels while
+        // the first sub-chunk is probed
+        uint32_t nt = 0;
+        if (threadIdx.x == 0) nt = atomicAdd(task_ctr, 1u);
+        lds_build<MODE>(rv, n, hshift, hmask, nb, rkey, rpay, head, nxt, scratch);
+        if (threadIdx.x == 0) next_s = nt;
+        __syncthreads();
+        const uint32_t ntu = next_s;
+        const bool has_next = ntu < ntasks;
+        const JoinTask nd = tasks[has_next ? ntu : 0];
+        for (uint32_t s0 = 0;; s0 += SUB) {
+            uint32_t ck[SI], fp[SI];
+#pragma unroll
+            for (int g0 = 0; g0 < SI; g0 += 4) {
+                uint32_t c4[4], f4[4];
+                lds_count_batch<MODE, 4>(sv, g0, s0, tk.s_len, hshift, hmask, rkey, rpay, head,
+                                         nxt, c4, f4, cks);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    ck[g0 + q] = c4[q];
+                    fp[g0 + q] = f4[q];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < SI; q++) {
+                matches += ck[q];
+                const uint32_t incl = wave_inclusive_scan(ck[q]);
+                if (lane == TBJ_WAVE - 1) wtot[ph][w][q] = incl;
+            }
+            __syncthreads();
+            const bool last = s0 + SUB >= tk.s_len;
+            // the payloads are kept for the writes; an item with several
+            // matches re-reads its tuple (rare: R keys repeat)
+            uint32_t cpay[SI];
+#pragma unroll
+            for (int k = 0; k < SI; k++) cpay[k] = sv[k].y;
+            if (!last) {
+                load_range(sv, Sb, tk.s_off, tk.s_len, s0 + SUB);
+            } else if (has_next) {
+                load_range(sv, Sb, nd.s_off, nd.s_len, 0);
+            }
+            // wave 0: output offsets (item q's slots follow items < q; inside
+            // an item, wave order) and the claim, issued after the loads so
+            // that its round trip overlaps theirs
+            if (w == 0) {
+                uint32_t allq = 0;
+                if (lane < SI)
+                    for (int ww = 0; ww < NW; ww++) allq += wtot[ph][ww][lane];
+                const uint32_t inc = wave_inclusive_scan(allq);
+                const uint32_t tot = __shfl(inc, TBJ_WAVE - 1, TBJ_WAVE);
+                if (lane < SI) {
+                    uint32_t run = inc - allq;
+                    for (int ww = 0; ww < NW; ww++) {
+                        const uint32_t v = wtot[ph][ww][lane];
+                        woff[ph][ww][lane] = run;
+                        run += v;
+                    }
+                }
+                if (lane == 0) obase_s = tot ? atomicAdd(&counters[2], (unsigned long long)tot) : 0;
+            }
+            __syncthreads();
+            const uint64_t obase = obase_s;
+#pragma unroll
+            for (int q = 0; q < SI; q++) {
+                const uint32_t incl = wave_inclusive_scan(ck[q]);
+                uint64_t o = obase + woff[ph][w][q] + incl - ck[q];
+                if (ck[q] == 1) {
+                    if (o < cap) pairs[o] = make_uint2(fp[q], cpay[q]);
+                } else if (ck[q] > 1) {
+                    uint32_t c2 = 0, f2 = 0;
+                    uint64_t k2 = 0;
+                    const uint2 v = Sb[tk.s_off + s0 + q * kJoinBlock + threadIdx.x];
+                    lds_probe<MODE, true>(v, hshift, hmask, rkey, rpay, head, nxt, c2, k2, f2,
+                                          pairs, cap, o);
+                }
+            }
+            ph ^= 1;
+            if (last) break;
+        }
+        if (has_next) load_range(rv, Rb, nd.r_off, nd.r_len, 0);
+        __syncthreads();  // the table is no longer read
+        for (uint32_t i = threadIdx.x; i <= nb && i < (uint32_t)NHEAD; i += kJoinBlock) head[i] = 0;
+        live = has_next;
+        tk = nd;
+        __syncthreads();
+    }
+    flush_counters(matches, cks, red_m, red_c, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,13 +650,17 @@ void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uin
                            nparts, cap, s_chunk, task_off, ntasks, tasks);
 }
 
-template <int MODE, bool MAT>
-static void launch_join_t(hipStream_t st, uint32_t grid, const uint2* Rb, const uint2* Sb,
-                          const JoinTask* tasks, const uint32_t* ntasks_dev, uint32_t* task_ctr,
-                          uint32_t hshift, uint2* pairs, uint64_t cap,
+template <int MODE>
+static void launch_join_t(hipStream_t st, bool mat, uint32_t grid, const uint2* Rb,
+                          const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
+                          uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
                           unsigned long long* counters) {
-    hipLaunchKernelGGL((k_join<MODE, MAT>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
-                       ntasks_dev, task_ctr, hshift, pairs, cap, counters);
+    if (mat)
+        hipLaunchKernelGGL((k_join_mat<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
+                           ntasks_dev, task_ctr, hshift, pairs, cap, counters);
+    else
+        hipLaunchKernelGGL((k_join<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
+                           ntasks_dev, task_ctr, hshift, counters);
 }
 
 void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, const uint2* Rb,
@@ -583,40 +672,30 @@ void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, cons
                            ntasks_dev, task_ctr, pairs, cap, materialize, counters);
         return;
     }
-    if (mode == kModeChain) {
-        if (materialize)
-            launch_join_t<kModeChain, true>(st, grid, Rb, Sb, tasks, ntasks_dev, task_ctr, hshift,
-                                            pairs, cap, counters);
-        else
-            launch_join_t<kModeChain, false>(st, grid, Rb, Sb, tasks, ntasks_dev, task_ctr, hshift,
-                                             pairs, cap, counters);
-    } else {
-        if (materialize)
-            launch_join_t<kModeHist, true>(st, grid, Rb, Sb, tasks, ntasks_dev, task_ctr, hshift,
-                                           pairs, cap, counters);
-        else
-            launch_join_t<kModeHist, false>(st, grid, Rb, Sb, tasks, ntasks_dev, task_ctr, hshift,
-                                            pairs, cap, counters);
-    }
+    if (mode == kModeChain)
+        launch_join_t<kModeChain>(st, materialize, grid, Rb, Sb, tasks, ntasks_dev, task_ctr,
+                                  hshift, pairs, cap, counters);
+    else
+        launch_join_t<kModeHist>(st, materialize, grid, Rb, Sb, tasks, ntasks_dev, task_ctr,
+                                 hshift, pairs, cap, counters);
+}
+
+template <typename K>
+static int blocks_per_cu(K kernel) {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, kernel, kJoinBlock, 0) !=
+            hipSuccess ||
+        blocks < 1)
+        blocks = 1;
+    return blocks;
 }
 
 int join_occupancy(int mode, bool materialize) {
-    int blocks = 0;
-    hipError_t e;
-    if (mode == kModeSort)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k_join_sort, kJoinBlock, 0);
-    else if (mode == kModeChain)
-        e = materialize ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, k_join<kModeChain, true>, kJoinBlock, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, k_join<kModeChain, false>, kJoinBlock, 0);
-    else
-        e = materialize ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, k_join<kModeHist, true>, kJoinBlock, 0)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                              &blocks, k_join<kModeHist, false>, kJoinBlock, 0);
-    if (e != hipSuccess || blocks < 1) blocks = 1;
-    return blocks;
+    if (mode == kModeSort) return blocks_per_cu(k_join_sort);
+    if (mode == kModeChain)
+        return materialize ? blocks_per_cu(k_join_mat<kModeChain>)
+                           : blocks_per_cu(k_join<kModeChain>);
+    return materialize ? blocks_per_cu(k_join_mat<kModeHist>) : blocks_per_cu(k_join<kModeHist>);
 }
 
 }  // namespace tbj

@@ -80,22 +80,12 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan_u64(
     unsigned long long v, unsigned long long* scratch, unsigned long long* total) {
     constexpr int NW = BLOCK / TBJ_WAVE;
     const uint32_t lane = lane_id(), wid = wave_id();
-    unsigned long long inc = v;
-#pragma unroll
-    for (int d = 1; d < TBJ_WAVE; d <<= 1) {
-        const unsigned long long y = __shfl_up(inc, d, TBJ_WAVE);
-        if (lane >= (uint32_t)d) inc += y;
-    }
+    const unsigned long long inc = wave_inclusive_scan_u64(v);
     if (lane == TBJ_WAVE - 1) scratch[wid] = inc;
     __syncthreads();
     if (wid == 0) {
         const unsigned long long w = lane < NW ? scratch[lane] : 0;
-        unsigned long long wi = w;
-#pragma unroll
-        for (int d = 1; d < NW; d <<= 1) {
-            const unsigned long long y = __shfl_up(wi, d, TBJ_WAVE);
-            if (lane >= (uint32_t)d) wi += y;
-        }
+        const unsigned long long wi = wave_inclusive_scan_u64(w);
         if (lane < NW) scratch[lane] = wi - w;
         if (lane == NW - 1) scratch[NW] = wi;
     }

@@ -255,6 +255,13 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(uint32_t* __restrict__
 // stage the tile bin-sorted in LDS, then stream each bin's run to its global
 // offset.  Consecutive lanes write consecutive slots of the same run.
 
+// Dynamic LDS of k_scatter: the learned model (if any), then the bin
+// offsets, sized by the pass's bin count so that a small pass leaves room
+// for a third workgroup per CU.
+__host__ __device__ constexpr size_t scatter_model_bytes(bool learned) {
+    return learned ? (sizeof(LdsModel) + 15) / 16 * 16 : 0;
+}
+
 __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
@@ -265,11 +272,11 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
                                                         BinSpec bs, uint32_t nbins) {
     __shared__ uint2 stage[kTile];
     __shared__ uint16_t sbin[kTile];
-    __shared__ uint32_t loff[kMaxBins];
-    __shared__ uint32_t goff[kMaxBins];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
+    uint32_t* loff = reinterpret_cast<uint32_t*>(dyn_lds + scatter_model_bytes(bs.kind == kBinLearned));
+    uint32_t* goff = loff + nbins;
     BinFn bin = make_binfn(bs, &lm);
     const uint32_t end = min(tile_end, *ntiles_dev);
     for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
@@ -702,9 +709,9 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
                     uint32_t nbins) {
     if (!ntiles) return;
     const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kPartBlock), model_lds(bs),
-                       st, in, out, tiles, segs, ntiles_dev, tile_base, tile_base + ntiles,
-                       scanned, bs, nbins);
+    const size_t lds = scatter_model_bytes(bs.kind == kBinLearned) + 2 * sizeof(uint32_t) * nbins;
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kPartBlock), lds, st, in, out, tiles, segs,
+                       ntiles_dev, tile_base, tile_base + ntiles, scanned, bs, nbins);
 }
 
 uint32_t scan_partials(uint64_t n) { return (uint32_t)((n + kScanChunk - 1) / kScanChunk) + 1; }

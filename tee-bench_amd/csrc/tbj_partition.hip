@@ -86,25 +86,6 @@ __device__ __forceinline__ bool sampled16(uint32_t idx) {
     return ((idx * 2654435761u) >> 28) == 0;
 }
 
-// Slot of a tile in its segment's offset order.  Workgroups are dispatched
-// to the 8 XCDs round-robin by blockIdx.x, so tiles whose block indices are
-// congruent mod 8 share an XCD and its L2.  Ordering a segment's tiles by
-// (XCD, tile) makes each XCD's runs of one partition adjacent in HBM, so a
-// cache line of a partition is filled inside one L2 instead of being split
-// between two XCDs' partial write-backs.
-constexpr uint32_t kNumXcd = 8;
-__device__ __forceinline__ uint32_t xcd_slot(uint32_t tis, uint32_t ntiles, uint32_t block) {
-    const uint32_t base_b = block - tis;  // block index of the segment's first tile
-    const uint32_t g = block % kNumXcd;
-    uint32_t before = 0;
-    for (uint32_t h = 0; h < g; h++) {
-        const uint32_t r0 = (h + kNumXcd - base_b % kNumXcd) % kNumXcd;
-        before += ntiles > r0 ? (ntiles - r0 + kNumXcd - 1) / kNumXcd : 0;
-    }
-    const uint32_t r0g = (g + kNumXcd - base_b % kNumXcd) % kNumXcd;
-    return before + (tis - r0g) / kNumXcd;
-}
-
 // Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
 // ... below tile_end and *ntiles_dev, loading the partition function (the
 // learned model) into LDS once.

@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         atomicMax(&mm[1], bmm[1]);
     }
     const SegDesc sd = segs[td.seg];
-    const uint32_t tis = t - sd.first_tile;
+    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
         hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
 }
@@ -354,7 +354,7 @@ __global__ __launch_bounds__(kSm64Block) void k_scatter64(
     const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
-    const uint32_t tis = t - sd.first_tile;
+    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
         next[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis];
     ulonglong2 v[kSm64Items];
@@ -577,11 +577,12 @@ __global__ void k_fill_tasks64(const uint2* __restrict__ bounds, uint32_t nparts
 }
 
 // ---------------------------------------------------------------------------
-// R chunk sort in LDS.  Buckets: n equal-width key ranges between the
-// chunk's min and max key (a monotone map, so bucket order is key order);
-// each bucket is then insertion-sorted by one thread.  A bucket above
+// R chunk sort in LDS, by (key, payload) like the merge passes that may
+// follow.  Buckets: n equal-width key ranges between the chunk's min and max
+// key (a monotone map, so bucket order is key order); each bucket is then
+// insertion-sorted by one thread.  A bucket above
 // kBucketMax tuples (clustered keys) sends the whole chunk to a bitonic sort
-// instead; a chunk of one key is sorted as it stands.
+// instead, and so does a chunk of one key (ordered by payload there).
 
 constexpr uint32_t kSortItems = kSm64Cap / kSm64Block;
 constexpr uint32_t kBucketMax = 32;
@@ -594,9 +595,48 @@ struct SortLds {
     uint32_t big;
 };
 
+// Bucket stage of chunk_sort for a chunk whose keys span lo .. lo + span
+// (span > 0): leaves L.a[0, n) sorted by (key, payload), or sets L.big when
+// a bucket is too large for its insertion sort.  Every thread must call it.
+__device__ void bucket_sort(const ulonglong2 (&v)[kSortItems], uint32_t n, uint64_t lo,
+                            uint64_t span, SortLds& L) {
+    const uint32_t tid = threadIdx.x;
+    const double scale = (double)n / ((double)span + 1.0);
+    uint32_t bk[kSortItems], rk[kSortItems];
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++) {
+        if (k * kSm64Block + tid < n) {
+            bk[k] = min(n - 1, (uint32_t)((double)(v[k].x - lo) * scale));
+            rk[k] = atomicAdd(&L.cnt[bk[k]], 1u);
+        }
+    }
+    __syncthreads();
+    block_scan_lds<kSm64Block>(L.cnt, n, L.scratch);
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++)
+        if (k * kSm64Block + tid < n) L.a[L.cnt[bk[k]] + rk[k]] = v[k];
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kSm64Block) {
+        const uint32_t s0 = L.cnt[i], s1 = i + 1 < n ? L.cnt[i + 1] : n;
+        if (s1 - s0 > kBucketMax) {
+            L.big = 1;
+        } else {
+            for (uint32_t x = s0 + 1; x < s1; x++) {
+                const ulonglong2 t = L.a[x];
+                uint32_t y = x;
+                while (y > s0 && tuple_less(t, L.a[y - 1])) {
+                    L.a[y] = L.a[y - 1];
+                    y--;
+                }
+                L.a[y] = t;
+            }
+        }
+    }
+}
+
 // v: item k of thread t is tuple k * kSm64Block + t (valid below n).  Leaves
-// the chunk sorted by key in L.a[0, n); every thread must call it; ends with
-// a barrier.
+// the chunk sorted by (key, payload) in L.a[0, n); every thread must call
+// it; ends with a barrier.
 __device__ void chunk_sort(const ulonglong2 (&v)[kSortItems], uint32_t n, SortLds& L) {
     const uint32_t tid = threadIdx.x;
     if (tid == 0) {
@@ -624,43 +664,13 @@ __device__ void chunk_sort(const ulonglong2 (&v)[kSortItems], uint32_t n, SortLd
     }
     __syncthreads();
     const uint64_t lo = L.mm[0], span = L.mm[1] - L.mm[0];
-    if (span == 0) {
+    if (span == 0) {  // one key: order by payload in the network below
 #pragma unroll
         for (uint32_t k = 0; k < kSortItems; k++)
             if (k * kSm64Block + tid < n) L.a[k * kSm64Block + tid] = v[k];
-        __syncthreads();
-        return;
-    }
-    const double scale = (double)n / ((double)span + 1.0);
-    uint32_t bk[kSortItems], rk[kSortItems];
-#pragma unroll
-    for (uint32_t k = 0; k < kSortItems; k++) {
-        if (k * kSm64Block + tid < n) {
-            bk[k] = min(n - 1, (uint32_t)((double)(v[k].x - lo) * scale));
-            rk[k] = atomicAdd(&L.cnt[bk[k]], 1u);
-        }
-    }
-    __syncthreads();
-    block_scan_lds<kSm64Block>(L.cnt, n, L.scratch);
-#pragma unroll
-    for (uint32_t k = 0; k < kSortItems; k++)
-        if (k * kSm64Block + tid < n) L.a[L.cnt[bk[k]] + rk[k]] = v[k];
-    __syncthreads();
-    for (uint32_t i = tid; i < n; i += kSm64Block) {
-        const uint32_t s0 = L.cnt[i], s1 = i + 1 < n ? L.cnt[i + 1] : n;
-        if (s1 - s0 > kBucketMax) {
-            L.big = 1;
-        } else {
-            for (uint32_t x = s0 + 1; x < s1; x++) {
-                const ulonglong2 t = L.a[x];
-                uint32_t y = x;
-                while (y > s0 && L.a[y - 1].x > t.x) {
-                    L.a[y] = L.a[y - 1];
-                    y--;
-                }
-                L.a[y] = t;
-            }
-        }
+        if (tid == 0) L.big = 1;
+    } else {
+        bucket_sort(v, n, lo, span, L);
     }
     __syncthreads();
     if (L.big) {

@@ -213,8 +213,8 @@ def bench_join32(args, tb):
     s_in = s_in_range(R, S) if args.workload == "c3" else nS
     pair_b = 8 * matches if mat else 0
     kernels = {
-        # k_join: reads every partitioned tuple once, writes the pairs
-        "join": (phases["join"], 8 * (nR + s_in) + pair_b, "k_join"),
+        # k_join / k_join_mat: reads every partitioned tuple once, writes the pairs
+        "join": (phases["join"], 8 * (nR + s_in) + pair_b, "k_join_mat" if mat else "k_join"),
         # second-pass scatter: reads and writes every surviving tuple once
         "scatter2": (phases["scatter2"], 16 * (nR + s_in), "k_scatter"),
         # second-pass sizing: the per-tile histogram read

@@ -66,7 +66,7 @@ struct SmTask {
     uint32_t s_off, s_len;  // S range of the same partition
     uint32_t rchunk;        // index of the R chunk
     uint32_t first;         // 1 for the chunk's first S range (writes the sorted chunk)
-    uint32_t multi;         // 1 if other tasks share this R chunk (more than one S range)
+    uint32_t multi;         // 1 if more than one task probes this R chunk (more than one S range)
     uint32_t pad;
 };
 
@@ -126,8 +126,10 @@ void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
                       const SmTask* rtasks, uint32_t nrtasks,
                       const unsigned long long* chunk_off, const unsigned long long* total,
                       unsigned long long* runs);
+// Emit: tasks of shared chunks claim slots in their runs; a chunk's only
+// task lays its runs out from chunk_off[rchunk] (the scanned volumes).
 void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
                const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
-               ulonglong2* pairs, uint64_t cap);
+               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap);
 
 }  // namespace tbj

@@ -553,10 +553,12 @@ __global__ void k_fill_tasks64(const uint2* __restrict__ bounds, uint32_t nparts
     if (i < nrtasks) {
         const uint32_t p = owner_part(roff, nparts, i);
         const uint2 r = bounds[p];
+        const uint2 s = bounds[nparts + p];
         SmTask t{};
         t.r_off = r.x + (i - roff[p]) * kSm64Cap;
         t.r_len = min((uint32_t)kSm64Cap, r.y - t.r_off);
         t.rchunk = i;
+        t.multi = s.y - s.x > (uint32_t)kSm64SChunk;  // probed by more than one task
         rtasks[i] = t;
     } else if (i < nrtasks + njtasks) {
         const uint32_t jt = i - nrtasks;
@@ -857,14 +859,15 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
             mt = wave_reduce_u64(mt);
             if (lane_id() == 0 && mt) atomicAdd(&tm, (unsigned long long)mt);
             __syncthreads();
-            // a chunk probed by this task alone is written plainly
+            // a chunk shared by several tasks accumulates its run counts for
+            // k_run_offsets64; a chunk probed by this task alone needs only its
+            // volume (k_emit64 recounts its runs)
             if (tk.multi) {
                 for (uint32_t j = threadIdx.x; j < n; j += kSm64Block)
                     if (L.cnt[j]) atomicAdd(&runs[tk.r_off + j], (unsigned long long)L.cnt[j]);
                 if (threadIdx.x == 0 && tm) atomicAdd(&chunk_vol[tk.rchunk], tm);
-            } else {  // every counter of the chunk, zeros included (no prior clear)
-                for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) runs[tk.r_off + j] = L.cnt[j];
-                if (threadIdx.x == 0) chunk_vol[tk.rchunk] = tm;
+            } else if (threadIdx.x == 0) {
+                chunk_vol[tk.rchunk] = tm;
             }
         }
         __syncthreads();
@@ -964,10 +967,11 @@ __global__ __launch_bounds__(kSm64Block) void k_run_offsets64(
     __shared__ unsigned long long scratch[kSm64Block / TBJ_WAVE + 1];
     constexpr uint32_t kPer = kSm64Cap / kSm64Block;
     for (uint32_t c = blockIdx.x; c < nrtasks; c += gridDim.x) {
+        const SmTask tk = rtasks[c];
+        if (!tk.multi) continue;  // k_emit64 places a single task's runs itself
         const unsigned long long base = chunk_off[c];
         const unsigned long long end = c + 1 < nrtasks ? chunk_off[c + 1] : *total;
         if (end == base) continue;  // no output: the chunk may be unsorted
-        const SmTask tk = rtasks[c];
         const uint32_t n = tk.r_len;
         const ulonglong2* Rc = Rsorted + tk.r_off;
         uint32_t cnt[kPer];
@@ -1012,9 +1016,11 @@ __global__ void k_clear_shared_runs(const SmTask* __restrict__ tasks, uint32_t n
 __global__ __launch_bounds__(kSm64Block) void k_emit64(
     const ulonglong2* __restrict__ Rsorted, const ulonglong2* __restrict__ Sb,
     const SmTask* __restrict__ tasks, const uint32_t* __restrict__ ntasks_dev,
-    unsigned long long* __restrict__ runs, ulonglong2* __restrict__ pairs, uint64_t cap) {
+    unsigned long long* __restrict__ runs, const unsigned long long* __restrict__ chunk_off,
+    ulonglong2* __restrict__ pairs, uint64_t cap) {
     __shared__ uint64_t rk[kSm64Cap];
     __shared__ uint32_t lcnt[kSm64Cap];
+    __shared__ uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
     __shared__ unsigned long long lbase[kSm64Cap];
     const uint32_t ntasks = *ntasks_dev;
     for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
@@ -1047,13 +1053,23 @@ __global__ __launch_bounds__(kSm64Block) void k_emit64(
             }
         }
         __syncthreads();
-        // a chunk probed by this task alone owns its runs' slots outright
-        for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
-            const uint32_t c = lcnt[j];
-            if (c)
-                lbase[j] = tk.multi ? atomicAdd(&runs[tk.r_off + j],
-                                                (unsigned long long)c * run_len(rk, n, j))
-                                    : runs[tk.r_off + j];
+        if (tk.multi) {  // claim slots inside each run's range (k_run_offsets64)
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
+                const uint32_t c = lcnt[j];
+                if (c)
+                    lbase[j] = atomicAdd(&runs[tk.r_off + j],
+                                         (unsigned long long)c * run_len(rk, n, j));
+            }
+        } else {  // the chunk's only task: runs laid out in key order from its offset
+            // (a run's volume is at most 4096 S tuples x 2048 R tuples: 32 bits)
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
+                const uint32_t c = lcnt[j];
+                lcnt[j] = c ? c * run_len(rk, n, j) : 0;
+            }
+            __syncthreads();
+            block_scan_lds<kSm64Block>(lcnt, n, scratch);
+            const unsigned long long base = chunk_off[tk.rchunk];
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) lbase[j] = base + lcnt[j];
         }
         __syncthreads();
 #pragma unroll
@@ -1205,9 +1221,9 @@ void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
 
 void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
                const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
-               ulonglong2* pairs, uint64_t cap) {
+               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap) {
     hipLaunchKernelGGL(k_emit64, dim3(grid), dim3(kSm64Block), 0, st, Rsorted, Sb, tasks,
-                       ntasks_dev, runs, pairs, cap);
+                       ntasks_dev, runs, chunk_off, pairs, cap);
 }
 
 }  // namespace tbj

@@ -24,6 +24,8 @@
 // whole output is in key order (radix: in key order inside each partition).
 #include <algorithm>
 
+#include <mutex>
+
 #include "tbj_device.h"
 #include "tbj_kernels.h"
 #include "tbj_sm64.h"
@@ -1153,8 +1155,36 @@ void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const
                            bounds, nparts, roff, joff, nrtasks, njtasks, rtasks, jtasks);
 }
 
+// The persistent kernels below stride their task lists by gridDim.x, so a
+// grid larger than what stays resident leaves the surplus workgroups to
+// start late and run a tail: clamp to the kernel's resident capacity.
+template <typename K>
+static uint32_t resident_grid(K kernel, uint32_t want) {
+    // capacity per kernel, looked up once (kernels of one signature share K)
+    static const void* keys[8] = {};
+    static uint32_t caps[8] = {};
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    uint32_t i = 0;
+    while (i < 8 && keys[i] && keys[i] != (const void*)kernel) i++;
+    if (i == 8) return want;
+    if (!keys[i]) {
+        int dev = 0, cus = 0, per = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kSm64Block, 0) !=
+                hipSuccess ||
+            per < 1)
+            per = 1;
+        caps[i] = (uint32_t)(cus > 0 ? cus : 256) * (uint32_t)per;
+        keys[i] = (const void*)kernel;
+    }
+    return want < caps[i] ? want : caps[i];
+}
+
 void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
                  uint32_t nrtasks) {
+    grid = resident_grid(k_sort_r64, grid);
     if (nrtasks && grid)
         hipLaunchKernelGGL(k_sort_r64, dim3(grid), dim3(kSm64Block), 0, st, Rb, rtasks, nrtasks);
 }
@@ -1177,19 +1207,22 @@ void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const u
                 ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
                 const uint32_t* ntasks_dev, unsigned long long* runs,
                 unsigned long long* chunk_vol, unsigned long long* counters) {
-    const dim3 g(grid), b(kSm64Block);
+    const dim3 b(kSm64Block);
     if (mat && presorted)
-        hipLaunchKernelGGL((k_count64<true, true>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
-                           runs, chunk_vol, counters);
+        hipLaunchKernelGGL((k_count64<true, true>), dim3(resident_grid(k_count64<true, true>, grid)),
+                           b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev, runs, chunk_vol, counters);
     else if (mat)
-        hipLaunchKernelGGL((k_count64<true, false>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
-                           runs, chunk_vol, counters);
+        hipLaunchKernelGGL((k_count64<true, false>),
+                           dim3(resident_grid(k_count64<true, false>, grid)), b, 0, st, Rs, Rout,
+                           Sb, tasks, ntasks_dev, runs, chunk_vol, counters);
     else if (presorted)
-        hipLaunchKernelGGL((k_count64<false, true>), g, b, 0, st, Rs, Rout, Sb, tasks, ntasks_dev,
-                           runs, chunk_vol, counters);
+        hipLaunchKernelGGL((k_count64<false, true>),
+                           dim3(resident_grid(k_count64<false, true>, grid)), b, 0, st, Rs, Rout,
+                           Sb, tasks, ntasks_dev, runs, chunk_vol, counters);
     else
-        hipLaunchKernelGGL((k_count64<false, false>), g, b, 0, st, Rs, Rout, Sb, tasks,
-                           ntasks_dev, runs, chunk_vol, counters);
+        hipLaunchKernelGGL((k_count64<false, false>),
+                           dim3(resident_grid(k_count64<false, false>, grid)), b, 0, st, Rs, Rout,
+                           Sb, tasks, ntasks_dev, runs, chunk_vol, counters);
 }
 
 void sm64_clear_shared_runs(hipStream_t st, const SmTask* tasks, uint32_t ntasks,
@@ -1215,15 +1248,16 @@ void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
                       const unsigned long long* chunk_off, const unsigned long long* total,
                       unsigned long long* runs) {
     if (nrtasks)
-        hipLaunchKernelGGL(k_run_offsets64, dim3(grid), dim3(kSm64Block), 0, st, Rsorted, rtasks,
-                           nrtasks, chunk_off, total, runs);
+        hipLaunchKernelGGL(k_run_offsets64, dim3(resident_grid(k_run_offsets64, grid)),
+                           dim3(kSm64Block), 0, st, Rsorted, rtasks, nrtasks, chunk_off, total,
+                           runs);
 }
 
 void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
                const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
                const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap) {
-    hipLaunchKernelGGL(k_emit64, dim3(grid), dim3(kSm64Block), 0, st, Rsorted, Sb, tasks,
-                       ntasks_dev, runs, chunk_off, pairs, cap);
+    hipLaunchKernelGGL(k_emit64, dim3(resident_grid(k_emit64, grid)), dim3(kSm64Block), 0, st,
+                       Rsorted, Sb, tasks, ntasks_dev, runs, chunk_off, pairs, cap);
 }
 
 }  // namespace tbj

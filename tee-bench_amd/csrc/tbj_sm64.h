@@ -16,6 +16,11 @@ constexpr int kSm64Items = 4;
 constexpr int kSm64Tile = kSm64Block * kSm64Items;  // 2048 tuples
 constexpr int kSm64Super = 4;
 constexpr int kSm64Span = kSm64Tile * kSm64Super;   // 8192 tuples per histogram tile
+// The scatter stages kSm64ScTile tuples at a time (a divisor of kSm64Span).
+constexpr int kSm64ScBlock = 1024;
+constexpr int kSm64ScItems = 4;
+constexpr int kSm64ScTile = kSm64ScBlock * kSm64ScItems;
+static_assert(kSm64Span % kSm64ScTile == 0, "scatter stages must tile a histogram tile");
 constexpr int kSm64PassBits = 10;
 constexpr int kSm64MaxBins = 1 << kSm64PassBits;
 // Join: R chunks of at most kSm64Cap tuples are sorted in LDS; a task is one
@@ -84,9 +89,12 @@ void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
                const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
                uint32_t* hist, const Bin64& b, uint32_t nbins, unsigned long long* mm);
+// one_per_cu: request enough LDS that only one workgroup runs per CU (the
+// second pass writes fewer partial lines that way; measured on C5).
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
                   const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins);
+                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins,
+                  bool one_per_cu = false);
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
                    uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
                    uint32_t max_tiles2);

@@ -339,52 +339,58 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
 }
 
-__global__ __launch_bounds__(kSm64Block) void k_scatter64(
+__host__ __device__ constexpr size_t sm64_model_bytes_aligned(uint32_t kind) {
+    return kind == kBinLearned ? (sizeof(LdsModel64) + 15) / 16 * 16 : 0;
+}
+
+__global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out,
     const TileDesc* __restrict__ tiles, const SegDesc* __restrict__ segs,
     const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base,
     const uint32_t* __restrict__ scanned, Bin64 b, uint32_t nbins) {
-    __shared__ ulonglong2 stage[kSm64Tile];
-    __shared__ uint16_t sbin[kSm64Tile];
-    __shared__ uint32_t loff[kSm64MaxBins];
-    __shared__ uint32_t goff[kSm64MaxBins];
-    __shared__ uint32_t next[kSm64MaxBins];
-    __shared__ uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
+    __shared__ ulonglong2 stage[kSm64ScTile];
+    __shared__ uint16_t sbin[kSm64ScTile];
+    __shared__ uint32_t scratch[kSm64ScBlock / TBJ_WAVE + 1];
+    // dynamic LDS: the learned model (if any), then three arrays of nbins
+    // words, so that a radix pass of 512 bins fits two workgroups per CU
     extern __shared__ __align__(16) uint8_t dyn_lds[];
+    uint32_t* loff = reinterpret_cast<uint32_t*>(dyn_lds + sm64_model_bytes_aligned(b.kind));
+    uint32_t* goff = loff + nbins;
+    uint32_t* next = goff + nbins;
     const uint32_t t = tile_base + blockIdx.x;
     if (t >= *ntiles_dev) return;
     const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
     const TileDesc td = tiles[t];
     const SegDesc sd = segs[td.seg];
     const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock)
         next[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis];
-    ulonglong2 v[kSm64Items];
+    ulonglong2 v[kSm64ScItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kSm64Items; k++) {
-        const uint32_t j = k * kSm64Block + threadIdx.x;
+    for (uint32_t k = 0; k < kSm64ScItems; k++) {
+        const uint32_t j = k * kSm64ScBlock + threadIdx.x;
         if (j < td.len) v[k] = in[td.in_off + j];
     }
-    for (uint32_t s0 = 0; s0 < td.len; s0 += kSm64Tile) {
-        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) loff[i] = 0;
+    for (uint32_t s0 = 0; s0 < td.len; s0 += kSm64ScTile) {
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) loff[i] = 0;
         __syncthreads();
-        uint32_t bb[kSm64Items], rank[kSm64Items];
+        uint32_t bb[kSm64ScItems], rank[kSm64ScItems];
 #pragma unroll
-        for (uint32_t k = 0; k < kSm64Items; k++) {
-            const uint32_t j = s0 + k * kSm64Block + threadIdx.x;
+        for (uint32_t k = 0; k < kSm64ScItems; k++) {
+            const uint32_t j = s0 + k * kSm64ScBlock + threadIdx.x;
             bb[k] = j < td.len ? bin(v[k].x) : kNone;
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kSm64Items; k++) rank[k] = agg_inc(loff, bb[k]);
+        for (uint32_t k = 0; k < kSm64ScItems; k++) rank[k] = agg_inc(loff, bb[k]);
         __syncthreads();
-        const uint32_t nvalid = block_scan_lds<kSm64Block>(loff, nbins, scratch);
-        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) {
+        const uint32_t nvalid = block_scan_lds<kSm64ScBlock>(loff, nbins, scratch);
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) {
             const uint32_t end = i + 1 < nbins ? loff[i + 1] : nvalid;
             goff[i] = next[i] - loff[i];
             next[i] += end - loff[i];
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kSm64Items; k++) {
+        for (uint32_t k = 0; k < kSm64ScItems; k++) {
             if (bb[k] != kNone) {
                 const uint32_t slot = loff[bb[k]] + rank[k];
                 stage[slot] = v[k];
@@ -393,12 +399,12 @@ __global__ __launch_bounds__(kSm64Block) void k_scatter64(
         }
         // the next stage's loads go out before this stage is written back
 #pragma unroll
-        for (uint32_t k = 0; k < kSm64Items; k++) {
-            const uint32_t j = s0 + kSm64Tile + k * kSm64Block + threadIdx.x;
+        for (uint32_t k = 0; k < kSm64ScItems; k++) {
+            const uint32_t j = s0 + kSm64ScTile + k * kSm64ScBlock + threadIdx.x;
             if (j < td.len) v[k] = in[td.in_off + j];
         }
         __syncthreads();
-        for (uint32_t j = threadIdx.x; j < nvalid; j += kSm64Block) out[goff[sbin[j]] + j] = stage[j];
+        for (uint32_t j = threadIdx.x; j < nvalid; j += kSm64ScBlock) out[goff[sbin[j]] + j] = stage[j];
         __syncthreads();
     }
 }
@@ -1118,9 +1124,17 @@ void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDe
 
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
                   const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins) {
+                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins,
+                  bool one_per_cu) {
     if (!grid) return;
-    hipLaunchKernelGGL(k_scatter64, dim3(grid), dim3(kSm64Block), sm64_model_lds(b), st, in, out,
+    size_t lds = sm64_model_bytes_aligned(b.kind) + 3 * sizeof(uint32_t) * nbins;
+    if (one_per_cu) {
+        constexpr size_t kStatic = sizeof(ulonglong2) * kSm64ScTile + sizeof(uint16_t) * kSm64ScTile +
+                                   sizeof(uint32_t) * (kSm64ScBlock / TBJ_WAVE + 1);
+        constexpr size_t kHalfCu = 80 * 1024;
+        if (kStatic + lds <= kHalfCu) lds = kHalfCu - kStatic + 1024;
+    }
+    hipLaunchKernelGGL(k_scatter64, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
                        tiles, segs, ntiles_dev, tile_base, scanned, b, nbins);
 }
 

@@ -137,21 +137,50 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
 }
 
 // Exact key range of a relation: minmax[0] = min, minmax[1] = max (must start
-// at {UINT_MAX, 0}).
+// at {UINT_MAX, 0}).  Grid-stride over pairs of tuples (16-byte loads when
+// the base is 16-byte aligned, which the host checks), a block reduction,
+// then one atomic pair per workgroup.
 __global__ __launch_bounds__(256) void k_minmax(const uint2* __restrict__ in, uint64_t n,
                                                 uint32_t* __restrict__ minmax) {
+    __shared__ uint32_t red[2][256 / TBJ_WAVE];
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t k = in[i].x;
-        mn = min(mn, k);
-        mx = max(mx, k);
+    const uint64_t npair = n / 2;
+    const uint4* in4 = reinterpret_cast<const uint4*>(in);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < npair; i += 4 * stride) {  // four 16-byte loads in flight
+        uint4 v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = in4[i + q * stride];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            mn = min(mn, min(v[q].x, v[q].z));
+            mx = max(mx, max(v[q].x, v[q].z));
+        }
+    }
+    for (; i < npair; i += stride) {
+        const uint4 v = in4[i];
+        mn = min(mn, min(v.x, v.z));
+        mx = max(mx, max(v.x, v.z));
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        mn = min(mn, in[n - 1].x);
+        mx = max(mx, in[n - 1].x);
     }
     for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
         mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
         mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
     }
     if (lane_id() == 0) {
+        red[0][wave_id()] = mn;
+        red[1][wave_id()] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / TBJ_WAVE; w++) {
+            mn = min(mn, red[0][w]);
+            mx = max(mx, red[1][w]);
+        }
         atomicMin(&minmax[0], mn);
         atomicMax(&minmax[1], mx);
     }
@@ -520,6 +549,8 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
                                                        CdfModel* __restrict__ model) {
     __shared__ uint32_t a[kFitMax];
     __shared__ uint32_t knots[kLdsKnots + 1];
+    __shared__ uint64_t kx[kLdsKnots + 1];
+    __shared__ uint32_t ky[kLdsKnots + 1];
     __shared__ uint32_t nk_s;
     for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) a[i] = sample[i];
     __syncthreads();
@@ -561,6 +592,8 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
         if (yi > 0xFFFFFFFFull) yi = 0xFFFFFFFFull;
         model->x[i] = xi;
         model->y[i] = (uint32_t)yi;
+        kx[i] = xi;
+        ky[i] = (uint32_t)yi;
     }
     __syncthreads();
     // cell table: the model at equal-width cell starts, in 1/256 partitions
@@ -577,12 +610,12 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
             uint32_t lo = 0, hi = nseg;  // largest i with x[i] <= key
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (model->x[mid] <= key) lo = mid;
+                if (kx[mid] <= key) lo = mid;
                 else hi = mid;
             }
-            const double dx = (double)(model->x[lo + 1] - model->x[lo]);
-            const double dy = (double)model->y[lo + 1] - (double)model->y[lo];
-            y = (double)model->y[lo] + (double)(key - model->x[lo]) * dy / dx;
+            const double dx = (double)(kx[lo + 1] - kx[lo]);
+            const double dy = (double)ky[lo + 1] - (double)ky[lo];
+            y = (double)ky[lo] + (double)(key - kx[lo]) * dy / dx;
         }
         model->cells[c] = (uint32_t)(y * to_cells);
     }
@@ -736,8 +769,15 @@ void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nse
 
 void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax) {
     if (!n) return;
-    uint64_t g = (n + 255) / 256;
-    if (g > 4096) g = 4096;
+    if (reinterpret_cast<uintptr_t>(in) % 16) {  // pair loads need 16-byte alignment
+        launch_minmax(st, in + 1, n - 1, minmax);
+        if (n == 1) return;
+        hipLaunchKernelGGL(k_minmax, dim3(1), dim3(256), 0, st, in, (uint64_t)1, minmax);
+        return;
+    }
+    uint64_t g = (n / 2 + 255) / 256;
+    if (g > 2048) g = 2048;
+    if (g < 1) g = 1;
     hipLaunchKernelGGL(k_minmax, dim3((uint32_t)g), dim3(256), 0, st, in, n, minmax);
 }
 

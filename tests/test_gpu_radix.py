@@ -64,6 +64,22 @@ def test_duplicates_and_extreme_keys(tbj, oracle):
         assert res.matches == n and np.array_equal(ga, ra) and np.array_equal(gb, sa)
 
 
+@pytest.mark.parametrize("algo", ["RHO", "RHT"])
+@pytest.mark.parametrize("ns", [1, 4095, 4096, 4097, 16384, 16385, 40000])
+def test_materialising_join_sub_chunk_edges(tbj, oracle, algo, ns):
+    # one partition (|R| below one table): the task list splits S into tasks of
+    # up to four sub-chunks of 4096 tuples, whose loads run ahead of the claims
+    rng = np.random.default_rng(ns)
+    R = rel(rng.integers(0, 1500, 2000))  # repeated R keys: several matches per S tuple
+    S = rel(rng.integers(0, 1800, ns))
+    n, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    res, got = gu.run(tbj, algo, R, S, cap=n)
+    assert res.matches == n and res.pairs_written == n
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+    assert res.checksum == gu.checksum(ra, sa)
+
+
 def test_oversized_partition_is_chunked(tbj, oracle):
     # one key repeated far beyond the LDS table capacity on the build side
     rk = np.concatenate([np.full(10000, 42), np.arange(1, 3000)])

@@ -36,8 +36,6 @@ constexpr int kJoinSTaskChunks = 4;
 
 enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
-size_t lds_model_bytes();
-
 // Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev).
 // fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,

@@ -683,8 +683,6 @@ __global__ void k_zero_tail(uint32_t* __restrict__ hist, uint64_t n,
 // ---------------------------------------------------------------------------
 // Host launchers
 
-size_t lds_model_bytes() { return sizeof(LdsModel); }
-
 static inline size_t model_lds(const BinSpec& bs) {
     return bs.kind == kBinLearned ? sizeof(LdsModel) : 0;
 }

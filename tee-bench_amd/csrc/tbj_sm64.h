@@ -8,15 +8,12 @@
 
 namespace tbj {
 
-// Partition passes: a workgroup of kSm64Block threads stages kSm64Tile
-// tuples (32 KiB) in LDS at a time and owns a histogram tile of kSm64Super
-// such stages, so the per-tile histograms stay small next to the data.
+// Partition passes: one histogram tile is kSm64Span tuples (128 KiB), so the
+// per-tile histograms stay small next to the data; the scatter's workgroup of
+// kSm64ScBlock threads stages kSm64ScTile tuples (64 KiB) of it in LDS at a
+// time.  The other 64-bit kernels use workgroups of kSm64Block threads.
 constexpr int kSm64Block = 512;
-constexpr int kSm64Items = 4;
-constexpr int kSm64Tile = kSm64Block * kSm64Items;  // 2048 tuples
-constexpr int kSm64Super = 4;
-constexpr int kSm64Span = kSm64Tile * kSm64Super;   // 8192 tuples per histogram tile
-// The scatter stages kSm64ScTile tuples at a time (a divisor of kSm64Span).
+constexpr int kSm64Span = 8192;
 constexpr int kSm64ScBlock = 1024;
 constexpr int kSm64ScItems = 4;
 constexpr int kSm64ScTile = kSm64ScBlock * kSm64ScItems;

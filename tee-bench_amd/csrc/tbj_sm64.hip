@@ -773,35 +773,69 @@ __global__ void k_fill_u64(unsigned long long* __restrict__ p, uint64_t n, unsig
         p[i] = v;
 }
 
-// Lower bounds of N keys at once (lo[q] = first position with key >= k[q],
-// n for invalid items): the N binary searches advance in lockstep, so every
-// step issues N independent LDS reads.  key_at(m) reads sorted key m.
-template <int N, typename KeyAt>
-__device__ __forceinline__ void lower_batch(KeyAt key_at, uint32_t n, const uint64_t (&k)[N],
-                                            const bool (&valid)[N], uint32_t (&lo)[N]) {
-    uint32_t hi[N];
-#pragma unroll
-    for (int q = 0; q < N; q++) {
-        lo[q] = valid[q] ? 0 : n;
-        hi[q] = n;
-    }
-    for (uint32_t w = n; w; w >>= 1) {
-#pragma unroll
-        for (int q = 0; q < N; q++) {
-            if (lo[q] < hi[q]) {
-                const uint32_t m = (lo[q] + hi[q]) >> 1;
-                if (key_at(m) < k[q]) lo[q] = m + 1;
-                else hi[q] = m;
-            }
-        }
-    }
-}
-
 // Length of the key run starting at sorted position j.
 __device__ __forceinline__ uint32_t run_len(const uint64_t* keys, uint32_t n, uint32_t j) {
     uint32_t len = 1;
     while (j + len < n && keys[j + len] == keys[j]) len++;
     return len;
+}
+
+// Bucket of a 64-bit key in a table of 2^hb buckets (multiplicative hash:
+// the high bits of key * 2^64/phi, so keys that share their low bits, as in
+// a radix partition, still spread).
+__device__ __forceinline__ uint32_t hash64(uint64_t key, uint32_t hb) {
+    return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - hb));
+}
+
+__device__ __forceinline__ uint32_t hash_bits(uint32_t n) {
+    uint32_t b = 1;
+    while ((1u << b) < n) b++;
+    return b;
+}
+
+// LDS index of the key runs of a sorted chunk of n keys: each run start j
+// (its key differs from key j - 1) is chained under hash64(key) (1-based,
+// 0 ends a chain).  head must be zero; a barrier must follow.
+template <typename KeyAt>
+__device__ __forceinline__ void index_runs(KeyAt key_at, uint32_t n, uint32_t hb, uint32_t* head,
+                                           uint16_t* nxt) {
+    for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
+        const uint64_t k = key_at(j);
+        if (j == 0 || key_at(j - 1) != k) nxt[j] = (uint16_t)atomicExch(&head[hash64(k, hb)], j + 1);
+    }
+}
+
+// Run starts of N keys at once through index_runs' table (lo[q] = first
+// sorted position of key q, n when absent or invalid): the N chains are
+// walked in lockstep, so every step issues N independent LDS reads.
+template <int N, typename KeyAt>
+__device__ __forceinline__ void find_runs(KeyAt key_at, uint32_t n, uint32_t hb,
+                                          const uint32_t* head, const uint16_t* nxt,
+                                          const uint64_t (&k)[N], const bool (&valid)[N],
+                                          uint32_t (&lo)[N]) {
+    uint32_t e[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        e[q] = valid[q] ? head[hash64(k[q], hb)] : 0;
+        lo[q] = n;
+    }
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            if (e[q]) {
+                const uint32_t i = e[q] - 1;
+                if (key_at(i) == k[q]) {
+                    lo[q] = i;
+                    e[q] = 0;
+                } else {
+                    e[q] = nxt[i];
+                }
+                any |= e[q] != 0;
+            }
+        }
+        if (!any) break;
+    }
 }
 
 // Count.  counters[0] += matches, counters[1] += sum of (R.payload +
@@ -814,13 +848,18 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
     const uint32_t* __restrict__ ntasks_dev, unsigned long long* __restrict__ runs,
     unsigned long long* __restrict__ chunk_vol, unsigned long long* __restrict__ counters) {
     __shared__ SortLds L;
+    __shared__ uint32_t head[kSm64Cap];
+    __shared__ uint16_t nxt[kSm64Cap];
     __shared__ unsigned long long tm;
     __shared__ unsigned long long red_m[kSm64Block / TBJ_WAVE], red_c[kSm64Block / TBJ_WAVE];
     uint64_t matches = 0, cks = 0;
     const uint32_t ntasks = *ntasks_dev;
+    for (uint32_t i = threadIdx.x; i < kSm64Cap; i += kSm64Block) head[i] = 0;
+    const auto key_at = [&](uint32_t m) { return (uint64_t)L.a[m].x; };
     for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const SmTask tk = tasks[t];
         const uint32_t n = tk.r_len;
+        const uint32_t hb = hash_bits(n);
         if (PRESORTED) {
             for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.a[j] = Rs[tk.r_off + j];
             __syncthreads();
@@ -831,12 +870,13 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
             if (MAT && tk.first)
                 for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) Rout[tk.r_off + j] = L.a[j];
         }
+        index_runs(key_at, n, hb, head, nxt);
         if (MAT)
             for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.cnt[j] = 0;
         if (threadIdx.x == 0) tm = 0;
         __syncthreads();
         uint64_t mt = 0;
-        constexpr int G = 4;  // S tuples searched in lockstep per thread
+        constexpr int G = 4;  // S tuples looked up in lockstep per thread
         for (uint32_t j0 = 0; j0 < tk.s_len; j0 += G * kSm64Block) {
             uint64_t sk[G], sp[G];
             bool ok[G];
@@ -849,10 +889,10 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
                 sk[q] = sv.x;
                 sp[q] = sv.y;
             }
-            lower_batch<G>([&](uint32_t m) { return (uint64_t)L.a[m].x; }, n, sk, ok, lo);
+            find_runs<G>(key_at, n, hb, head, nxt, sk, ok, lo);
 #pragma unroll
             for (int q = 0; q < G; q++) {
-                const bool hit = lo[q] < n && L.a[lo[q]].x == sk[q];
+                const bool hit = lo[q] < n;
                 if (hit) {
                     for (uint32_t e = lo[q]; e < n && L.a[e].x == sk[q]; e++) {
                         mt++;
@@ -879,6 +919,8 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
             }
         }
         __syncthreads();
+        // (the next task's sort or load ends with a barrier before its index)
+        for (uint32_t i = threadIdx.x; i < (1u << hb); i += kSm64Block) head[i] = 0;
     }
     matches = wave_reduce_u64(matches);
     cks = wave_reduce_u64(cks);
@@ -1021,7 +1063,7 @@ __global__ void k_clear_shared_runs(const SmTask* __restrict__ tasks, uint32_t n
 
 // Emit.  runs[g] holds the next free output slot of the run starting at g;
 // the S tuples of a task claim run-length blocks from it.
-__global__ __launch_bounds__(kSm64Block) void k_emit64(
+__global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6))) void k_emit64(
     const ulonglong2* __restrict__ Rsorted, const ulonglong2* __restrict__ Sb,
     const SmTask* __restrict__ tasks, const uint32_t* __restrict__ ntasks_dev,
     unsigned long long* __restrict__ runs, const unsigned long long* __restrict__ chunk_off,
@@ -1030,15 +1072,22 @@ __global__ __launch_bounds__(kSm64Block) void k_emit64(
     __shared__ uint32_t lcnt[kSm64Cap];
     __shared__ uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
     __shared__ unsigned long long lbase[kSm64Cap];
+    __shared__ uint32_t head[kSm64Cap];
+    __shared__ uint16_t nxt[kSm64Cap];
     const uint32_t ntasks = *ntasks_dev;
+    for (uint32_t i = threadIdx.x; i < kSm64Cap; i += kSm64Block) head[i] = 0;
+    const auto key_at = [&](uint32_t m) { return rk[m]; };
     for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
         const SmTask tk = tasks[t];
         const uint32_t n = tk.r_len;
+        const uint32_t hb = hash_bits(n);
         const ulonglong2* Rc = Rsorted + tk.r_off;
         for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
             rk[j] = Rc[j].x;
             lcnt[j] = 0;
         }
+        __syncthreads();
+        index_runs(key_at, n, hb, head, nxt);
         __syncthreads();
         uint64_t sp[kSm64SItems];
         uint32_t slo[kSm64SItems], srank[kSm64SItems];
@@ -1053,10 +1102,10 @@ __global__ __launch_bounds__(kSm64Block) void k_emit64(
                 sk[k] = s.x;
                 sp[k] = s.y;
             }
-            lower_batch<kSm64SItems>([&](uint32_t m) { return rk[m]; }, n, sk, ok, slo);
+            find_runs<kSm64SItems>(key_at, n, hb, head, nxt, sk, ok, slo);
 #pragma unroll
             for (uint32_t k = 0; k < kSm64SItems; k++) {
-                if (!(slo[k] < n && rk[slo[k]] == sk[k])) slo[k] = kNone;
+                if (slo[k] >= n) slo[k] = kNone;
                 srank[k] = agg_inc(lcnt, slo[k]);
             }
         }
@@ -1091,6 +1140,8 @@ __global__ __launch_bounds__(kSm64Block) void k_emit64(
             }
         }
         __syncthreads();
+        // (the next task's key load ends with a barrier before its index)
+        for (uint32_t i = threadIdx.x; i < (1u << hb); i += kSm64Block) head[i] = 0;
     }
 }
 

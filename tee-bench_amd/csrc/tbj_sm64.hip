@@ -9,17 +9,20 @@
 //   1. partition R and S in one or two passes (radix on the low key bits, or
 //      the learned range model fitted on an R sample, whose partitions are
 //      key ranges in key order);
-//   2. count: each task sorts its R chunk (at most kSm64Cap tuples) by key
-//      in LDS (bucket sort; the chunk's first task also writes the sorted
-//      chunk out), then every S tuple finds its R key run by binary search;
-//      per run the matching S tuples are counted (one global counter per
-//      sorted R position) and per chunk, the pairs.  R partitions above one
-//      chunk are instead sorted beforehand (chunks, then merge passes);
+//   2. count: each task sorts its R chunk (at most kSm64Cap tuples) by
+//      (key, payload) in LDS (bucket sort; the chunk's first task also writes
+//      the sorted chunk out), indexes the first position of every key run in
+//      an LDS hash table, and every S tuple finds its run there; per chunk
+//      the pairs are counted, and for a chunk probed by several tasks also
+//      the S tuples per run (one global counter per sorted R position).  R
+//      partitions above one chunk are instead sorted beforehand (chunks, then
+//      merge passes);
 //   3. a scan of the per-chunk pair counts gives each chunk's output offset,
-//      and one pass per chunk turns the run counts into run output offsets
-//      (scan of count x run length in key order);
-//   4. emit: each S tuple claims a slot block of run-length pairs in its run
-//      and writes them.
+//      and for shared chunks one pass turns the run counts into run output
+//      offsets (scan of count x run length in key order);
+//   4. emit: each S tuple finds its run again and writes run-length pairs at
+//      its slot: a chunk's only task lays its runs out from the chunk offset
+//      itself, tasks of shared chunks claim slot blocks in each run.
 // So pairs of one key are contiguous, and with the learned partitioner the
 // whole output is in key order (radix: in key order inside each partition).
 #include <algorithm>

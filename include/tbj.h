@@ -171,7 +171,7 @@ typedef struct tbj_batch_plan tbj_batch_plan;
 typedef struct tbj_batch_tuning {
     uint32_t target_r;          /* average R tuples per partition; 0 = 2048 */
     uint32_t one_pass_bits;     /* most partition bits done in one pass; 0 = 8 */
-    uint32_t s_chunk;           /* S tuples per join task; 0 = 8192 */
+    uint32_t s_chunk;           /* S tuples per join task (probed 8192 at a time); 0 = 32768 */
     uint32_t wave_tuples;       /* partitioned tuples per wave; 0 = 2^31 */
 } tbj_batch_tuning;
 

@@ -133,35 +133,46 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
             }
         }
         __syncthreads();
-        // chains of G S tuples walked in lockstep: G independent LDS reads per step
+        // the task's S range in sub-chunks of SITEMS * kJoinBlock against the
+        // one build (sub-chunk 0 was loaded with R); chains of G S tuples
+        // walked in lockstep: G independent LDS reads per step
         uint32_t c = 0;
         constexpr int G = 8;
+        for (uint32_t s0 = 0; s0 < tk.s_len; s0 += SITEMS * kJoinBlock) {
+            if (s0) {
 #pragma unroll
-        for (int k0 = 0; k0 < SITEMS; k0 += G) {
-            uint32_t e[G];
-#pragma unroll
-            for (int q = 0; q < G; q++) {
-                const uint32_t j = (k0 + q) * kJoinBlock + threadIdx.x;
-                e[q] = j < tk.s_len ? head[(sv[k0 + q].x >> hshift) & hmask] : 0;
+                for (int k = 0; k < SITEMS; k++) {
+                    const uint32_t j = s0 + k * kJoinBlock + threadIdx.x;
+                    if (j < tk.s_len) sv[k] = Sb[tk.s_off + j];
+                }
             }
-            for (;;) {
-                uint32_t rk[G], nx[G];
-                bool any = false;
+#pragma unroll
+            for (int k0 = 0; k0 < SITEMS; k0 += G) {
+                uint32_t e[G];
 #pragma unroll
                 for (int q = 0; q < G; q++) {
-                    const uint32_t i = e[q] ? e[q] - 1 : 0;
-                    rk[q] = rkey[i];
-                    nx[q] = nxt[i];
+                    const uint32_t j = s0 + (k0 + q) * kJoinBlock + threadIdx.x;
+                    e[q] = j < tk.s_len ? head[(sv[k0 + q].x >> hshift) & hmask] : 0;
                 }
+                for (;;) {
+                    uint32_t rk[G], nx[G];
+                    bool any = false;
 #pragma unroll
-                for (int q = 0; q < G; q++) {
-                    if (e[q]) {
-                        c += rk[q] == sv[k0 + q].x;
-                        e[q] = nx[q];
-                        any |= e[q] != 0;
+                    for (int q = 0; q < G; q++) {
+                        const uint32_t i = e[q] ? e[q] - 1 : 0;
+                        rk[q] = rkey[i];
+                        nx[q] = nxt[i];
                     }
+#pragma unroll
+                    for (int q = 0; q < G; q++) {
+                        if (e[q]) {
+                            c += rk[q] == sv[k0 + q].x;
+                            e[q] = nx[q];
+                            any |= e[q] != 0;
+                        }
+                    }
+                    if (!any) break;
                 }
-                if (!any) break;
             }
         }
         // the next task, decoded by wave 0 while the other waves finish probing

@@ -160,7 +160,9 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     tbj_batch_tuning& T = P->tune;
     if (!T.target_r) T.target_r = kJoinCap / 2;
     if (!T.one_pass_bits) T.one_pass_bits = 8;
-    if (!T.s_chunk || T.s_chunk > (uint32_t)kJoinSChunk) T.s_chunk = kJoinSChunk;
+    // s_chunk: S tuples per task, probed in sub-chunks of kJoinSChunk
+    const uint32_t max_span = (uint32_t)kJoinSChunk * kJoinSTaskChunks;
+    if (!T.s_chunk || T.s_chunk > max_span) T.s_chunk = max_span;
     if (!T.wave_tuples) T.wave_tuples = 1u << 31;
     T.one_pass_bits = std::min<uint32_t>(T.one_pass_bits, kMaxPassBits);
     const uint32_t n = P->n;
@@ -369,7 +371,7 @@ int tbj_batch_autotune(int algo, const tbj_tuple* arena, const tbj_batch_item* i
     if (reps < 1) reps = 1;
     const uint32_t targets[] = {1024, 2048};
     const uint32_t onepass[] = {6, 8, 10};
-    const uint32_t chunks[] = {4096, 8192};
+    const uint32_t chunks[] = {8192, 32768};
     float bms = 1e30f;
     tbj_batch_tuning bt{};
     for (uint32_t tr : targets)

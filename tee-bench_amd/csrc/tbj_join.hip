@@ -503,46 +503,89 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // ---------------------------------------------------------------------------
 // Sort-merge task (RSM: radix_sortmerge_join.c:74-100; PSM's merge:
 // parallel_sortmerge_join.h:18-51).  The R range is sorted in LDS by
-// (key, payload) with a bitonic network; each S tuple then finds its
-// equal-key run with a lower-bound search and walks it.  The merge is
-// bounded on both sides (the reference's unbounded inner loops overrun the
-// arrays and miscount, parallel_sortmerge_join.h:26-46).
+// (key, payload) -- a bucket sort over the range's key span, each bucket
+// insertion-sorted by one thread, with a bitonic network when a bucket is
+// too large (clustered keys); every S tuple then finds its equal-key run with
+// a lower-bound search (several searches in lockstep per thread) and walks it.
+// The merge is bounded on both sides (the reference's unbounded inner loops
+// overrun the arrays and miscount, parallel_sortmerge_join.h:26-46).  A task
+// sorts its R range once and probes its S range in sub-chunks.
 
-__global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restrict__ Rb,
-                                                          const uint2* __restrict__ Sb,
-                                                          const JoinTask* __restrict__ tasks,
-                                                          const uint32_t* __restrict__ ntasks_dev,
-                                                          uint32_t* __restrict__ task_ctr,
-                                                          uint2* __restrict__ pairs, uint64_t cap,
-                                                          bool mat,
-                                                          unsigned long long* __restrict__ counters) {
-    __shared__ unsigned long long srt[kJoinCap];
-    __shared__ uint32_t scratch[kJoinBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t task_s;
-    __shared__ unsigned long long obase_s;
-    __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
-    const uint32_t ntasks = *ntasks_dev;
-    uint64_t matches = 0, cks = 0;
-    for (;;) {
-        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
-        __syncthreads();
-        const uint32_t t = task_s;
-        if (t >= ntasks) break;
-        const JoinTask tk = tasks[t];
-        const uint32_t n = tk.r_len;
-        const uint32_t np2 = next_pow2_u32(n);
-        for (uint32_t j = threadIdx.x; j < np2; j += kJoinBlock) {
-            if (j < n) {
-                const uint2 v = Rb[tk.r_off + j];
-                srt[j] = ((unsigned long long)v.x << 32) | v.y;
-            } else {
-                srt[j] = ~0ull;
+constexpr uint32_t kSortBucketMax = 32;
+
+// Sorts the n values v (item k of the thread at k * kJoinBlock + threadIdx.x)
+// into srt[0, n) ascending; cnt (n words), scratch and mm/big are LDS work
+// space.  Every thread must call it; ends with a barrier.
+template <int ITEMS>
+__device__ void lds_sort_u64(const unsigned long long (&v)[ITEMS], uint32_t n,
+                             unsigned long long* srt, uint32_t* cnt, uint32_t* scratch,
+                             uint32_t* mm, uint32_t* big) {
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) {
+        mm[0] = 0xFFFFFFFFu;
+        mm[1] = 0;
+        *big = 0;
+    }
+    for (uint32_t i = tid; i < n; i += kJoinBlock) cnt[i] = 0;
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        if (k * kJoinBlock + tid < n) {
+            const uint32_t key = (uint32_t)(v[k] >> 32);
+            mn = min(mn, key);
+            mx = max(mx, key);
+        }
+    }
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
+    }
+    __syncthreads();
+    if (lane_id() == 0) {
+        atomicMin(&mm[0], mn);
+        atomicMax(&mm[1], mx);
+    }
+    __syncthreads();
+    const uint32_t lo = mm[0];
+    const uint64_t span = (uint64_t)mm[1] - lo + 1;  // keys map monotonically to n buckets
+    uint32_t bk[ITEMS], rk[ITEMS];
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) {
+        if (k * kJoinBlock + tid < n) {
+            bk[k] = (uint32_t)(((uint64_t)((uint32_t)(v[k] >> 32) - lo) * n) / span);
+            rk[k] = atomicAdd(&cnt[bk[k]], 1u);
+        }
+    }
+    __syncthreads();
+    block_scan_lds<kJoinBlock>(cnt, n, scratch);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++)
+        if (k * kJoinBlock + tid < n) srt[cnt[bk[k]] + rk[k]] = v[k];
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += kJoinBlock) {
+        const uint32_t s0 = cnt[i], s1 = i + 1 < n ? cnt[i + 1] : n;
+        if (s1 - s0 > kSortBucketMax) {
+            *big = 1;
+        } else {
+            for (uint32_t x = s0 + 1; x < s1; x++) {
+                const unsigned long long t = srt[x];
+                uint32_t y = x;
+                while (y > s0 && srt[y - 1] > t) {
+                    srt[y] = srt[y - 1];
+                    y--;
+                }
+                srt[y] = t;
             }
         }
+    }
+    __syncthreads();
+    if (*big) {
+        const uint32_t np2 = next_pow2_u32(n);
+        for (uint32_t j = n + tid; j < np2; j += kJoinBlock) srt[j] = ~0ull;
         __syncthreads();
         for (uint32_t k = 2; k <= np2; k <<= 1) {
             for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < np2; i += kJoinBlock) {
+                for (uint32_t i = tid; i < np2; i += kJoinBlock) {
                     const uint32_t ixj = i ^ j;
                     if (ixj > i) {
                         const bool up = (i & k) == 0;
@@ -556,29 +599,70 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
                 __syncthreads();
             }
         }
-        for (uint32_t base = 0; base < tk.s_len; base += kJoinBlock * kJoinProbeItems) {
-            uint2 sv[kJoinProbeItems];
-            uint32_t lo[kJoinProbeItems];
+    }
+}
+
+__global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restrict__ Rb,
+                                                          const uint2* __restrict__ Sb,
+                                                          const JoinTask* __restrict__ tasks,
+                                                          const uint32_t* __restrict__ ntasks_dev,
+                                                          uint32_t* __restrict__ task_ctr,
+                                                          uint2* __restrict__ pairs, uint64_t cap,
+                                                          bool mat,
+                                                          unsigned long long* __restrict__ counters) {
+    constexpr int RITEMS = kJoinCap / kJoinBlock;
+    constexpr int G = kJoinProbeItems;  // S tuples searched in lockstep per thread
+    __shared__ unsigned long long srt[kJoinCap];
+    __shared__ uint32_t cnt[kJoinCap];
+    __shared__ uint32_t scratch[kJoinBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t mm[2], big;
+    __shared__ uint32_t task_s;
+    __shared__ unsigned long long obase_s;
+    __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
+    const uint32_t ntasks = *ntasks_dev;
+    uint64_t matches = 0, cks = 0;
+    for (;;) {
+        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+        __syncthreads();
+        const uint32_t t = task_s;
+        if (t >= ntasks) break;
+        const JoinTask tk = tasks[t];
+        const uint32_t n = tk.r_len;
+        {
+            unsigned long long v[RITEMS];
+#pragma unroll
+            for (int k = 0; k < RITEMS; k++) {
+                const uint32_t j = k * kJoinBlock + threadIdx.x;
+                const uint2 r = j < n ? Rb[tk.r_off + j] : make_uint2(0, 0);
+                v[k] = ((unsigned long long)r.x << 32) | r.y;
+            }
+            lds_sort_u64<RITEMS>(v, n, srt, cnt, scratch, mm, &big);
+        }
+        for (uint32_t base = 0; base < tk.s_len; base += kJoinBlock * G) {
+            uint2 sv[G];
+            uint32_t lo[G], hi[G];
             uint32_t c = 0;
 #pragma unroll
-            for (int k = 0; k < kJoinProbeItems; k++) {
+            for (int k = 0; k < G; k++) {
                 const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                sv[k] = j < tk.s_len ? Sb[tk.s_off + j] : make_uint2(0, 0);
+                const bool ok = j < tk.s_len;
+                sv[k] = ok ? Sb[tk.s_off + j] : make_uint2(0, 0);
+                lo[k] = ok ? 0 : n;  // first sorted entry with key >= S key
+                hi[k] = n;
+            }
+            for (uint32_t w = n; w; w >>= 1) {
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    if (lo[k] < hi[k]) {
+                        const uint32_t m = (lo[k] + hi[k]) >> 1;
+                        if (srt[m] < ((unsigned long long)sv[k].x << 32)) lo[k] = m + 1;
+                        else hi[k] = m;
+                    }
+                }
             }
 #pragma unroll
-            for (int k = 0; k < kJoinProbeItems; k++) {
-                const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                lo[k] = n;
-                if (j >= tk.s_len) continue;
-                const unsigned long long probe = (unsigned long long)sv[k].x << 32;
-                uint32_t L = 0, H = n;  // first sorted entry with key >= S key
-                while (L < H) {
-                    const uint32_t m = (L + H) >> 1;
-                    if (srt[m] < probe) L = m + 1;
-                    else H = m;
-                }
-                lo[k] = L;
-                for (uint32_t e = L; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
+            for (int k = 0; k < G; k++) {
+                for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
                     c++;
                     cks += (uint64_t)(uint32_t)srt[e] + sv[k].y;
                 }
@@ -593,7 +677,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
                 uint64_t o = obase_s + ex;
                 if (c) {
 #pragma unroll
-                    for (int k = 0; k < kJoinProbeItems; k++) {
+                    for (int k = 0; k < G; k++) {
                         for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
                             if (o < cap) pairs[o] = make_uint2((uint32_t)srt[e], sv[k].y);
                             o++;
@@ -605,22 +689,7 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
         }
         __syncthreads();
     }
-    matches = wave_reduce_u64(matches);
-    cks = wave_reduce_u64(cks);
-    if (lane_id() == 0) {
-        red_m[wave_id()] = matches;
-        red_c[wave_id()] = cks;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long m = 0, c2 = 0;
-        for (int w = 0; w < kJoinBlock / TBJ_WAVE; w++) {
-            m += red_m[w];
-            c2 += red_c[w];
-        }
-        if (m) atomicAdd(&counters[0], m);
-        if (c2) atomicAdd(&counters[1], c2);
-    }
+    flush_counters(matches, cks, red_m, red_c, counters);
 }
 
 // ---------------------------------------------------------------------------

@@ -281,9 +281,7 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
     TBJ_HIP(hipEventRecord(C.ev[1], st));
 
     // ---- join tasks
-    const uint32_t max_chunk = J.mode == kModeSort
-                                   ? (uint32_t)kJoinSChunk
-                                   : (uint32_t)join_s_chunk(J.mat) * kJoinSTaskChunks;
+    const uint32_t max_chunk = (uint32_t)join_s_chunk(J.mat) * kJoinSTaskChunks;
     const uint32_t s_chunk = (opt.s_chunk > 0 && (uint32_t)opt.s_chunk < max_chunk)
                                  ? (uint32_t)opt.s_chunk : max_chunk;
     launch_count_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, tpartial, &sm->ntasks,

@@ -58,14 +58,20 @@ struct BinFn {
     }
 };
 
-// Loads the cell table into LDS (no barrier: the caller's next barrier covers it).
+// GMODEL = false: loads the cell table into LDS (no barrier: the caller's
+// next barrier covers it).  GMODEL = true: reads it in place from the model
+// (global memory, cached), so that one tile per workgroup needs no copy.
+// A template parameter, so that each kernel's model reads keep one address
+// space (LDS reads stay ds_read).
+template <bool GMODEL = false>
 __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     BinFn f;
     f.bs = bs;
-    f.lm = lm;
+    f.lm = GMODEL && bs.kind == kBinLearned ? reinterpret_cast<const LdsModel*>(bs.model->cells)
+                                            : lm;
     f.kmin = f.kmax = f.cshift = f.ncells = f.last = 0;
     if (bs.kind == kBinLearned) {
-        load_model(lm, bs.model);
+        if (!GMODEL) load_model(lm, bs.model);
         f.kmin = bs.model->kmin;
         f.kmax = bs.model->kmax;
         f.cshift = bs.model->cshift;
@@ -272,6 +278,7 @@ __host__ __device__ constexpr size_t scatter_model_bytes(bool learned) {
     return learned ? (sizeof(LdsModel) + 15) / 16 * 16 : 0;
 }
 
+template <bool GMODEL>
 __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
@@ -285,9 +292,10 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
-    uint32_t* loff = reinterpret_cast<uint32_t*>(dyn_lds + scatter_model_bytes(bs.kind == kBinLearned));
+    uint32_t* loff = reinterpret_cast<uint32_t*>(
+        dyn_lds + scatter_model_bytes(!GMODEL && bs.kind == kBinLearned));
     uint32_t* goff = loff + nbins;
-    BinFn bin = make_binfn(bs, &lm);
+    BinFn bin = make_binfn<GMODEL>(bs, &lm);
     const uint32_t end = min(tile_end, *ntiles_dev);
     for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
@@ -709,10 +717,11 @@ void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDes
     const BinSpec b2 = bs2 ? *bs2 : bs;
     // the learned model is loaded into LDS once per workgroup: persistent
     // grid; radix: one tile per workgroup
+    // the learned model is loaded into LDS once per workgroup: persistent
+    // grid; radix: one tile per workgroup
     const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
-    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st,
-                       in, tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins,
-                       fine, b2, nbins2);
+    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles, segs,
+                       ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins, fine, b2, nbins2);
 }
 
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
@@ -720,10 +729,17 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
                     uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
                     uint32_t nbins) {
     if (!ntiles) return;
-    const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
-    const size_t lds = scatter_model_bytes(bs.kind == kBinLearned) + 2 * sizeof(uint32_t) * nbins;
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(kPartBlock), lds, st, in, out, tiles, segs,
-                       ntiles_dev, tile_base, tile_base + ntiles, scanned, bs, nbins);
+    // one tile per workgroup; the learned model is read in place (cached)
+    // rather than copied into LDS per tile
+    const size_t lds = 2 * sizeof(uint32_t) * nbins;
+    if (bs.kind == kBinLearned)
+        hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
+                           tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, scanned, bs,
+                           nbins);
+    else
+        hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
+                           tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, scanned, bs,
+                           nbins);
 }
 
 uint32_t scan_partials(uint64_t n) { return (uint32_t)((n + kScanChunk - 1) / kScanChunk) + 1; }

@@ -105,21 +105,26 @@ void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, ui
 void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,
                       uint32_t mask, const void* pool, uint2* pairs, uint64_t cap,
                       unsigned long long* counters);
-// CHT build: bits (nwords), cnt (nwords + 1), taken ((|R| + 2 + 31) / 32 words),
-// arr (|R| + 2 tuples), spill (|R|), spilled ((|R| + 31) / 32) must be zeroed
-// where noted by the caller (bits, taken, spilled, spill_ctr).
+// CHT state on the device: overflow count (zeroed by the caller), phase-A
+// spills, whether the dense array is exactly full, and the number of set bits.
+struct ChtState {
+    uint32_t spill_ctr, spill_a, dense, nset;
+};
+// CHT build: words (nwords {bitmap word, rank}, zeroed), cnt (nwords + 1),
+// taken ((|R| + 2 + 31) / 32 words, zeroed), arr (|R| + 2 tuples), spill
+// (|R|), spilled ((|R| + 31) / 32 words, zeroed).
 void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmask,
-                      uint32_t pmask, uint32_t* bits, uint32_t* cnt, uint32_t* partial,
-                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spill_ctr,
-                      uint32_t* spilled);
+                      uint32_t pmask, uint2* words, uint32_t* cnt, uint32_t* partial,
+                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spilled,
+                      ChtState* state);
 void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,
-                         const uint32_t* spill_ctr, uint32_t nspill, uint32_t ov_mask,
+                         const ChtState* state, uint32_t nspill, uint32_t ov_mask,
                          uint32_t* ov_used, uint2* ov);
-void launch_cht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, uint32_t bmask,
-                      uint32_t pmask, uint32_t ov_mask, const uint32_t* bits,
-                      const uint32_t* cnt, const uint32_t* taken, const uint2* arr,
-                      uint32_t nslots, const uint32_t* ov_used, const uint2* ov,
-                      const uint32_t* spill_ctr, uint2* pairs, uint64_t cap,
-                      unsigned long long* counters);
+// dense: ChtState::dense as read back after the build.
+void launch_cht_probe(hipStream_t st, bool mat, bool dense, const uint2* S, uint64_t nS,
+                      uint32_t bmask, uint32_t pmask, uint32_t ov_mask, const uint2* words,
+                      const uint32_t* taken, const uint2* arr, uint32_t nslots,
+                      const uint32_t* ov_used, const uint2* ov, const ChtState* state,
+                      uint2* pairs, uint64_t cap, unsigned long long* counters);
 
 }  // namespace tbj

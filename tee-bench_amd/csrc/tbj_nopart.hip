@@ -173,52 +173,60 @@ __device__ __forceinline__ uint32_t cht_next_in_part(uint32_t pos, uint32_t pmas
 }
 
 // Phase A: set the home bit, or the next bit inside the partition
-// (CHT::setBit).  Tuples that find both taken go to the overflow list.
+// (CHT::setBit).  Tuples that find both taken go to the overflow list.  The
+// bitmap lives in words[w].x; words[w].y later receives the rank of the
+// word's first bit, so a lookup reads one 8-byte entry.
 __global__ __launch_bounds__(256) void k_cht_bits(const uint2* __restrict__ R, uint64_t nR,
-                                                  ChtParams p, uint32_t* __restrict__ bits,
+                                                  ChtParams p, uint2* __restrict__ words,
                                                   uint32_t* __restrict__ spill,
-                                                  uint32_t* __restrict__ spill_ctr) {
+                                                  ChtState* __restrict__ state) {
+    uint32_t* bw = reinterpret_cast<uint32_t*>(words);
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t h = R[i].x & p.bmask;
         const uint32_t m = 1u << (h & 31);
-        if (!(atomicOr(&bits[h >> 5], m) & m)) continue;
+        if (!(atomicOr(&bw[2 * (h >> 5)], m) & m)) continue;
         const uint32_t h2 = cht_next_in_part(h, p.pmask);
         const uint32_t m2 = 1u << (h2 & 31);
-        if (!(atomicOr(&bits[h2 >> 5], m2) & m2)) continue;
-        spill[atomicAdd(spill_ctr, 1u)] = (uint32_t)i;
+        if (!(atomicOr(&bw[2 * (h2 >> 5)], m2) & m2)) continue;
+        spill[atomicAdd(&state->spill_ctr, 1u)] = (uint32_t)i;
     }
 }
 
-__global__ void k_cht_popc(const uint32_t* __restrict__ bits, uint32_t nwords,
+__global__ void k_cht_popc(const uint2* __restrict__ words, uint32_t nwords,
                            uint32_t* __restrict__ cnt) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nwords) cnt[i] = __popc(bits[i]);
+    if (i < nwords) cnt[i] = __popc(words[i].x);
     else if (i == nwords) cnt[i] = 0;
 }
 
-__device__ __forceinline__ uint32_t cht_pos(const uint32_t* bits, const uint32_t* cnt,
-                                            uint32_t h) {
-    const uint32_t w = h >> 5;
-    return cnt[w] + __popc(bits[w] & ((1u << (h & 31)) - 1u));
+// words[w].y = rank of word w's first set bit (the scanned popcounts)
+__global__ void k_cht_ranks(const uint32_t* __restrict__ cnt, uint32_t nwords,
+                            uint2* __restrict__ words) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nwords) words[i].y = cnt[i];
+}
+
+__device__ __forceinline__ uint32_t cht_pos(const uint2 w, uint32_t h) {
+    return w.y + __popc(w.x & ((1u << (h & 31)) - 1u));
 }
 
 // Phase C: place each tuple at the dense slot of its home bit or the next one
 // (CHT::setTuple); a slot is claimed through a taken-bit.  Tuples already in
 // the overflow list from phase A are skipped (marked in `spilled`).
 __global__ __launch_bounds__(256) void k_cht_place(const uint2* __restrict__ R, uint64_t nR,
-                                                   ChtParams p, const uint32_t* __restrict__ bits,
-                                                   const uint32_t* __restrict__ cnt,
+                                                   ChtParams p, const uint2* __restrict__ words,
                                                    const uint32_t* __restrict__ spilled,
                                                    uint32_t* __restrict__ taken,
                                                    uint2* __restrict__ arr,
                                                    uint32_t* __restrict__ spill,
-                                                   uint32_t* __restrict__ spill_ctr) {
+                                                   ChtState* __restrict__ state) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
          i += (uint64_t)gridDim.x * blockDim.x) {
         if (spilled[i >> 5] & (1u << (i & 31))) continue;
         const uint2 t = R[i];
-        const uint32_t pos = cht_pos(bits, cnt, t.x & p.bmask);
+        const uint32_t h = t.x & p.bmask;
+        const uint32_t pos = cht_pos(words[h >> 5], h);
         uint32_t m = 1u << (pos & 31);
         if (!(atomicOr(&taken[pos >> 5], m) & m)) {
             arr[pos] = t;
@@ -230,15 +238,30 @@ __global__ __launch_bounds__(256) void k_cht_place(const uint2* __restrict__ R, 
             arr[pos2] = t;
             continue;
         }
-        spill[atomicAdd(spill_ctr, 1u)] = (uint32_t)i;
+        spill[atomicAdd(&state->spill_ctr, 1u)] = (uint32_t)i;
     }
 }
 
+// After phase C: the dense array is exactly full when no tuple spilled in
+// phase C and none went past the last set bit's slot.  nset tuples then fill
+// slots [0, nset) and nothing lies beyond, so a probe can tell a taken slot
+// by its index alone.
+__global__ void k_cht_dense(const uint32_t* __restrict__ cnt, uint32_t nwords,
+                            const uint32_t* __restrict__ taken, ChtState* __restrict__ state) {
+    if (threadIdx.x != 0) return;
+    const uint32_t nset = cnt[nwords];
+    const bool past = (taken[nset >> 5] >> (nset & 31)) & 1u;
+    state->nset = nset;
+    state->dense = (state->spill_ctr == state->spill_a && !past) ? 1u : 0u;
+}
+
+// Marks the phase-A spills in `spilled` and records their number.
 __global__ void k_cht_mark_spilled(const uint32_t* __restrict__ spill,
-                                   const uint32_t* __restrict__ spill_ctr,
-                                   uint32_t* __restrict__ spilled) {
+                                   ChtState* __restrict__ state, uint32_t* __restrict__ spilled) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *spill_ctr) atomicOr(&spilled[spill[i] >> 5], 1u << (spill[i] & 31));
+    const uint32_t n = state->spill_ctr;
+    if (i == 0) state->spill_a = n;
+    if (i < n) atomicOr(&spilled[spill[i] >> 5], 1u << (spill[i] & 31));
 }
 
 // Overflow table: open addressing, linear probing, (key, payload) with an
@@ -254,26 +277,29 @@ __global__ void k_cht_overflow(const uint2* __restrict__ R, const uint32_t* __re
     ov[h] = t;
 }
 
-template <bool MAT>
+// DENSE: the dense array is exactly full (k_cht_dense), so slot p is taken
+// iff p < nset and the taken bitmap is not read.
+template <bool MAT, bool DENSE>
 __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
-    const uint2* __restrict__ S, uint64_t nS, ChtParams p, const uint32_t* __restrict__ bits,
-    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ taken,
-    const uint2* __restrict__ arr, uint32_t nslots, const uint32_t* __restrict__ ov_used,
-    const uint2* __restrict__ ov, const uint32_t* __restrict__ spill_ctr, uint2* __restrict__ pairs,
-    uint64_t cap, unsigned long long* __restrict__ counters) {
+    const uint2* __restrict__ S, uint64_t nS, ChtParams p, const uint2* __restrict__ words,
+    const uint32_t* __restrict__ taken, const uint2* __restrict__ arr, uint32_t nslots,
+    const uint32_t* __restrict__ ov_used, const uint2* __restrict__ ov,
+    const ChtState* __restrict__ state, uint2* __restrict__ pairs, uint64_t cap,
+    unsigned long long* __restrict__ counters) {
     __shared__ uint32_t scratch[kProbeBlock / TBJ_WAVE + 1];
     __shared__ unsigned long long obase_s;
     __shared__ unsigned long long red_m[kProbeBlock / TBJ_WAVE], red_c[kProbeBlock / TBJ_WAVE];
-    const bool any_ov = *spill_ctr != 0;
+    const bool any_ov = state->spill_ctr != 0;
+    const uint32_t nset = DENSE ? state->nset : 0;
     uint64_t matches = 0, cks = 0;
     const uint64_t per_round = (uint64_t)kProbeBlock * kProbeItems;
     for (uint64_t base = (uint64_t)blockIdx.x * per_round; base < nS;
          base += (uint64_t)gridDim.x * per_round) {
         // every level of the lookup is issued for all items before any is
-        // used: S, then bitmap word + prefix count, then the two dense slots
-        // and their taken bits
-        uint2 sv[kProbeItems];
-        uint32_t bw[kProbeItems], cw[kProbeItems], pos[kProbeItems];
+        // used: S, then the bitmap word with its rank, then the two dense
+        // slots (and their taken bits unless DENSE)
+        uint2 sv[kProbeItems], bw[kProbeItems];
+        uint32_t pos[kProbeItems];
         uint2 a0[kProbeItems], a1[kProbeItems];
         uint32_t t0[kProbeItems], t1[kProbeItems];
         uint32_t c = 0;
@@ -283,20 +309,17 @@ __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
             sv[k] = j < nS ? S[j] : make_uint2(0, 0);
         }
 #pragma unroll
-        for (int k = 0; k < kProbeItems; k++) {
-            const uint32_t w = (sv[k].x & p.bmask) >> 5;
-            bw[k] = bits[w];
-            cw[k] = cnt[w];
-        }
+        for (int k = 0; k < kProbeItems; k++) bw[k] = words[(sv[k].x & p.bmask) >> 5];
 #pragma unroll
         for (int k = 0; k < kProbeItems; k++) {
-            const uint32_t h = sv[k].x & p.bmask;
-            pos[k] = cw[k] + __popc(bw[k] & ((1u << (h & 31)) - 1u));
+            pos[k] = cht_pos(bw[k], sv[k].x & p.bmask);
             const uint32_t p0 = min(pos[k], nslots - 1), p1 = min(pos[k] + 1, nslots - 1);
             a0[k] = arr[p0];
             a1[k] = arr[p1];
-            t0[k] = taken[p0 >> 5];
-            t1[k] = taken[p1 >> 5];
+            if (!DENSE) {
+                t0[k] = taken[p0 >> 5];
+                t1[k] = taken[p1 >> 5];
+            }
         }
         uint32_t hit[kProbeItems];  // bit 0: slot pos matches, bit 1: slot pos + 1
 #pragma unroll
@@ -304,8 +327,16 @@ __global__ __launch_bounds__(kProbeBlock) void k_cht_probe(
             const uint64_t j = base + k * kProbeBlock + threadIdx.x;
             const uint32_t p0 = pos[k], p1 = pos[k] + 1;
             const bool ok = j < nS;
-            const bool m0 = ok && p0 < nslots && ((t0[k] >> (p0 & 31)) & 1u) && a0[k].x == sv[k].x;
-            const bool m1 = ok && p1 < nslots && ((t1[k] >> (p1 & 31)) & 1u) && a1[k].x == sv[k].x;
+            bool u0, u1;  // slot taken
+            if (DENSE) {
+                u0 = p0 < nset;
+                u1 = p1 < nset;
+            } else {
+                u0 = p0 < nslots && ((t0[k] >> (p0 & 31)) & 1u);
+                u1 = p1 < nslots && ((t1[k] >> (p1 & 31)) & 1u);
+            }
+            const bool m0 = ok && u0 && a0[k].x == sv[k].x;
+            const bool m1 = ok && u1 && a1[k].x == sv[k].x;
             hit[k] = (m0 ? 1u : 0u) | (m1 ? 2u : 0u);
             if (m0) cks += (uint64_t)a0[k].y + sv[k].y;
             if (m1) cks += (uint64_t)a1[k].y + sv[k].y;
@@ -407,44 +438,48 @@ void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, con
 }
 
 void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmask,
-                      uint32_t pmask, uint32_t* bits, uint32_t* cnt, uint32_t* partial,
-                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spill_ctr,
-                      uint32_t* spilled) {
+                      uint32_t pmask, uint2* words, uint32_t* cnt, uint32_t* partial,
+                      uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spilled,
+                      ChtState* state) {
     ChtParams p{bmask, pmask, 0};
     const uint32_t nwords = (bmask + 1) / 32;
     hipLaunchKernelGGL(k_cht_bits, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR, p,
-                       bits, spill, spill_ctr);
-    hipLaunchKernelGGL(k_cht_popc, dim3((nwords + 1 + 255) / 256), dim3(256), 0, st, bits, nwords,
-                       cnt);
+                       words, spill, state);
+    hipLaunchKernelGGL(k_cht_popc, dim3((nwords + 1 + 255) / 256), dim3(256), 0, st, words,
+                       nwords, cnt);
     launch_scan(st, cnt, (uint64_t)nwords + 1, partial, nullptr);
+    hipLaunchKernelGGL(k_cht_ranks, dim3((nwords + 255) / 256), dim3(256), 0, st, cnt, nwords,
+                       words);
     hipLaunchKernelGGL(k_cht_mark_spilled, dim3(grid_for(nR, 256, 1u << 30)), dim3(256), 0, st,
-                       spill, spill_ctr, spilled);
+                       spill, state, spilled);
     hipLaunchKernelGGL(k_cht_place, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR, p,
-                       bits, cnt, spilled, taken, arr, spill, spill_ctr);
+                       words, spilled, taken, arr, spill, state);
+    hipLaunchKernelGGL(k_cht_dense, dim3(1), dim3(64), 0, st, cnt, nwords, taken, state);
 }
 
 void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,
-                         const uint32_t* spill_ctr, uint32_t nspill, uint32_t ov_mask,
+                         const ChtState* state, uint32_t nspill, uint32_t ov_mask,
                          uint32_t* ov_used, uint2* ov) {
     if (!nspill) return;
     hipLaunchKernelGGL(k_cht_overflow, dim3((nspill + 255) / 256), dim3(256), 0, st, R, spill,
-                       spill_ctr, ov_mask, ov_used, ov);
+                       &state->spill_ctr, ov_mask, ov_used, ov);
 }
 
-void launch_cht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, uint32_t bmask,
-                      uint32_t pmask, uint32_t ov_mask, const uint32_t* bits,
-                      const uint32_t* cnt, const uint32_t* taken, const uint2* arr,
-                      uint32_t nslots, const uint32_t* ov_used, const uint2* ov,
-                      const uint32_t* spill_ctr, uint2* pairs, uint64_t cap,
-                      unsigned long long* counters) {
+void launch_cht_probe(hipStream_t st, bool mat, bool dense, const uint2* S, uint64_t nS,
+                      uint32_t bmask, uint32_t pmask, uint32_t ov_mask, const uint2* words,
+                      const uint32_t* taken, const uint2* arr, uint32_t nslots,
+                      const uint32_t* ov_used, const uint2* ov, const ChtState* state,
+                      uint2* pairs, uint64_t cap, unsigned long long* counters) {
     ChtParams p{bmask, pmask, ov_mask};
     const uint32_t g = grid_for(nS, kProbeBlock * kProbeItems, 16384);
-    if (mat)
-        hipLaunchKernelGGL(k_cht_probe<true>, dim3(g), dim3(kProbeBlock), 0, st, S, nS, p, bits,
-                           cnt, taken, arr, nslots, ov_used, ov, spill_ctr, pairs, cap, counters);
-    else
-        hipLaunchKernelGGL(k_cht_probe<false>, dim3(g), dim3(kProbeBlock), 0, st, S, nS, p, bits,
-                           cnt, taken, arr, nslots, ov_used, ov, spill_ctr, pairs, cap, counters);
+#define TBJ_CHT_PROBE(M, D)                                                                      \
+    hipLaunchKernelGGL((k_cht_probe<M, D>), dim3(g), dim3(kProbeBlock), 0, st, S, nS, p, words, \
+                       taken, arr, nslots, ov_used, ov, state, pairs, cap, counters)
+    if (mat && dense) TBJ_CHT_PROBE(true, true);
+    else if (mat) TBJ_CHT_PROBE(true, false);
+    else if (dense) TBJ_CHT_PROBE(false, true);
+    else TBJ_CHT_PROBE(false, false);
+#undef TBJ_CHT_PROBE
 }
 
 }  // namespace tbj

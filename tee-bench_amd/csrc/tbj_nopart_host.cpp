@@ -17,7 +17,8 @@ uint32_t next_pow2(uint64_t v) {
 }
 
 struct Small {
-    uint32_t pool_ctr, spill_ctr, pad[2];
+    uint32_t pool_ctr, pad[3];
+    ChtState cht;
     unsigned long long counters[4];
 };
 
@@ -94,36 +95,35 @@ int run_cht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tb
     const uint32_t nslots = (uint32_t)nR + 2;
     const uint32_t twords = (nslots + 31) / 32;
     const uint32_t swords = (uint32_t)((nR + 31) / 32);
-    uint32_t *bits = nullptr, *cnt = nullptr, *partial = nullptr, *taken = nullptr,
-             *spill = nullptr, *spilled = nullptr;
-    uint2* arr = nullptr;
-    TBJ_TRY(W.get("cht_bits", 4ull * nwords, (void**)&bits));
+    uint32_t *cnt = nullptr, *partial = nullptr, *taken = nullptr, *spill = nullptr,
+             *spilled = nullptr;
+    uint2 *words = nullptr, *arr = nullptr;
+    TBJ_TRY(W.get("cht_words", 8ull * nwords, (void**)&words));
     TBJ_TRY(W.get("cht_cnt", 4ull * (nwords + 1), (void**)&cnt));
     TBJ_TRY(W.get("cht_partial", 4ull * scan_partials((uint64_t)nwords + 1), (void**)&partial));
     TBJ_TRY(W.get("cht_taken", 4ull * twords, (void**)&taken));
     TBJ_TRY(W.get("cht_arr", 8ull * nslots, (void**)&arr));
     TBJ_TRY(W.get("cht_spill", 4ull * nR, (void**)&spill));
     TBJ_TRY(W.get("cht_spilled", 4ull * swords, (void**)&spilled));
-    launch_fill_u32(st, bits, nwords, 0);
+    launch_fill_u32(st, (uint32_t*)words, 2ull * nwords, 0);
     launch_fill_u32(st, taken, twords, 0);
     launch_fill_u32(st, spilled, swords, 0);
-    launch_cht_build(st, (const uint2*)R, nR, bmsize - 1, psize - 1, bits, cnt, partial, taken, arr,
-                     spill, &sm->spill_ctr, spilled);
+    launch_cht_build(st, (const uint2*)R, nR, bmsize - 1, psize - 1, words, cnt, partial, taken,
+                     arr, spill, spilled, &sm->cht);
     Small h;
     TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small), hipMemcpyDeviceToHost, st));
     TBJ_HIP(hipStreamSynchronize(st));
-    const uint32_t nspill = h.spill_ctr;
+    const uint32_t nspill = h.cht.spill_ctr;
     const uint32_t ovslots = next_pow2(2ull * nspill + 2);
     uint32_t* ov_used = nullptr;
     uint2* ov = nullptr;
     TBJ_TRY(W.get("cht_ov_used", 4ull * ovslots, (void**)&ov_used));
     TBJ_TRY(W.get("cht_ov", 8ull * ovslots, (void**)&ov));
     launch_fill_u32(st, ov_used, ovslots, 0);
-    launch_cht_overflow(st, (const uint2*)R, spill, &sm->spill_ctr, nspill, ovslots - 1, ov_used,
-                        ov);
+    launch_cht_overflow(st, (const uint2*)R, spill, &sm->cht, nspill, ovslots - 1, ov_used, ov);
     TBJ_HIP(hipEventRecord(C.ev[1], st));
-    launch_cht_probe(st, mat, (const uint2*)S, nS, bmsize - 1, psize - 1, ovslots - 1, bits, cnt,
-                     taken, arr, nslots, ov_used, ov, &sm->spill_ctr, (uint2*)pairs,
+    launch_cht_probe(st, mat, h.cht.dense != 0, (const uint2*)S, nS, bmsize - 1, psize - 1,
+                     ovslots - 1, words, taken, arr, nslots, ov_used, ov, &sm->cht, (uint2*)pairs,
                      mat ? pairs_cap : 0, sm->counters);
     TBJ_HIP(hipGetLastError());
     res->num_partitions = 128;

@@ -51,3 +51,20 @@ def test_empty_and_tiny(tbj, oracle, algo):
     assert res.matches == 2 and sorted(got[:, 1].tolist()) == [0, 1]
     res, _ = gu.run(tbj, algo, rel([]), rel([1]), materialize=False)
     assert res.matches == 0
+
+
+def test_cht_slot_past_last_bit(tbj, oracle):
+    # Two R tuples of key 31 with a 64-bit bitmap of 32-bit partitions: the
+    # second sets bit 0 (the partition wraps), so both place from rank 1 and
+    # one lands past the last set bit's slot, leaving slot 0 empty.  A prior
+    # run leaves key 0 in slot 0: the probe must not take the stale slot.
+    R0 = rel(np.arange(64))
+    res, _ = gu.run(tbj, "CHT", R0, rel([0, 5]))
+    assert res.matches == 2
+    R = rel([31, 31])
+    S = rel([0, 31, 1, 31, 32])
+    res, got = gu.run(tbj, "CHT", R, S)
+    _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert res.matches == 4 and np.array_equal(ga, ra) and np.array_equal(gb, sa)
+    assert res.checksum == gu.checksum(ra, sa)

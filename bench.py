@@ -21,7 +21,9 @@ HBM peak from its algorithmic bytes and its average duration (HIP events on
 the library's stream, over the timed steps); `pipeline_roofline` rates the
 whole step with the partitioned-join byte floor T x (|R| + |S|) +
 2T x (|R| + |S_in|) + P x matches (T, P = tuple and pair bytes, S_in = S
-tuples inside R's key range).  The CPU baseline is the reference's own RHO
+tuples inside R's key range); for the no-partitioning joins (--algo PHT,
+NPO_st, CHT) the rated kernel is the probe (T x |S| + (T + P) x matches) and
+the floor 2T x |R| + T x |S| + (T + P) x matches.  The CPU baseline is the reference's own RHO
 (built from its sources into oracle/_ref) timed on this host's cores on the
 same inputs.
 """
@@ -212,7 +214,19 @@ def bench_join32(args, tb):
               for k in tb.Result.PHASES}
     s_in = s_in_range(R, S) if args.workload == "c3" else nS
     pair_b = 8 * matches if mat else 0
-    kernels = {
+    nopart = algo in ("PHT", "NPO_st", "CHT")
+    if nopart:
+        # one shared table, no partitioning: the probe kernel is the step's
+        # longest (its time is the library's probe interval, HIP events)
+        probe_ms = statistics.mean(r.ms_join for r in results)
+        psym = "k_cht_probe" if algo == "CHT" else "k_pht_probe"
+        # S read once, one R tuple read per match, pairs written
+        kernels = {"probe": (probe_ms, 8 * nS + 8 * matches + pair_b, psym)}
+        # + R read once and written once into the table
+        b_alg = 16 * nR + 8 * nS + 8 * matches + pair_b
+    else:
+        kernels = None
+    kernels = kernels or {
         # k_join / k_join_mat: reads every partitioned tuple once, writes the pairs
         "join": (phases["join"], 8 * (nR + s_in) + pair_b, "k_join_mat" if mat else "k_join"),
         # second-pass scatter: reads and writes every surviving tuple once
@@ -222,7 +236,8 @@ def bench_join32(args, tb):
     }
     kname = max(kernels, key=lambda k: kernels[k][0])
     kms, kbytes, ksym = kernels[kname]
-    b_alg = 8 * (nR + nS) + 16 * (nR + s_in) + pair_b
+    if not nopart:
+        b_alg = 8 * (nR + nS) + 16 * (nR + s_in) + pair_b
     pipe = b_alg / (ms_per_step * 1e-3) / 1e9
     line = {
         "metric": METRIC, "value": value, "unit": "tuples/s", "n_gpus": 1,

@@ -34,8 +34,8 @@ __global__ void k_bbounds(const uint32_t* __restrict__ scanned, const SegDesc* _
     const uint32_t seg = passes == 1 ? 2 * j + rel : (2 * j + rel) * F1 + p / F2;
     const uint32_t bin = passes == 1 ? p : p % F2;
     const SegDesc sd = segs[seg];
-    bounds[i] = make_uint2(scanned[sd.hist_base + (uint64_t)bin * sd.ntiles],
-                           scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ntiles]);
+    bounds[i] = make_uint2(scanned[sd.hist_base + (uint64_t)bin * sd.ngroups],
+                           scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ngroups]);
 }
 
 struct TaskView {

@@ -30,7 +30,7 @@ struct Group {
     uint32_t nj;
     uint64_t tuples;
     uint32_t T1, T2max;
-    uint64_t L1, L2;
+    uint64_t L1, L2;       // group-sum lengths of the two passes
     uint32_t NP;
     uint64_t base_off;     // arena tuple offset of the wave window
     size_t segs_off;       // byte offsets into the plan's constant block
@@ -87,8 +87,8 @@ struct tbj_batch_plan {
     std::vector<Group> groups;
     uint32_t waves = 0;
     uint64_t tuples = 0;
-    DevBuf c1, c2, tiles1, hist1, tiles2, hist2, segs2, bounds, task_off, partial, konst, small,
-        total;
+    DevBuf c1, c2, tiles1, hist1, gs1, tiles2, hist2, gs2, segs2, bounds, task_off, partial,
+        konst, small, total;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -118,27 +118,33 @@ struct tbj_batch_plan {
             } else {
                 TileDesc* t1 = (TileDesc*)tiles1.p;
                 uint32_t* h1 = (uint32_t*)hist1.p;
+                uint32_t* q1 = (uint32_t*)gs1.p;
                 const BinSpec s1 = radix_pass(g.B, 0, g.F1);
                 launch_tiles_from_segs(st, segs1, 2 * g.nj, sm, t1, g.T1);
-                launch_hist(st, g.T1, base, t1, segs1, sm, 0, h1, s1, g.F1);
-                launch_fill_u32(st, h1 + (g.L1 - 1), 1, 0);
-                launch_scan(st, h1, g.L1, (uint32_t*)partial.p, sm + 6);
-                launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, segs1, sm, 0, h1, s1, g.F1);
+                launch_hist(st, g.T1, base, t1, sm, 0, h1, s1, g.F1);
+                launch_fill_u32(st, q1 + (g.L1 - 1), 1, 0);
+                launch_tile_offsets(st, g.T1, h1, t1, segs1, sm, g.F1, q1, g.L1,
+                                    (uint32_t*)partial.p, sm + 6);
+                launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, sm, 0, h1, s1, g.F1);
                 if (g.passes == 1) {
-                    batch_bounds(st, h1, segs1, g.nj, g.F1, 1, 1, bnd);
+                    batch_bounds(st, q1, segs1, g.nj, g.F1, 1, 1, bnd);
                     Rb = (const uint2*)c1.p;
                 } else {
                     TileDesc* t2 = (TileDesc*)tiles2.p;
                     uint32_t* h2 = (uint32_t*)hist2.p;
+                    uint32_t* q2 = (uint32_t*)gs2.p;
                     SegDesc* s2d = (SegDesc*)segs2.p;
                     const BinSpec s2 = radix_pass(g.B, g.b1, g.F2);
-                    launch_make_segments(st, h1, segs1, 2 * g.nj, g.F1, g.F2, s2d, t2, sm + 1,
+                    launch_make_segments(st, q1, segs1, 2 * g.nj, g.F1, g.F2, s2d, t2, sm + 1,
                                          g.T2max);
-                    launch_hist(st, g.T2max, (const uint2*)c1.p, t2, s2d, sm + 1, 0, h2, s2, g.F2);
-                    launch_scan(st, h2, g.L2, (uint32_t*)partial.p, sm + 7);
-                    launch_scatter(st, g.T2max, (const uint2*)c1.p, (uint2*)c2.p, t2, s2d, sm + 1,
-                                   0, h2, s2, g.F2);
-                    batch_bounds(st, h2, s2d, g.nj, g.F1, g.F2, 2, bnd);
+                    launch_hist(st, g.T2max, (const uint2*)c1.p, t2, sm + 1, 0, h2, s2, g.F2);
+                    // group sums past the segments' own (up to g.L2) are not
+                    // read: every bound lies before them
+                    launch_tile_offsets(st, g.T2max, h2, t2, s2d, sm + 1, g.F2, q2, g.L2,
+                                        (uint32_t*)partial.p, sm + 7);
+                    launch_scatter(st, g.T2max, (const uint2*)c1.p, (uint2*)c2.p, t2, sm + 1, 0,
+                                   h2, s2, g.F2);
+                    batch_bounds(st, q2, s2d, g.nj, g.F1, g.F2, 2, bnd);
                     Rb = (const uint2*)c2.p;
                 }
             }
@@ -182,6 +188,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
 
     std::vector<char> konst;
     size_t max_tup = 0, max_T1 = 0, max_T2 = 0, max_L1 = 0, max_L2 = 0, max_s2 = 0, max_NP = 0;
+    size_t max_H1 = 0, max_H2 = 0;
     auto align = [&](size_t a) { konst.resize((konst.size() + a - 1) / a * a); };
     size_t w0 = 0;
     while (w0 < n) {
@@ -223,6 +230,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             g.NP = g.nj * g.F1 * g.F2;
             std::vector<SegDesc> segs(2 * g.nj);
             uint32_t tiles = 0;
+            uint64_t groups = 0;
             for (uint32_t j = 0; j < g.nj; j++) {
                 const tbj_batch_item& it = items[js[j]];
                 for (int rel = 0; rel < 2; rel++) {
@@ -231,15 +239,21 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
                     d.n = (uint32_t)(rel ? it.nS : it.nR);
                     d.ntiles = (d.n + kTile - 1) / kTile;
                     d.first_tile = tiles;
-                    d.hist_base = (uint64_t)tiles * g.F1;
+                    d.ngroups = seg_groups(tiles, d.ntiles);
+                    d.pad = 0;
+                    d.hist_base = groups * g.F1;
                     tiles += d.ntiles;
+                    groups += d.ngroups;
                     g.tuples += d.n;
                 }
             }
             g.T1 = tiles;
-            g.L1 = (uint64_t)g.F1 * g.T1 + 1;
+            g.L1 = (uint64_t)g.F1 * groups + 1;
             g.T2max = g.passes == 2 ? (uint32_t)(g.tuples / kTile) + 2 * g.nj * g.F1 + 1 : 0;
-            g.L2 = g.passes == 2 ? (uint64_t)g.F2 * g.T2max + 1 : 0;
+            // second-pass groups: tiles / kGroupTiles plus two per segment
+            g.L2 = g.passes == 2
+                       ? (uint64_t)g.F2 * (g.T2max / kGroupTiles + 2ull * (2 * g.nj * g.F1) + 1) + 1
+                       : 0;
             align(64);
             g.segs_off = konst.size();
             konst.resize(konst.size() + segs.size() * sizeof(SegDesc));
@@ -252,6 +266,8 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             if (g.passes) max_tup = std::max<size_t>(max_tup, g.tuples);
             max_T1 = std::max<size_t>(max_T1, g.T1);
             max_T2 = std::max<size_t>(max_T2, g.T2max);
+            max_H1 = std::max<size_t>(max_H1, (size_t)g.F1 * g.T1);
+            max_H2 = std::max<size_t>(max_H2, (size_t)g.F2 * g.T2max);
             max_L1 = std::max<size_t>(max_L1, g.L1);
             max_L2 = std::max<size_t>(max_L2, g.L2);
             max_s2 = std::max<size_t>(max_s2, g.passes == 2 ? 2ull * g.nj * g.F1 : 0);
@@ -264,9 +280,11 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     TBJ_TRY(P->c1.get(sizeof(uint2) * max_tup));
     TBJ_TRY(P->c2.get(sizeof(uint2) * max_tup));
     TBJ_TRY(P->tiles1.get(sizeof(TileDesc) * max_T1));
-    TBJ_TRY(P->hist1.get(sizeof(uint32_t) * max_L1));
+    TBJ_TRY(P->hist1.get(sizeof(uint32_t) * max_H1));
+    TBJ_TRY(P->gs1.get(sizeof(uint32_t) * max_L1));
     TBJ_TRY(P->tiles2.get(sizeof(TileDesc) * max_T2));
-    TBJ_TRY(P->hist2.get(sizeof(uint32_t) * max_L2));
+    TBJ_TRY(P->hist2.get(sizeof(uint32_t) * max_H2));
+    TBJ_TRY(P->gs2.get(sizeof(uint32_t) * max_L2));
     TBJ_TRY(P->segs2.get(sizeof(SegDesc) * max_s2));
     TBJ_TRY(P->bounds.get(sizeof(uint2) * 2 * max_NP));
     TBJ_TRY(P->task_off.get(sizeof(uint32_t) * (max_NP + 1)));

@@ -41,8 +41,8 @@ __global__ void k_part_bounds(const uint32_t* __restrict__ scanned,
         bin = p % inner;
     }
     const SegDesc sd = segs[seg];
-    const uint32_t st = scanned[sd.hist_base + (uint64_t)bin * sd.ntiles];
-    const uint32_t en = scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ntiles];
+    const uint32_t st = scanned[sd.hist_base + (uint64_t)bin * sd.ngroups];
+    const uint32_t en = scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ngroups];
     bounds[i] = make_uint2(st, en);
 }
 

@@ -36,23 +36,36 @@ constexpr int kJoinSTaskChunks = 4;
 
 enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
-// Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev).
+// Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev); tile t's
+// bin counts go to hist[t * nbins, (t + 1) * nbins).
 // fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
-                 const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
+                 const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist,
+                 const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
                  const BinSpec* bs2 = nullptr, uint32_t nbins2 = 1);
+// Per-tile counts (rows of nbins) -> per-tile output offsets in place, for
+// the tiles below *ntiles_dev (at most max_tiles).  gsum (glen words, the
+// last one zero and glen - 1 = nbins * the groups of all segments) receives
+// the scanned group sums: partition (segment s, bin b) starts at
+// gsum[s.hist_base + b * s.ngroups].  partial: scan_partials(glen) words.
+void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, uint32_t* counts,
+                         const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
+                         uint32_t nbins, uint32_t* gsum, uint64_t glen, uint32_t* partial,
+                         uint32_t* grand_total);
+// offs: the per-tile offsets from launch_tile_offsets.
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
-                    const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                    uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
-                    uint32_t nbins);
+                    const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
                  uint32_t* grand_total);
+// Second-pass segments from the first pass's scanned group sums;
+// *ngroups2_dev (optional) receives the groups of all of them.
 void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1,
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
-                          TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2);
+                          TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2,
+                          uint32_t* ngroups2_dev = nullptr);
 // Tiles of kTile tuples over a segment list (segment first_tile / ntiles set).
 void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
                             const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles);
@@ -77,7 +90,8 @@ void launch_make_tiles1(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t
 void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t* count_dev,
                       uint32_t mult, uint64_t max_tail);
 
-// Final partition bounds of R and S: bounds[rel * nparts + p] = {start, end}.
+// Final partition bounds of R and S: bounds[rel * nparts + p] = {start, end}
+// (scanned: the scanned group sums).
 void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* segs,
                         uint32_t nbins_outer, uint32_t nbins_inner, uint32_t passes,
                         uint32_t nR, uint32_t nS, uint2* bounds);

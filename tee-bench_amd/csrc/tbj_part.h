@@ -1,18 +1,39 @@
 // Partition-function and work-descriptor types shared by host and device.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace tbj {
 
 // One contiguous input range partitioned independently (a relation in the
 // first pass, a first-pass partition of a relation in the second pass).
+//
+// 8-byte tuples: the histogram keeps one row of bin counts per tile (row t
+// = tile t of the pass, so every tile writes one contiguous row), and the
+// output offsets come from a scan over group sums.  Group g of a pass is
+// tiles [g * kGroupTiles, (g + 1) * kGroupTiles); a segment owns the
+// ngroups groups its tiles touch, and its group sums sit bin-major from
+// hist_base: (bin b, group g) at hist_base + b * ngroups + g - first_tile /
+// kGroupTiles.  Scanned in that order, the entry of bin b's first group is
+// where partition (segment, b) starts.
+// 16-byte tuples: hist_base locates the segment's bin-major per-tile counts
+// (bin b, tile slot k) at hist_base + b * ntiles + k; ngroups is unused.
 struct SegDesc {
-    uint64_t hist_base;   // first slot of this segment's bin-major histogram
+    uint64_t hist_base;
     uint32_t in_off;      // first input tuple
     uint32_t n;           // tuples
     uint32_t ntiles;      // tiles of the segment
     uint32_t first_tile;  // global index of its first tile
+    uint32_t ngroups;
+    uint32_t pad;
 };
+
+constexpr uint32_t kGroupTiles = 32;
+
+// Groups touched by tiles [first_tile, first_tile + ntiles).
+inline __host__ __device__ uint32_t seg_groups(uint32_t first_tile, uint32_t ntiles) {
+    return ntiles ? (first_tile + ntiles - 1) / kGroupTiles - first_tile / kGroupTiles + 1 : 0;
+}
 
 struct TileDesc {
     uint32_t in_off;      // first input tuple of the tile

@@ -3,11 +3,12 @@
 //
 // The first pass restates parallel_radix_partition (radix_join.c:629-703):
 // per-chunk histograms, exclusive offsets, scatter.  Here a chunk is a tile
-// of TILE tuples owned by one workgroup; the offsets of all (bin, tile)
-// pairs come from one bin-major device-wide scan, so every partition is one
-// contiguous run and a tile writes each bin as one coalesced run staged in
-// LDS (the role of the reference's software write-combining buffers,
-// radix_join.c:733-827).  The second pass restates serial_radix_partition
+// of TILE tuples owned by one workgroup; each tile writes its bin counts as
+// one contiguous row, and the offsets of all (bin, tile) pairs follow the
+// bin-major order of a device-wide scan over 32-tile group sums, so every
+// partition is one contiguous run and a tile writes each bin as one
+// coalesced run staged in LDS (the role of the reference's software
+// write-combining buffers, radix_join.c:733-827).  The second pass restates serial_radix_partition
 // (radix_join.c:573-619) with every first-pass partition as its own segment.
 #include "tbj_device.h"
 #include "tbj_kernels.h"
@@ -94,10 +95,10 @@ __device__ __forceinline__ bool sampled16(uint32_t idx) {
 
 // Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
 // ... below tile_end and *ntiles_dev, loading the partition function (the
-// learned model) into LDS once.
+// learned model) into LDS once.  Tile t writes its counts to row t of hist
+// (nbins words).
 __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ in,
                                                      const TileDesc* __restrict__ tiles,
-                                                     const SegDesc* __restrict__ segs,
                                                      const uint32_t* __restrict__ ntiles_dev,
                                                      uint32_t tile_base, uint32_t tile_end,
                                                      uint32_t* __restrict__ hist, BinSpec bs,
@@ -114,7 +115,6 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
         __syncthreads();
         const TileDesc td = tiles[t];
-        const SegDesc sd = segs[td.seg];
         uint32_t key[kPartItems];
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
@@ -135,9 +135,8 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
             }
         }
         __syncthreads();
-        const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-            hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
+            hist[(uint64_t)t * nbins + i] = cnt[i];
         __syncthreads();
     }
 }
@@ -270,6 +269,19 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(uint32_t* __restrict__
 // Scatter: load the tile, rank every tuple inside its bin with LDS atomics,
 // stage the tile bin-sorted in LDS, then stream each bin's run to its global
 // offset.  Consecutive lanes write consecutive slots of the same run.
+//
+// One tile per workgroup.  Workgroups go to the XCDs round-robin by
+// blockIdx.x; block b < n takes a tile of the (b % 8)-th eighth of the n
+// tiles, in order, so each XCD walks one contiguous range of tiles (n is
+// the launch's actual tile count: an eighth of an upper bound could leave
+// XCDs idle).  A
+// partition's output then holds one run per XCD (tiles are scanned in
+// index order) and its cache lines fill inside one L2 rather than in two
+// XCDs' partial write-backs.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
+    return x * q + min(x, r) + b / kNumXcd;
+}
 
 // Dynamic LDS of k_scatter: the learned model (if any), then the bin
 // offsets, sized by the pass's bin count so that a small pass leaves room
@@ -278,14 +290,14 @@ __host__ __device__ constexpr size_t scatter_model_bytes(bool learned) {
     return learned ? (sizeof(LdsModel) + 15) / 16 * 16 : 0;
 }
 
+// offs: row t = the output offset of each bin's run of tile t.
 template <bool GMODEL>
 __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
-                                                        const SegDesc* __restrict__ segs,
                                                         const uint32_t* __restrict__ ntiles_dev,
                                                         uint32_t tile_base, uint32_t tile_end,
-                                                        const uint32_t* __restrict__ scanned,
+                                                        const uint32_t* __restrict__ offs,
                                                         BinSpec bs, uint32_t nbins) {
     __shared__ uint2 stage[kTile];
     __shared__ uint16_t sbin[kTile];
@@ -296,8 +308,11 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
         dyn_lds + scatter_model_bytes(!GMODEL && bs.kind == kBinLearned));
     uint32_t* goff = loff + nbins;
     BinFn bin = make_binfn<GMODEL>(bs, &lm);
+    // the launch's tiles that exist (the grid may be an upper bound)
     const uint32_t end = min(tile_end, *ntiles_dev);
-    for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
+    const uint32_t n = end > tile_base ? end - tile_base : 0;
+    if (blockIdx.x < n) {
+        const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
         __syncthreads();
         const TileDesc td = tiles[t];
@@ -320,10 +335,8 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
         }
         __syncthreads();
         const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
-        const SegDesc sd = segs[td.seg];
-        const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-            goff[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis] - loff[i];
+            goff[i] = offs[(uint64_t)t * nbins + i] - loff[i];
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
             if (b[k] != TBJ_DROP) {
@@ -337,7 +350,6 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
             const uint32_t bb = sbin[j];
             out[goff[bb] + j] = stage[j];
         }
-        __syncthreads();
     }
 }
 
@@ -470,11 +482,11 @@ __global__ void k_bounds_est(const uint32_t* __restrict__ starts,
 __global__ __launch_bounds__(kScanBlock) void k_make_segments(
     const uint32_t* __restrict__ scanned1, const SegDesc* __restrict__ segs1, uint32_t nseg1,
     uint32_t nbins1, uint32_t nbins2, SegDesc* __restrict__ segs2,
-    uint32_t* __restrict__ ntiles2_dev) {
+    uint32_t* __restrict__ ntiles2_dev, uint32_t* __restrict__ ngroups2_dev) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t carry_s;
+    __shared__ uint32_t carry_s, gcarry_s;
     const uint32_t nseg2 = nseg1 * nbins1;
-    if (threadIdx.x == 0) carry_s = 0;
+    if (threadIdx.x == 0) carry_s = gcarry_s = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nseg2; base += kScanBlock) {
         const uint32_t s2 = base + threadIdx.x;
@@ -482,27 +494,38 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments(
         if (s2 < nseg2) {
             const uint32_t s1 = s2 / nbins1, b = s2 % nbins1;
             const SegDesc sd = segs1[s1];
-            st = scanned1[sd.hist_base + (uint64_t)b * sd.ntiles];
-            const uint32_t en = scanned1[sd.hist_base + (uint64_t)(b + 1) * sd.ntiles];
+            st = scanned1[sd.hist_base + (uint64_t)b * sd.ngroups];
+            const uint32_t en = scanned1[sd.hist_base + (uint64_t)(b + 1) * sd.ngroups];
             n = en - st;
             nt = (n + kTile - 1) / kTile;
         }
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kScanBlock>(nt, scratch, &total);
+        uint32_t total, gtotal;
+        const uint32_t ft = carry_s + block_exclusive_scan<kScanBlock>(nt, scratch, &total);
+        const uint32_t ng = seg_groups(ft, nt);
+        __syncthreads();
+        const uint32_t gex = block_exclusive_scan<kScanBlock>(ng, scratch, &gtotal);
         if (s2 < nseg2) {
             SegDesc d;
             d.in_off = st;
             d.n = n;
             d.ntiles = nt;
-            d.first_tile = carry_s + ex;
-            d.hist_base = (uint64_t)d.first_tile * nbins2;
+            d.first_tile = ft;
+            d.ngroups = ng;
+            d.pad = 0;
+            d.hist_base = (uint64_t)(gcarry_s + gex) * nbins2;
             segs2[s2] = d;
         }
         __syncthreads();
-        if (threadIdx.x == 0) carry_s += total;
+        if (threadIdx.x == 0) {
+            carry_s += total;
+            gcarry_s += gtotal;
+        }
         __syncthreads();
     }
-    if (threadIdx.x == 0) *ntiles2_dev = carry_s;
+    if (threadIdx.x == 0) {
+        *ntiles2_dev = carry_s;
+        if (ngroups2_dev) *ngroups2_dev = gcarry_s;
+    }
 }
 
 // One thread per second-pass tile: find the owning segment by binary search
@@ -669,14 +692,94 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
         a.n = nR;
         a.ntiles = tR;
         a.first_tile = 0;
-        b.hist_base = (uint64_t)nbins * tR;
+        a.ngroups = seg_groups(0, tR);
+        a.pad = 0;
+        b.hist_base = (uint64_t)nbins * a.ngroups;
         b.in_off = 0;
         b.n = nS;
         b.ntiles = tS;
         b.first_tile = tR;
+        b.ngroups = seg_groups(tR, tS);
+        b.pad = 0;
         segs[0] = a;
         segs[1] = b;
         *ntiles_dev = tR + tS;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Offsets from per-tile counts.  One workgroup per group of kGroupTiles
+// tiles, one thread per bin: the group's rows are loaded at once (each load
+// a contiguous row), then walked in tile order; a tile of another segment
+// than its predecessor starts a new sum (segments may be shorter than a
+// group).
+
+// gsum[(segment, bin, group)] = the counts of the group's tiles of that
+// segment.
+__global__ __launch_bounds__(256) void k_group_sums(const uint32_t* __restrict__ counts,
+                                                    const TileDesc* __restrict__ tiles,
+                                                    const SegDesc* __restrict__ segs,
+                                                    const uint32_t* __restrict__ ntiles_dev,
+                                                    uint32_t nbins, uint32_t* __restrict__ gsum) {
+    const uint32_t g = blockIdx.x, t0 = g * kGroupTiles, nt = *ntiles_dev;
+    if (t0 >= nt) return;
+    const uint32_t len = min((uint32_t)kGroupTiles, nt - t0);
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) {
+        uint32_t c[kGroupTiles];
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupTiles; k++)
+            c[k] = counts[(uint64_t)(t0 + min(k, len - 1)) * nbins + b];
+        uint32_t seg = tiles[t0].seg, acc = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupTiles; k++) {
+            if (k < len) {
+                const uint32_t sk = tiles[t0 + k].seg;
+                if (sk != seg) {
+                    const SegDesc sd = segs[seg];
+                    gsum[sd.hist_base + (uint64_t)b * sd.ngroups + g - sd.first_tile / kGroupTiles] =
+                        acc;
+                    seg = sk;
+                    acc = 0;
+                }
+                acc += c[k];
+            }
+        }
+        const SegDesc sd = segs[seg];
+        gsum[sd.hist_base + (uint64_t)b * sd.ngroups + g - sd.first_tile / kGroupTiles] = acc;
+    }
+}
+
+// counts -> offsets in place: a tile's bin offset is its group's scanned
+// sum plus the counts of the segment's earlier tiles in the group.
+__global__ __launch_bounds__(256) void k_group_offsets(uint32_t* __restrict__ counts,
+                                                       const TileDesc* __restrict__ tiles,
+                                                       const SegDesc* __restrict__ segs,
+                                                       const uint32_t* __restrict__ ntiles_dev,
+                                                       uint32_t nbins,
+                                                       const uint32_t* __restrict__ gsum) {
+    const uint32_t g = blockIdx.x, t0 = g * kGroupTiles, nt = *ntiles_dev;
+    if (t0 >= nt) return;
+    const uint32_t len = min((uint32_t)kGroupTiles, nt - t0);
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) {
+        uint32_t c[kGroupTiles];
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupTiles; k++)
+            c[k] = counts[(uint64_t)(t0 + min(k, len - 1)) * nbins + b];
+        uint32_t seg = ~0u, run = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kGroupTiles; k++) {
+            if (k < len) {
+                const uint32_t sk = tiles[t0 + k].seg;
+                if (sk != seg) {
+                    const SegDesc sd = segs[sk];
+                    run = gsum[sd.hist_base + (uint64_t)b * sd.ngroups + g -
+                               sd.first_tile / kGroupTiles];
+                    seg = sk;
+                }
+                counts[(uint64_t)(t0 + k) * nbins + b] = run;
+                run += c[k];
+            }
+        }
     }
 }
 
@@ -710,36 +813,45 @@ static uint32_t persistent_grid(uint32_t ntiles) {
 }
 
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
-                 const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-                 uint32_t* hist, const BinSpec& bs, uint32_t nbins, uint32_t* fine,
-                 const BinSpec* bs2, uint32_t nbins2) {
+                 const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist,
+                 const BinSpec& bs, uint32_t nbins, uint32_t* fine, const BinSpec* bs2,
+                 uint32_t nbins2) {
     if (!ntiles) return;
     const BinSpec b2 = bs2 ? *bs2 : bs;
     // the learned model is loaded into LDS once per workgroup: persistent
     // grid; radix: one tile per workgroup
-    // the learned model is loaded into LDS once per workgroup: persistent
-    // grid; radix: one tile per workgroup
     const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
-    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles, segs,
-                       ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins, fine, b2, nbins2);
+    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles,
+                       ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins, fine, b2,
+                       nbins2);
 }
 
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
-                    const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                    uint32_t tile_base, const uint32_t* scanned, const BinSpec& bs,
-                    uint32_t nbins) {
+                    const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
     const size_t lds = 2 * sizeof(uint32_t) * nbins;
     if (bs.kind == kBinLearned)
         hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
-                           tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, scanned, bs,
-                           nbins);
+                           tiles, ntiles_dev, tile_base, tile_base + ntiles, offs, bs, nbins);
     else
         hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
-                           tiles, segs, ntiles_dev, tile_base, tile_base + ntiles, scanned, bs,
-                           nbins);
+                           tiles, ntiles_dev, tile_base, tile_base + ntiles, offs, bs, nbins);
+}
+
+void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, uint32_t* counts,
+                         const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
+                         uint32_t nbins, uint32_t* gsum, uint64_t glen, uint32_t* partial,
+                         uint32_t* grand_total) {
+    const uint32_t ng = (max_tiles + kGroupTiles - 1) / kGroupTiles;
+    if (!ng) return;
+    hipLaunchKernelGGL(k_group_sums, dim3(ng), dim3(256), 0, st, counts, tiles, segs, ntiles_dev,
+                       nbins, gsum);
+    launch_scan(st, gsum, glen, partial, grand_total);
+    hipLaunchKernelGGL(k_group_offsets, dim3(ng), dim3(256), 0, st, counts, tiles, segs,
+                       ntiles_dev, nbins, gsum);
 }
 
 uint32_t scan_partials(uint64_t n) { return (uint32_t)((n + kScanChunk - 1) / kScanChunk) + 1; }
@@ -756,9 +868,10 @@ void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
 
 void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1,
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
-                          TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2) {
+                          TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2,
+                          uint32_t* ngroups2_dev) {
     hipLaunchKernelGGL(k_make_segments, dim3(1), dim3(kScanBlock), 0, st, scanned1, segs1, nseg1,
-                       nbins1, nbins2, segs2, ntiles2_dev);
+                       nbins1, nbins2, segs2, ntiles2_dev, ngroups2_dev);
     if (max_tiles2)
         hipLaunchKernelGGL(k_make_tiles2, dim3((max_tiles2 + 255) / 256), dim3(256), 0, st, segs2,
                            nseg1 * nbins1, ntiles2_dev, tiles2);

@@ -29,7 +29,7 @@ struct Small {  // device scalars, one allocation
     uint32_t stats[2];
     uint32_t minmax[2];
     uint32_t total1, total2, overflow, region_total;
-    uint32_t pad[4];
+    uint32_t ngroups2, pad[3];
     unsigned long long counters[4];
 };
 
@@ -145,13 +145,18 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
         const uint32_t T1 = tR + tS;
         TileDesc* tiles1 = nullptr;
         SegDesc* segs1 = nullptr;
-        uint32_t *hist1 = nullptr, *partial = nullptr, *fine = nullptr;
-        const uint64_t L1 = (uint64_t)plan.F1 * T1 + 1;
+        uint32_t *hist1 = nullptr, *gs1 = nullptr, *partial = nullptr, *fine = nullptr;
+        // per-tile counts (T1 rows) and the group sums of R's and S's tiles
+        const uint64_t L1 = (uint64_t)plan.F1 * (seg_groups(0, tR) + seg_groups(tR, tS)) + 1;
         TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
         TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
-        TBJ_TRY(W.get("hist1", sizeof(uint32_t) * L1, (void**)&hist1));
+        TBJ_TRY(W.get("hist1", sizeof(uint32_t) * plan.F1 * T1, (void**)&hist1));
+        TBJ_TRY(W.get("gs1", sizeof(uint32_t) * L1, (void**)&gs1));
         const uint32_t T2max = plan.passes == 2 ? (uint32_t)(ntot / kTile) + 2 * plan.F1 + 2 : 0;
-        const uint64_t L2max = exact2 ? (uint64_t)plan.F2 * T2max + 1 : 0;
+        // groups of the second pass: at most one more than tiles / kGroupTiles
+        // per segment, plus one straddling each segment start
+        const uint64_t G2max = T2max / kGroupTiles + 2ull * (2 * plan.F1) + 1;
+        const uint64_t L2max = exact2 ? (uint64_t)plan.F2 * G2max + 1 : 0;
         const uint64_t nsub = 2ull * plan.F1 * plan.F2;
         TBJ_TRY(W.get("partial",
                       sizeof(uint32_t) * scan_partials(std::max({L1, L2max, nsub + 1, (uint64_t)nparts + 1})),
@@ -188,18 +193,17 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
 
         // ---- pass 1 (exact): histogram, scan, scatter
         launch_make_tiles1(st, tiles1, segs1, nR, nS, plan.F1, &sm->ntiles1);
-        launch_hist(st, tR, Rb, tiles1, segs1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine, &s2,
-                    plan.F2);
-        launch_hist(st, tS, Sb, tiles1, segs1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2,
-                    plan.F2);
-        launch_fill_u32(st, hist1 + (L1 - 1), 1, 0);
+        launch_hist(st, tR, Rb, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine, &s2, plan.F2);
+        launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2, plan.F2);
+        launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
         TBJ_TRY(pt.mark(kPhHist1));
-        launch_scan(st, hist1, L1, partial, &sm->total1);
+        launch_tile_offsets(st, T1, hist1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1, partial,
+                            &sm->total1);
 
         if (learned) {
             // Judge the model on the first-pass R partitions; fall back to radix
             // when one would overflow the per-partition bound.
-            launch_part_bounds(st, hist1, segs1, 1, plan.F1, 1, nR, nS, bounds);
+            launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, bounds);
             launch_count_tasks(st, bounds, plan.F1, kJoinCap, 1u << 30, task_off, tpartial,
                                &sm->ntasks, sm->stats);
             Small h;
@@ -212,24 +216,25 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                 s1r = s1s = radix_spec(plan, 1);
                 s2 = radix_spec(plan, 2);
                 if (sample_fine) launch_fill_u32(st, fine, nsub, 0);
-                launch_hist(st, tR, Rb, tiles1, segs1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine,
-                            &s2, plan.F2);
-                launch_hist(st, tS, Sb, tiles1, segs1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine,
-                            &s2, plan.F2);
-                launch_fill_u32(st, hist1 + (L1 - 1), 1, 0);
-                launch_scan(st, hist1, L1, partial, &sm->total1);
+                launch_hist(st, tR, Rb, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine, &s2,
+                            plan.F2);
+                launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2,
+                            plan.F2);
+                launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
+                launch_tile_offsets(st, T1, hist1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1,
+                                    partial, &sm->total1);
             } else {
                 res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
             }
             launch_fill_u32(st, sm->stats, 2, 0);
         }
         TBJ_TRY(pt.mark(kPhScan1));
-        launch_scatter(st, tR, Rb, C1, tiles1, segs1, &sm->ntiles1, 0, hist1, s1r, plan.F1);
-        launch_scatter(st, tS, Sb, C1, tiles1, segs1, &sm->ntiles1, tR, hist1, s1s, plan.F1);
+        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1);
+        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1);
         TBJ_TRY(pt.mark(kPhScatter1));
 
         if (plan.passes == 1) {
-            launch_part_bounds(st, hist1, segs1, 1, plan.F1, 1, nR, nS, bounds);
+            launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, bounds);
             Rb = Sb = C1;
         } else {
             // ---- pass 2: every first-pass partition is a segment
@@ -237,21 +242,23 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
             SegDesc* segs2 = nullptr;
             TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2max, (void**)&tiles2));
             TBJ_TRY(W.get("segs2", sizeof(SegDesc) * 2 * plan.F1, (void**)&segs2));
-            launch_make_segments(st, hist1, segs1, 2, plan.F1, plan.F2, segs2, tiles2,
-                                 &sm->ntiles2, T2max);
+            launch_make_segments(st, gs1, segs1, 2, plan.F1, plan.F2, segs2, tiles2,
+                                 &sm->ntiles2, T2max, &sm->ngroups2);
             uint2* C2 = nullptr;
             if (exact2) {
-                uint32_t* hist2 = nullptr;
+                uint32_t *hist2 = nullptr, *gs2 = nullptr;
                 TBJ_TRY(W.get("c2", sizeof(uint2) * ntot, (void**)&C2));
-                TBJ_TRY(W.get("hist2", sizeof(uint32_t) * L2max, (void**)&hist2));
-                launch_hist(st, T2max, C1, tiles2, segs2, &sm->ntiles2, 0, hist2, s2, plan.F2);
-                launch_zero_tail(st, hist2, L2max, &sm->ntiles2, plan.F2,
-                                 (uint64_t)plan.F2 * (2 * plan.F1 + 2) + 1);
-                launch_scan(st, hist2, L2max, partial, &sm->total2);
+                TBJ_TRY(W.get("hist2", sizeof(uint32_t) * plan.F2 * T2max, (void**)&hist2));
+                TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2max, (void**)&gs2));
+                launch_hist(st, T2max, C1, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
+                // group sums past the segments' own are zero, so the scan's
+                // total is the tuple count
+                launch_zero_tail(st, gs2, L2max, &sm->ngroups2, plan.F2, L2max);
+                launch_tile_offsets(st, T2max, hist2, tiles2, segs2, &sm->ntiles2, plan.F2, gs2,
+                                    L2max, partial, &sm->total2);
                 TBJ_TRY(pt.mark(kPhSize2));
-                launch_scatter(st, T2max, C1, C2, tiles2, segs2, &sm->ntiles2, 0, hist2, s2,
-                               plan.F2);
-                launch_part_bounds(st, hist2, segs2, plan.F1, plan.F2, 2, nR, nS, bounds);
+                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
+                launch_part_bounds(st, gs2, segs2, plan.F1, plan.F2, 2, nR, nS, bounds);
             } else {
                 // regions sized from the 1/16 sample (ratio estimate inside each
                 // first-pass partition); room for 1.1x + 6 sigma + 64 tuples

@@ -96,23 +96,18 @@ __device__ __forceinline__ uint32_t block_scan_lds(uint32_t* a, uint32_t n, uint
     return total;
 }
 
-// Slot of a tile in its segment's offset order.  Workgroups are dispatched
-// to the 8 XCDs round-robin by blockIdx.x, so tiles whose block indices are
-// congruent mod 8 share an XCD and its L2.  Ordering a segment's tiles by
-// (XCD, tile) makes each XCD's runs of one partition adjacent in HBM, so a
-// cache line of a partition is filled inside one L2 instead of being split
-// between two XCDs' partial write-backs.
+// Tile of block b among a launch's n tiles (one tile per workgroup).
+// Workgroups go to the 8 XCDs round-robin by blockIdx.x; block b < n takes a
+// tile of the (b % 8)-th eighth of the n tiles, in order, so each XCD walks
+// one contiguous range of tiles.  Partitions are laid out with the tiles'
+// runs in tile order, so each partition then holds one run per XCD and its
+// cache lines fill inside one L2 rather than in two XCDs' partial
+// write-backs.  n must be the launch's actual tile count: eighths of an
+// upper bound could leave XCDs idle.
 constexpr uint32_t kNumXcd = 8;
-__device__ __forceinline__ uint32_t xcd_slot(uint32_t tis, uint32_t ntiles, uint32_t block) {
-    const uint32_t base_b = block - tis;  // block index of the segment's first tile
-    const uint32_t g = block % kNumXcd;
-    uint32_t before = 0;
-    for (uint32_t h = 0; h < g; h++) {
-        const uint32_t r0 = (h + kNumXcd - base_b % kNumXcd) % kNumXcd;
-        before += ntiles > r0 ? (ntiles - r0 + kNumXcd - 1) / kNumXcd : 0;
-    }
-    const uint32_t r0g = (g + kNumXcd - base_b % kNumXcd) % kNumXcd;
-    return before + (tis - r0g) / kNumXcd;
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+    const uint32_t x = b % kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
+    return x * q + min(x, r) + b / kNumXcd;
 }
 
 __device__ __forceinline__ uint32_t next_pow2_u32(uint32_t v) {

@@ -270,18 +270,7 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(uint32_t* __restrict__
 // stage the tile bin-sorted in LDS, then stream each bin's run to its global
 // offset.  Consecutive lanes write consecutive slots of the same run.
 //
-// One tile per workgroup.  Workgroups go to the XCDs round-robin by
-// blockIdx.x; block b < n takes a tile of the (b % 8)-th eighth of the n
-// tiles, in order, so each XCD walks one contiguous range of tiles (n is
-// the launch's actual tile count: an eighth of an upper bound could leave
-// XCDs idle).  A
-// partition's output then holds one run per XCD (tiles are scanned in
-// index order) and its cache lines fill inside one L2 rather than in two
-// XCDs' partial write-backs.
-__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
-    const uint32_t x = b % kNumXcd, q = n / kNumXcd, r = n % kNumXcd;
-    return x * q + min(x, r) + b / kNumXcd;
-}
+// One tile per workgroup, blocks mapped to tiles by xcd_tile.
 
 // Dynamic LDS of k_scatter: the learned model (if any), then the bin
 // offsets, sized by the pass's bin count so that a small pass leaves room

@@ -82,16 +82,17 @@ void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint
               uint32_t bits, uint64_t* sample, Model64* model);
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                      uint32_t nbins, uint32_t* ntiles_dev);
-// mm (optional): receives atomic min / max of the keys of the tiles.
+// Tile t's bin counts go to row t of hist (nbins words); mm (optional):
+// receives atomic min / max of the keys of the tiles.
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
-               const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-               uint32_t* hist, const Bin64& b, uint32_t nbins, unsigned long long* mm);
-// one_per_cu: request enough LDS that only one workgroup runs per CU (the
-// second pass writes fewer partial lines that way; measured on C5).
+               const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist, const Bin64& b,
+               uint32_t nbins, unsigned long long* mm);
+// Tiles [tile_base, tile_base + grid) below *ntiles_dev; offs: the rows of
+// per-tile offsets (launch_tile_offsets).  one_per_cu: request enough LDS
+// that only one workgroup runs per CU.
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
-                  const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins,
-                  bool one_per_cu = false);
+                  const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
+                  const uint32_t* offs, const Bin64& b, uint32_t nbins, bool one_per_cu = false);
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
                    uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
                    uint32_t max_tiles2);

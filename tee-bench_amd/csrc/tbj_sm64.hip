@@ -281,15 +281,16 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
 }
 
 // ---------------------------------------------------------------------------
-// Partition passes: per histogram tile, a bin-major count; one device scan;
-// then each tile scatters its stages through LDS.  First pass: segment 0 is
-// R and segment 1 is S; second pass: one segment per (relation, first-pass
-// bin).
+// Partition passes: per histogram tile, a row of bin counts; the counts
+// become per-tile offsets (group sums, scan, in-place offsets: the 8-byte
+// path's launch_tile_offsets); then each tile scatters its stages through
+// LDS.  First pass: segment 0 is R and segment 1 is S; second pass: one
+// segment per (relation, first-pass bin).
 
 __global__ __launch_bounds__(kSm64Block) void k_hist64(
     const ulonglong2* __restrict__ in, const TileDesc* __restrict__ tiles,
-    const SegDesc* __restrict__ segs, const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base,
-    uint32_t* __restrict__ hist, Bin64 b, uint32_t nbins, unsigned long long* __restrict__ mm) {
+    const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base, uint32_t* __restrict__ hist,
+    Bin64 b, uint32_t nbins, unsigned long long* __restrict__ mm) {
     __shared__ uint32_t cnt[kSm64MaxBins];
     __shared__ unsigned long long bmm[2];
     extern __shared__ __align__(16) uint8_t dyn_lds[];
@@ -336,21 +337,21 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         atomicMin(&mm[0], bmm[0]);
         atomicMax(&mm[1], bmm[1]);
     }
-    const SegDesc sd = segs[td.seg];
-    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
-        hist[sd.hist_base + (uint64_t)i * sd.ntiles + tis] = cnt[i];
+        hist[(uint64_t)t * nbins + i] = cnt[i];
 }
 
 __host__ __device__ constexpr size_t sm64_model_bytes_aligned(uint32_t kind) {
     return kind == kBinLearned ? (sizeof(LdsModel64) + 15) / 16 * 16 : 0;
 }
 
+// One tile per workgroup, blocks mapped to tiles by xcd_tile; offs row t =
+// tile t's output offset of each bin.
 __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out,
-    const TileDesc* __restrict__ tiles, const SegDesc* __restrict__ segs,
-    const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base,
-    const uint32_t* __restrict__ scanned, Bin64 b, uint32_t nbins) {
+    const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev,
+    uint32_t tile_base, uint32_t tile_end, const uint32_t* __restrict__ offs, Bin64 b,
+    uint32_t nbins) {
     __shared__ ulonglong2 stage[kSm64ScTile];
     __shared__ uint16_t sbin[kSm64ScTile];
     __shared__ uint32_t scratch[kSm64ScBlock / TBJ_WAVE + 1];
@@ -360,14 +361,14 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     uint32_t* loff = reinterpret_cast<uint32_t*>(dyn_lds + sm64_model_bytes_aligned(b.kind));
     uint32_t* goff = loff + nbins;
     uint32_t* next = goff + nbins;
-    const uint32_t t = tile_base + blockIdx.x;
-    if (t >= *ntiles_dev) return;
+    const uint32_t end = min(tile_end, *ntiles_dev);
+    const uint32_t n = end > tile_base ? end - tile_base : 0;
+    if (blockIdx.x >= n) return;
+    const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
     const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
     const TileDesc td = tiles[t];
-    const SegDesc sd = segs[td.seg];
-    const uint32_t tis = xcd_slot(t - sd.first_tile, sd.ntiles, t);
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock)
-        next[i] = scanned[sd.hist_base + (uint64_t)i * sd.ntiles + tis];
+        next[i] = offs[(uint64_t)t * nbins + i];
     ulonglong2 v[kSm64ScItems];
 #pragma unroll
     for (uint32_t k = 0; k < kSm64ScItems; k++) {
@@ -431,10 +432,12 @@ __global__ void k_make_tiles64(TileDesc* __restrict__ tiles, SegDesc* __restrict
         SegDesc a{}, c{};
         a.n = nR;
         a.ntiles = tR;
-        c.hist_base = (uint64_t)nbins * tR;
+        a.ngroups = seg_groups(0, tR);
+        c.hist_base = (uint64_t)nbins * a.ngroups;
         c.n = nS;
         c.ntiles = tS;
         c.first_tile = tR;
+        c.ngroups = seg_groups(tR, tS);
         segs[0] = a;
         segs[1] = c;
         *ntiles_dev = tR + tS;
@@ -445,9 +448,9 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments64(
     const uint32_t* __restrict__ scanned1, const SegDesc* __restrict__ segs1, uint32_t nbins1,
     uint32_t nbins2, SegDesc* __restrict__ segs2, uint32_t* __restrict__ ntiles2_dev) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t carry_s;
+    __shared__ uint32_t carry_s, gcarry_s;
     const uint32_t nseg2 = 2 * nbins1;
-    if (threadIdx.x == 0) carry_s = 0;
+    if (threadIdx.x == 0) carry_s = gcarry_s = 0;
     __syncthreads();
     for (uint32_t base = 0; base < nseg2; base += kScanBlock) {
         const uint32_t s2 = base + threadIdx.x;
@@ -455,23 +458,31 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments64(
         if (s2 < nseg2) {
             const SegDesc sd = segs1[s2 / nbins1];
             const uint32_t bb = s2 % nbins1;
-            st = scanned1[sd.hist_base + (uint64_t)bb * sd.ntiles];
-            n = scanned1[sd.hist_base + (uint64_t)(bb + 1) * sd.ntiles] - st;
+            st = scanned1[sd.hist_base + (uint64_t)bb * sd.ngroups];
+            n = scanned1[sd.hist_base + (uint64_t)(bb + 1) * sd.ngroups] - st;
             nt = (n + kSm64Span - 1) / kSm64Span;
         }
-        uint32_t total;
-        const uint32_t ex = block_exclusive_scan<kScanBlock>(nt, scratch, &total);
+        uint32_t total, gtotal;
+        const uint32_t ft = carry_s + block_exclusive_scan<kScanBlock>(nt, scratch, &total);
+        const uint32_t ng = seg_groups(ft, nt);
+        __syncthreads();
+        const uint32_t gex = block_exclusive_scan<kScanBlock>(ng, scratch, &gtotal);
         if (s2 < nseg2) {
             SegDesc d;
             d.in_off = st;
             d.n = n;
             d.ntiles = nt;
-            d.first_tile = carry_s + ex;
-            d.hist_base = (uint64_t)d.first_tile * nbins2;
+            d.first_tile = ft;
+            d.ngroups = ng;
+            d.pad = 0;
+            d.hist_base = (uint64_t)(gcarry_s + gex) * nbins2;
             segs2[s2] = d;
         }
         __syncthreads();
-        if (threadIdx.x == 0) carry_s += total;
+        if (threadIdx.x == 0) {
+            carry_s += total;
+            gcarry_s += gtotal;
+        }
         __syncthreads();
     }
     if (threadIdx.x == 0) *ntiles2_dev = carry_s;
@@ -516,8 +527,8 @@ __global__ void k_bounds64(const uint32_t* __restrict__ scanned, const SegDesc* 
     const uint32_t seg = passes == 1 ? rel : rel * outer + p / inner;
     const uint32_t bin = passes == 1 ? p : p % inner;
     const SegDesc sd = segs[seg];
-    bounds[i] = make_uint2(scanned[sd.hist_base + (uint64_t)bin * sd.ntiles],
-                           scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ntiles]);
+    bounds[i] = make_uint2(scanned[sd.hist_base + (uint64_t)bin * sd.ngroups],
+                           scanned[sd.hist_base + (uint64_t)(bin + 1) * sd.ngroups]);
 }
 
 // Per partition: R chunks and join tasks (R chunk x S chunk); word nparts is
@@ -1169,17 +1180,16 @@ void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR
 }
 
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
-               const SegDesc* segs, const uint32_t* ntiles_dev, uint32_t tile_base,
-               uint32_t* hist, const Bin64& b, uint32_t nbins, unsigned long long* mm) {
+               const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist, const Bin64& b,
+               uint32_t nbins, unsigned long long* mm) {
     if (!grid) return;
     hipLaunchKernelGGL(k_hist64, dim3(grid), dim3(kSm64Block), sm64_model_lds(b), st, in, tiles,
-                       segs, ntiles_dev, tile_base, hist, b, nbins, mm);
+                       ntiles_dev, tile_base, hist, b, nbins, mm);
 }
 
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
-                  const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                  uint32_t tile_base, const uint32_t* scanned, const Bin64& b, uint32_t nbins,
-                  bool one_per_cu) {
+                  const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
+                  const uint32_t* offs, const Bin64& b, uint32_t nbins, bool one_per_cu) {
     if (!grid) return;
     size_t lds = sm64_model_bytes_aligned(b.kind) + 3 * sizeof(uint32_t) * nbins;
     if (one_per_cu) {
@@ -1189,7 +1199,7 @@ void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong
         if (kStatic + lds <= kHalfCu) lds = kHalfCu - kStatic + 1024;
     }
     hipLaunchKernelGGL(k_scatter64, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
-                       tiles, segs, ntiles_dev, tile_base, scanned, b, nbins);
+                       tiles, ntiles_dev, tile_base, tile_base + grid, offs, b, nbins);
 }
 
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,

@@ -87,8 +87,8 @@ struct tbj_batch_plan {
     std::vector<Group> groups;
     uint32_t waves = 0;
     uint64_t tuples = 0;
-    DevBuf c1, c2, tiles1, hist1, gs1, tiles2, hist2, gs2, segs2, bounds, task_off, partial,
-        konst, small, total;
+    DevBuf c1, c2, tiles1, hist1, offs1, gs1, tiles2, hist2, offs2, gs2, segs2, bounds, task_off,
+        partial, konst, small, total;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -117,21 +117,23 @@ struct tbj_batch_plan {
                 batch_bounds(st, nullptr, segs1, g.nj, 1, 1, 0, bnd);
             } else {
                 TileDesc* t1 = (TileDesc*)tiles1.p;
-                uint32_t* h1 = (uint32_t*)hist1.p;
+                uint16_t* h1 = (uint16_t*)hist1.p;
+                uint32_t* o1 = (uint32_t*)offs1.p;
                 uint32_t* q1 = (uint32_t*)gs1.p;
                 const BinSpec s1 = radix_pass(g.B, 0, g.F1);
                 launch_tiles_from_segs(st, segs1, 2 * g.nj, sm, t1, g.T1);
                 launch_hist(st, g.T1, base, t1, sm, 0, h1, s1, g.F1);
                 launch_fill_u32(st, q1 + (g.L1 - 1), 1, 0);
-                launch_tile_offsets(st, g.T1, h1, t1, segs1, sm, g.F1, q1, g.L1,
+                launch_tile_offsets(st, g.T1, h1, o1, t1, segs1, sm, g.F1, q1, g.L1,
                                     (uint32_t*)partial.p, sm + 6);
-                launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, sm, 0, h1, s1, g.F1);
+                launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, sm, 0, o1, s1, g.F1);
                 if (g.passes == 1) {
                     batch_bounds(st, q1, segs1, g.nj, g.F1, 1, 1, bnd);
                     Rb = (const uint2*)c1.p;
                 } else {
                     TileDesc* t2 = (TileDesc*)tiles2.p;
-                    uint32_t* h2 = (uint32_t*)hist2.p;
+                    uint16_t* h2 = (uint16_t*)hist2.p;
+                    uint32_t* o2 = (uint32_t*)offs2.p;
                     uint32_t* q2 = (uint32_t*)gs2.p;
                     SegDesc* s2d = (SegDesc*)segs2.p;
                     const BinSpec s2 = radix_pass(g.B, g.b1, g.F2);
@@ -140,10 +142,10 @@ struct tbj_batch_plan {
                     launch_hist(st, g.T2max, (const uint2*)c1.p, t2, sm + 1, 0, h2, s2, g.F2);
                     // group sums past the segments' own (up to g.L2) are not
                     // read: every bound lies before them
-                    launch_tile_offsets(st, g.T2max, h2, t2, s2d, sm + 1, g.F2, q2, g.L2,
+                    launch_tile_offsets(st, g.T2max, h2, o2, t2, s2d, sm + 1, g.F2, q2, g.L2,
                                         (uint32_t*)partial.p, sm + 7);
                     launch_scatter(st, g.T2max, (const uint2*)c1.p, (uint2*)c2.p, t2, sm + 1, 0,
-                                   h2, s2, g.F2);
+                                   o2, s2, g.F2);
                     batch_bounds(st, q2, s2d, g.nj, g.F1, g.F2, 2, bnd);
                     Rb = (const uint2*)c2.p;
                 }
@@ -280,10 +282,12 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     TBJ_TRY(P->c1.get(sizeof(uint2) * max_tup));
     TBJ_TRY(P->c2.get(sizeof(uint2) * max_tup));
     TBJ_TRY(P->tiles1.get(sizeof(TileDesc) * max_T1));
-    TBJ_TRY(P->hist1.get(sizeof(uint32_t) * max_H1));
+    TBJ_TRY(P->hist1.get(sizeof(uint16_t) * max_H1));
+    TBJ_TRY(P->offs1.get(sizeof(uint32_t) * max_H1));
     TBJ_TRY(P->gs1.get(sizeof(uint32_t) * max_L1));
     TBJ_TRY(P->tiles2.get(sizeof(TileDesc) * max_T2));
-    TBJ_TRY(P->hist2.get(sizeof(uint32_t) * max_H2));
+    TBJ_TRY(P->hist2.get(sizeof(uint16_t) * max_H2));
+    TBJ_TRY(P->offs2.get(sizeof(uint32_t) * max_H2));
     TBJ_TRY(P->gs2.get(sizeof(uint32_t) * max_L2));
     TBJ_TRY(P->segs2.get(sizeof(SegDesc) * max_s2));
     TBJ_TRY(P->bounds.get(sizeof(uint2) * 2 * max_NP));

@@ -37,21 +37,23 @@ constexpr int kJoinSTaskChunks = 4;
 enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
 // Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev); tile t's
-// bin counts go to hist[t * nbins, (t + 1) * nbins).
+// bin counts go to hist[t * nbins, (t + 1) * nbins) (u16: a tile holds at
+// most kTile tuples).
 // fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
-                 const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist,
+                 const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
                  const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
                  const BinSpec* bs2 = nullptr, uint32_t nbins2 = 1);
-// Per-tile counts (rows of nbins) -> per-tile output offsets in place, for
-// the tiles below *ntiles_dev (at most max_tiles).  gsum (glen words, the
-// last one zero and glen - 1 = nbins * the groups of all segments) receives
-// the scanned group sums: partition (segment s, bin b) starts at
-// gsum[s.hist_base + b * s.ngroups].  partial: scan_partials(glen) words.
-void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, uint32_t* counts,
-                         const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                         uint32_t nbins, uint32_t* gsum, uint64_t glen, uint32_t* partial,
-                         uint32_t* grand_total);
+// Per-tile counts (u16 rows of nbins) -> per-tile output offsets (u32 rows
+// in offs), for the tiles below *ntiles_dev (at most max_tiles).  gsum (glen
+// words, the last one zero and glen - 1 >= nbins * the groups of all
+// segments) receives the scanned group sums: partition (segment s, bin b)
+// starts at gsum[s.hist_base + b * s.ngroups].  partial: scan_partials(glen)
+// words.
+void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,
+                         uint32_t* offs, const TileDesc* tiles, const SegDesc* segs,
+                         const uint32_t* ntiles_dev, uint32_t nbins, uint32_t* gsum,
+                         uint64_t glen, uint32_t* partial, uint32_t* grand_total);
 // offs: the per-tile offsets from launch_tile_offsets.
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,

@@ -96,12 +96,12 @@ __device__ __forceinline__ bool sampled16(uint32_t idx) {
 // Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
 // ... below tile_end and *ntiles_dev, loading the partition function (the
 // learned model) into LDS once.  Tile t writes its counts to row t of hist
-// (nbins words).
+// (nbins u16: a tile holds at most kTile tuples).
 __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ in,
                                                      const TileDesc* __restrict__ tiles,
                                                      const uint32_t* __restrict__ ntiles_dev,
                                                      uint32_t tile_base, uint32_t tile_end,
-                                                     uint32_t* __restrict__ hist, BinSpec bs,
+                                                     uint16_t* __restrict__ hist, BinSpec bs,
                                                      uint32_t nbins, uint32_t* __restrict__ fine,
                                                      BinSpec bs2, uint32_t nbins2) {
     __shared__ uint32_t cnt[kMaxBins];
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
         }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-            hist[(uint64_t)t * nbins + i] = cnt[i];
+            hist[(uint64_t)t * nbins + i] = (uint16_t)cnt[i];
         __syncthreads();
     }
 }
@@ -705,7 +705,7 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
 
 // gsum[(segment, bin, group)] = the counts of the group's tiles of that
 // segment.
-__global__ __launch_bounds__(256) void k_group_sums(const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(256) void k_group_sums(const uint16_t* __restrict__ counts,
                                                     const TileDesc* __restrict__ tiles,
                                                     const SegDesc* __restrict__ segs,
                                                     const uint32_t* __restrict__ ntiles_dev,
@@ -738,9 +738,10 @@ __global__ __launch_bounds__(256) void k_group_sums(const uint32_t* __restrict__
     }
 }
 
-// counts -> offsets in place: a tile's bin offset is its group's scanned
-// sum plus the counts of the segment's earlier tiles in the group.
-__global__ __launch_bounds__(256) void k_group_offsets(uint32_t* __restrict__ counts,
+// Per-tile offsets: a tile's bin offset is its group's scanned sum plus the
+// counts of the segment's earlier tiles in the group.
+__global__ __launch_bounds__(256) void k_group_offsets(const uint16_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ offs,
                                                        const TileDesc* __restrict__ tiles,
                                                        const SegDesc* __restrict__ segs,
                                                        const uint32_t* __restrict__ ntiles_dev,
@@ -765,7 +766,7 @@ __global__ __launch_bounds__(256) void k_group_offsets(uint32_t* __restrict__ co
                                sd.first_tile / kGroupTiles];
                     seg = sk;
                 }
-                counts[(uint64_t)(t0 + k) * nbins + b] = run;
+                offs[(uint64_t)(t0 + k) * nbins + b] = run;
                 run += c[k];
             }
         }
@@ -802,7 +803,7 @@ static uint32_t persistent_grid(uint32_t ntiles) {
 }
 
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
-                 const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist,
+                 const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
                  const BinSpec& bs, uint32_t nbins, uint32_t* fine, const BinSpec* bs2,
                  uint32_t nbins2) {
     if (!ntiles) return;
@@ -830,16 +831,16 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
                            tiles, ntiles_dev, tile_base, tile_base + ntiles, offs, bs, nbins);
 }
 
-void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, uint32_t* counts,
-                         const TileDesc* tiles, const SegDesc* segs, const uint32_t* ntiles_dev,
-                         uint32_t nbins, uint32_t* gsum, uint64_t glen, uint32_t* partial,
-                         uint32_t* grand_total) {
+void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,
+                         uint32_t* offs, const TileDesc* tiles, const SegDesc* segs,
+                         const uint32_t* ntiles_dev, uint32_t nbins, uint32_t* gsum,
+                         uint64_t glen, uint32_t* partial, uint32_t* grand_total) {
     const uint32_t ng = (max_tiles + kGroupTiles - 1) / kGroupTiles;
     if (!ng) return;
     hipLaunchKernelGGL(k_group_sums, dim3(ng), dim3(256), 0, st, counts, tiles, segs, ntiles_dev,
                        nbins, gsum);
     launch_scan(st, gsum, glen, partial, grand_total);
-    hipLaunchKernelGGL(k_group_offsets, dim3(ng), dim3(256), 0, st, counts, tiles, segs,
+    hipLaunchKernelGGL(k_group_offsets, dim3(ng), dim3(256), 0, st, counts, offs, tiles, segs,
                        ntiles_dev, nbins, gsum);
 }
 

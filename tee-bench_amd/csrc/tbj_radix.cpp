@@ -145,12 +145,15 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
         const uint32_t T1 = tR + tS;
         TileDesc* tiles1 = nullptr;
         SegDesc* segs1 = nullptr;
-        uint32_t *hist1 = nullptr, *gs1 = nullptr, *partial = nullptr, *fine = nullptr;
-        // per-tile counts (T1 rows) and the group sums of R's and S's tiles
+        uint16_t* hist1 = nullptr;
+        uint32_t *offs1 = nullptr, *gs1 = nullptr, *partial = nullptr, *fine = nullptr;
+        // per-tile counts and offsets (T1 rows) and the group sums of R's and
+        // S's tiles
         const uint64_t L1 = (uint64_t)plan.F1 * (seg_groups(0, tR) + seg_groups(tR, tS)) + 1;
         TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
         TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
-        TBJ_TRY(W.get("hist1", sizeof(uint32_t) * plan.F1 * T1, (void**)&hist1));
+        TBJ_TRY(W.get("hist1", sizeof(uint16_t) * plan.F1 * T1, (void**)&hist1));
+        TBJ_TRY(W.get("offs1", sizeof(uint32_t) * plan.F1 * T1, (void**)&offs1));
         TBJ_TRY(W.get("gs1", sizeof(uint32_t) * L1, (void**)&gs1));
         const uint32_t T2max = plan.passes == 2 ? (uint32_t)(ntot / kTile) + 2 * plan.F1 + 2 : 0;
         // groups of the second pass: at most one more than tiles / kGroupTiles
@@ -197,8 +200,8 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
         launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2, plan.F2);
         launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
         TBJ_TRY(pt.mark(kPhHist1));
-        launch_tile_offsets(st, T1, hist1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1, partial,
-                            &sm->total1);
+        launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1,
+                            partial, &sm->total1);
 
         if (learned) {
             // Judge the model on the first-pass R partitions; fall back to radix
@@ -221,16 +224,16 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                 launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2,
                             plan.F2);
                 launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
-                launch_tile_offsets(st, T1, hist1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1,
-                                    partial, &sm->total1);
+                launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1,
+                                    gs1, L1, partial, &sm->total1);
             } else {
                 res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
             }
             launch_fill_u32(st, sm->stats, 2, 0);
         }
         TBJ_TRY(pt.mark(kPhScan1));
-        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1);
-        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1);
+        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, plan.F1);
+        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, plan.F1);
         TBJ_TRY(pt.mark(kPhScatter1));
 
         if (plan.passes == 1) {
@@ -246,18 +249,20 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                                  &sm->ntiles2, T2max, &sm->ngroups2);
             uint2* C2 = nullptr;
             if (exact2) {
-                uint32_t *hist2 = nullptr, *gs2 = nullptr;
+                uint16_t* hist2 = nullptr;
+                uint32_t *offs2 = nullptr, *gs2 = nullptr;
                 TBJ_TRY(W.get("c2", sizeof(uint2) * ntot, (void**)&C2));
-                TBJ_TRY(W.get("hist2", sizeof(uint32_t) * plan.F2 * T2max, (void**)&hist2));
+                TBJ_TRY(W.get("hist2", sizeof(uint16_t) * plan.F2 * T2max, (void**)&hist2));
+                TBJ_TRY(W.get("offs2", sizeof(uint32_t) * plan.F2 * T2max, (void**)&offs2));
                 TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2max, (void**)&gs2));
                 launch_hist(st, T2max, C1, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
                 // group sums past the segments' own are zero, so the scan's
                 // total is the tuple count
                 launch_zero_tail(st, gs2, L2max, &sm->ngroups2, plan.F2, L2max);
-                launch_tile_offsets(st, T2max, hist2, tiles2, segs2, &sm->ntiles2, plan.F2, gs2,
-                                    L2max, partial, &sm->total2);
+                launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, plan.F2,
+                                    gs2, L2max, partial, &sm->total2);
                 TBJ_TRY(pt.mark(kPhSize2));
-                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
+                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, plan.F2);
                 launch_part_bounds(st, gs2, segs2, plan.F1, plan.F2, 2, nR, nS, bounds);
             } else {
                 // regions sized from the 1/16 sample (ratio estimate inside each

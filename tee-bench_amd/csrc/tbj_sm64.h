@@ -82,10 +82,11 @@ void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint
               uint32_t bits, uint64_t* sample, Model64* model);
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                      uint32_t nbins, uint32_t* ntiles_dev);
-// Tile t's bin counts go to row t of hist (nbins words); mm (optional):
+// Tile t's bin counts go to row t of hist (nbins u16: a tile holds at most
+// kSm64Span tuples); mm (optional):
 // receives atomic min / max of the keys of the tiles.
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
-               const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist, const Bin64& b,
+               const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist, const Bin64& b,
                uint32_t nbins, unsigned long long* mm);
 // Tiles [tile_base, tile_base + grid) below *ntiles_dev; offs: the rows of
 // per-tile offsets (launch_tile_offsets).  one_per_cu: request enough LDS

@@ -289,7 +289,7 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
 
 __global__ __launch_bounds__(kSm64Block) void k_hist64(
     const ulonglong2* __restrict__ in, const TileDesc* __restrict__ tiles,
-    const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base, uint32_t* __restrict__ hist,
+    const uint32_t* __restrict__ ntiles_dev, uint32_t tile_base, uint16_t* __restrict__ hist,
     Bin64 b, uint32_t nbins, unsigned long long* __restrict__ mm) {
     __shared__ uint32_t cnt[kSm64MaxBins];
     __shared__ unsigned long long bmm[2];
@@ -338,7 +338,7 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         atomicMax(&mm[1], bmm[1]);
     }
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
-        hist[(uint64_t)t * nbins + i] = cnt[i];
+        hist[(uint64_t)t * nbins + i] = (uint16_t)cnt[i];
 }
 
 __host__ __device__ constexpr size_t sm64_model_bytes_aligned(uint32_t kind) {
@@ -1180,7 +1180,7 @@ void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR
 }
 
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
-               const uint32_t* ntiles_dev, uint32_t tile_base, uint32_t* hist, const Bin64& b,
+               const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist, const Bin64& b,
                uint32_t nbins, unsigned long long* mm) {
     if (!grid) return;
     hipLaunchKernelGGL(k_hist64, dim3(grid), dim3(kSm64Block), sm64_model_lds(b), st, in, tiles,

@@ -118,10 +118,12 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
             passes == 2 ? (uint64_t)F2 * (T2max / kGroupTiles + 2ull * (2 * F1) + 1) + 1 : 0;
         TileDesc *tiles1 = nullptr, *tiles2 = nullptr;
         SegDesc *segs1 = nullptr, *segs2 = nullptr;
-        uint32_t *hist1 = nullptr, *hist2 = nullptr, *gs1 = nullptr, *gs2 = nullptr;
+        uint16_t *hist1 = nullptr, *hist2 = nullptr;
+        uint32_t *offs1 = nullptr, *offs2 = nullptr, *gs1 = nullptr, *gs2 = nullptr;
         TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
         TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
-        TBJ_TRY(W.get("hist1", sizeof(uint32_t) * F1 * T1, (void**)&hist1));
+        TBJ_TRY(W.get("hist1", sizeof(uint16_t) * F1 * T1, (void**)&hist1));
+        TBJ_TRY(W.get("offs1", sizeof(uint32_t) * F1 * T1, (void**)&offs1));
         TBJ_TRY(W.get("gs1", sizeof(uint32_t) * L1, (void**)&gs1));
         TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(std::max({L1, L2, (uint64_t)nparts + 1})),
                       (void**)&partial));
@@ -151,11 +153,11 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
         sm64_hist(st, tS, Sin, tiles1, &sm->ntiles1, tR, hist1, s1s, F1, nullptr);
         launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
         TBJ_TRY(pt.mark(kPhHist1));
-        launch_tile_offsets(st, T1, hist1, tiles1, segs1, &sm->ntiles1, F1, gs1, L1, partial,
-                            &sm->total1);
+        launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, F1, gs1, L1,
+                            partial, &sm->total1);
         TBJ_TRY(pt.mark(kPhScan1));
-        sm64_scatter(st, tR, Rin, C1, tiles1, &sm->ntiles1, 0, hist1, s1r, F1);
-        sm64_scatter(st, tS, Sin, C1, tiles1, &sm->ntiles1, tR, hist1, s1s, F1);
+        sm64_scatter(st, tR, Rin, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, F1);
+        sm64_scatter(st, tS, Sin, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, F1);
         TBJ_TRY(pt.mark(kPhScatter1));
         if (passes == 1) {
             sm64_bounds(st, gs1, segs1, 1, F1, 1, nR, nS, bounds);
@@ -165,16 +167,17 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
             // ---- pass 2: one segment per (relation, first-pass partition)
             TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2max, (void**)&tiles2));
             TBJ_TRY(W.get("segs2", sizeof(SegDesc) * 2 * F1, (void**)&segs2));
-            TBJ_TRY(W.get("hist2", sizeof(uint32_t) * F2 * T2max, (void**)&hist2));
+            TBJ_TRY(W.get("hist2", sizeof(uint16_t) * F2 * T2max, (void**)&hist2));
+            TBJ_TRY(W.get("offs2", sizeof(uint32_t) * F2 * T2max, (void**)&offs2));
             TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2, (void**)&gs2));
             sm64_segments(st, gs1, segs1, F1, F2, segs2, tiles2, &sm->ntiles2, T2max);
             sm64_hist(st, T2max, C1, tiles2, &sm->ntiles2, 0, hist2, s2, F2, nullptr);
             // group sums past the segments' own (never written) only follow
             // every partition bound in the scan, so they need no clearing
-            launch_tile_offsets(st, T2max, hist2, tiles2, segs2, &sm->ntiles2, F2, gs2, L2,
+            launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, F2, gs2, L2,
                                 partial, &sm->total2);
             TBJ_TRY(pt.mark(kPhSize2));
-            sm64_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, hist2, s2, F2, true);
+            sm64_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, F2, true);
             sm64_bounds(st, gs2, segs2, F1, F2, 2, nR, nS, bounds);
             TBJ_TRY(pt.mark(kPhScatter2));
             Pb = C2;

@@ -8,16 +8,16 @@
 
 namespace tbj {
 
-// Partition passes: one histogram tile is kSm64Span tuples (128 KiB), so the
-// per-tile histograms stay small next to the data; the scatter's workgroup of
-// kSm64ScBlock threads stages kSm64ScTile tuples (64 KiB) of it in LDS at a
-// time.  The other 64-bit kernels use workgroups of kSm64Block threads.
+// Partition passes: one tile is kSm64Span tuples (128 KiB), so the per-tile
+// histograms stay small next to the data; the scatter's workgroup of
+// kSm64ScBlock threads stages the whole tile in LDS.  The other 64-bit
+// kernels use workgroups of kSm64Block threads.
 constexpr int kSm64Block = 512;
 constexpr int kSm64Span = 8192;
 constexpr int kSm64ScBlock = 1024;
-constexpr int kSm64ScItems = 4;
+constexpr int kSm64ScItems = 8;
 constexpr int kSm64ScTile = kSm64ScBlock * kSm64ScItems;
-static_assert(kSm64Span % kSm64ScTile == 0, "scatter stages must tile a histogram tile");
+static_assert(kSm64Span == kSm64ScTile, "the scatter stages a whole tile");
 constexpr int kSm64PassBits = 10;
 constexpr int kSm64MaxBins = 1 << kSm64PassBits;
 // Join: R chunks of at most kSm64Cap tuples are sorted in LDS; a task is one
@@ -89,11 +89,10 @@ void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDe
                const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist, const Bin64& b,
                uint32_t nbins, unsigned long long* mm);
 // Tiles [tile_base, tile_base + grid) below *ntiles_dev; offs: the rows of
-// per-tile offsets (launch_tile_offsets).  one_per_cu: request enough LDS
-// that only one workgroup runs per CU.
+// per-tile offsets (launch_tile_offsets).
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
                   const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                  const uint32_t* offs, const Bin64& b, uint32_t nbins, bool one_per_cu = false);
+                  const uint32_t* offs, const Bin64& b, uint32_t nbins);
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
                    uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
                    uint32_t max_tiles2);

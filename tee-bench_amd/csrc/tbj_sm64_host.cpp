@@ -177,7 +177,7 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
             launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, F2, gs2, L2,
                                 partial, &sm->total2);
             TBJ_TRY(pt.mark(kPhSize2));
-            sm64_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, F2, true);
+            sm64_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, F2);
             sm64_bounds(st, gs2, segs2, F1, F2, 2, nR, nS, bounds);
             TBJ_TRY(pt.mark(kPhScatter2));
             Pb = C2;

@@ -128,9 +128,14 @@ def test_out_of_range_s_is_dropped(tbj, oracle):
     check_pairs(oracle, R, S, got, res, ordered=True)
 
 
+# 512 and 1024 bins in a pass exercise the scatter's byte-wide staged bins
+# (bin >> 1, bin >> 2 and the low bits from the bin starts); learned with 20
+# bits puts the model and 1024 bins into one workgroup's LDS
 @pytest.mark.parametrize("opts", [dict(radix_bits=9, passes=1), dict(radix_bits=16, passes=2),
                                   dict(radix_bits=4), dict(partitioner="learned", radix_bits=12),
-                                  dict(partitioner="learned", radix_bits=17)])
+                                  dict(partitioner="learned", radix_bits=17),
+                                  dict(radix_bits=10, passes=1), dict(radix_bits=20, passes=2),
+                                  dict(partitioner="learned", radix_bits=20)])
 def test_forced_plans(tbj, oracle, opts):
     R = tbj.gen_unique64(13, 200000)
     S = tbj.gen_zipf_fk64(14, 200000, R, 1.2)

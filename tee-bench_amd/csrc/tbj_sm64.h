@@ -107,12 +107,17 @@ void sm64_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, const
 // Sorts every R chunk by key in place (bucket sort in LDS).
 void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rtasks,
                  uint32_t nrtasks);
+// Sorts the R chunks whose tasks are several (SmTask::multi) from src into
+// dst (same offsets).
+void sm64_sort_shared(hipStream_t st, const ulonglong2* src, ulonglong2* dst,
+                      const SmTask* rtasks, uint32_t nrtasks);
 void sm64_merge(hipStream_t st, const ulonglong2* src, ulonglong2* dst, const MergeBlock* blocks,
                 uint32_t nblocks, uint64_t width);
 void sm64_fill_u64(hipStream_t st, unsigned long long* p, uint64_t n, unsigned long long v);
-// Count.  presorted: R chunks are already sorted in Rs; otherwise each task
-// sorts its chunk in LDS and, materialising, the chunk's first task writes
-// the sorted chunk to Rout (same offsets).
+// Count.  presorted: R chunks are already sorted in Rs; otherwise the chunks
+// of several tasks are already sorted in Rout (sm64_sort_shared), and each
+// task of a single-task chunk sorts it in LDS and, materialising, writes the
+// sorted chunk to Rout (same offsets).
 void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const ulonglong2* Rs,
                 ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
                 const uint32_t* ntasks_dev, unsigned long long* runs,

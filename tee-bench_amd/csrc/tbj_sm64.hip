@@ -726,6 +726,59 @@ __device__ __forceinline__ void load_chunk(const ulonglong2* __restrict__ src, u
     }
 }
 
+// load_chunk for n >= 1 with every load issued: items past n re-read the
+// last tuple.
+__device__ __forceinline__ void load_chunk_clamped(const ulonglong2* __restrict__ src, uint32_t n,
+                                                   ulonglong2 (&v)[kSortItems]) {
+#pragma unroll
+    for (uint32_t k = 0; k < kSortItems; k++)
+        v[k] = src[min(k * kSm64Block + threadIdx.x, n - 1)];
+}
+
+// S tuples j0 + q * kSm64Block + threadIdx.x (q < N) of a range of len >= 1
+// tuples, every load issued (ok[q] = inside the range).
+template <int N>
+__device__ __forceinline__ void load_s_group(const ulonglong2* __restrict__ s, uint32_t len,
+                                             uint32_t j0, uint64_t (&sk)[N], uint64_t (&sp)[N],
+                                             bool (&ok)[N]) {
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        const uint32_t j = j0 + q * kSm64Block + threadIdx.x;
+        ok[q] = j < len;
+        const ulonglong2 sv = s[min(j, len - 1)];
+        sk[q] = sv.x;
+        sp[q] = sv.y;
+    }
+}
+
+// Sorts the R chunks that several join tasks probe (multi) from src into dst
+// at the same offsets, so that the count kernel sorts each of them once
+// rather than once per task (a hot key of a skewed S spreads its partition
+// over thousands of tasks).  One workgroup per kSm64Block task descriptors:
+// the multi ones are listed in LDS, then sorted one after the other.
+__global__ __launch_bounds__(kSm64Block) void k_sort_shared64(const ulonglong2* __restrict__ src,
+                                                              ulonglong2* __restrict__ dst,
+                                                              const SmTask* __restrict__ rtasks,
+                                                              uint32_t nrtasks) {
+    __shared__ SortLds L;
+    __shared__ uint32_t list[kSm64Block];
+    __shared__ uint32_t nlist;
+    if (threadIdx.x == 0) nlist = 0;
+    __syncthreads();
+    const uint32_t i = blockIdx.x * kSm64Block + threadIdx.x;
+    if (i < nrtasks && rtasks[i].multi) list[atomicAdd(&nlist, 1u)] = i;
+    __syncthreads();
+    const uint32_t m = nlist;
+    for (uint32_t q = 0; q < m; q++) {
+        const SmTask tk = rtasks[list[q]];
+        ulonglong2 v[kSortItems];
+        load_chunk(src + tk.r_off, tk.r_len, v);
+        chunk_sort(v, tk.r_len, L);
+        for (uint32_t j = threadIdx.x; j < tk.r_len; j += kSm64Block) dst[tk.r_off + j] = L.a[j];
+        __syncthreads();
+    }
+}
+
 // Sorts every R chunk in place (partitions above one chunk, before merging).
 __global__ __launch_bounds__(kSm64Block) void k_sort_r64(ulonglong2* __restrict__ Rb,
                                                          const SmTask* __restrict__ rtasks,
@@ -865,41 +918,49 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
     __shared__ unsigned long long red_m[kSm64Block / TBJ_WAVE], red_c[kSm64Block / TBJ_WAVE];
     uint64_t matches = 0, cks = 0;
     const uint32_t ntasks = *ntasks_dev;
+    if (blockIdx.x >= ntasks) return;  // no task: nothing to count or flush
     for (uint32_t i = threadIdx.x; i < kSm64Cap; i += kSm64Block) head[i] = 0;
     const auto key_at = [&](uint32_t m) { return (uint64_t)L.a[m].x; };
+    constexpr int G = 4;  // S tuples looked up in lockstep per thread
+    // Software pipeline: a task's first S group is loaded before its R chunk
+    // is sorted, and the next task's R chunk is loaded before this task's
+    // probes (clamped, unconditional loads, so that nothing waits for them
+    // early).
+    // chunks that several tasks probe were sorted into Rout beforehand
+    // (k_sort_shared64) unless the whole input is presorted
+    const auto chunk_src = [&](const SmTask& x) {
+        return (!PRESORTED && x.multi ? (const ulonglong2*)Rout : Rs) + x.r_off;
+    };
+    SmTask tk = tasks[blockIdx.x];
+    ulonglong2 v[kSortItems];
+    load_chunk_clamped(chunk_src(tk), tk.r_len, v);
     for (uint32_t t = blockIdx.x; t < ntasks; t += gridDim.x) {
-        const SmTask tk = tasks[t];
         const uint32_t n = tk.r_len;
         const uint32_t hb = hash_bits(n);
-        if (PRESORTED) {
-            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.a[j] = Rs[tk.r_off + j];
+        uint64_t sk[G], sp[G];
+        bool ok[G];
+        load_s_group<G>(Sb + tk.s_off, tk.s_len, 0, sk, sp, ok);
+        if (PRESORTED || tk.multi) {
+            // slots past n (re-read copies of the last tuple) are never read
+#pragma unroll
+            for (uint32_t k = 0; k < kSortItems; k++) L.a[k * kSm64Block + threadIdx.x] = v[k];
             __syncthreads();
         } else {
-            ulonglong2 v[kSortItems];
-            load_chunk(Rs + tk.r_off, n, v);
             chunk_sort(v, n, L);
             if (MAT && tk.first)
                 for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) Rout[tk.r_off + j] = L.a[j];
         }
+        const uint32_t tn = t + gridDim.x;
+        const SmTask tkn = tasks[tn < ntasks ? tn : t];
+        load_chunk_clamped(chunk_src(tkn), tkn.r_len, v);
         index_runs(key_at, n, hb, head, nxt);
         if (MAT)
             for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.cnt[j] = 0;
         if (threadIdx.x == 0) tm = 0;
         __syncthreads();
         uint64_t mt = 0;
-        constexpr int G = 4;  // S tuples looked up in lockstep per thread
-        for (uint32_t j0 = 0; j0 < tk.s_len; j0 += G * kSm64Block) {
-            uint64_t sk[G], sp[G];
-            bool ok[G];
+        for (uint32_t j0 = 0;;) {
             uint32_t lo[G];
-#pragma unroll
-            for (int q = 0; q < G; q++) {
-                const uint32_t j = j0 + q * kSm64Block + threadIdx.x;
-                ok[q] = j < tk.s_len;
-                const ulonglong2 sv = ok[q] ? Sb[tk.s_off + j] : make_ulonglong2(0, 0);
-                sk[q] = sv.x;
-                sp[q] = sv.y;
-            }
             find_runs<G>(key_at, n, hb, head, nxt, sk, ok, lo);
 #pragma unroll
             for (int q = 0; q < G; q++) {
@@ -912,6 +973,9 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
                 }
                 if (MAT) agg_add(L.cnt, hit ? lo[q] : kNone);
             }
+            j0 += G * kSm64Block;
+            if (j0 >= tk.s_len) break;
+            load_s_group<G>(Sb + tk.s_off, tk.s_len, j0, sk, sp, ok);
         }
         matches += mt;
         if (MAT) {
@@ -932,6 +996,7 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         __syncthreads();
         // (the next task's sort or load ends with a barrier before its index)
         for (uint32_t i = threadIdx.x; i < (1u << hb); i += kSm64Block) head[i] = 0;
+        tk = tkn;
     }
     matches = wave_reduce_u64(matches);
     cks = wave_reduce_u64(cks);
@@ -1256,6 +1321,13 @@ void sm64_sort_r(hipStream_t st, uint32_t grid, ulonglong2* Rb, const SmTask* rt
     grid = resident_grid(k_sort_r64, grid);
     if (nrtasks && grid)
         hipLaunchKernelGGL(k_sort_r64, dim3(grid), dim3(kSm64Block), 0, st, Rb, rtasks, nrtasks);
+}
+
+void sm64_sort_shared(hipStream_t st, const ulonglong2* src, ulonglong2* dst,
+                      const SmTask* rtasks, uint32_t nrtasks) {
+    if (!nrtasks) return;
+    hipLaunchKernelGGL(k_sort_shared64, dim3((nrtasks + kSm64Block - 1) / kSm64Block),
+                       dim3(kSm64Block), 0, st, src, dst, rtasks, nrtasks);
 }
 
 void sm64_merge(hipStream_t st, const ulonglong2* src, ulonglong2* dst, const MergeBlock* blocks,

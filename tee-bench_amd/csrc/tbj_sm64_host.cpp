@@ -204,8 +204,8 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
     sm64_fill_tasks(st, bounds, nparts, rcnt, jcnt, h.nrtasks, h.njtasks, rtasks, jtasks);
 
     // ---- R partitions above one chunk: sort the chunks, then merge passes.
-    // Otherwise the count kernel sorts each chunk in LDS and its first task
-    // writes the sorted chunk to Tmp.
+    // Otherwise the chunks probed by several tasks are sorted into Tmp here,
+    // and the count kernel sorts each other chunk in LDS and writes it to Tmp.
     const bool presorted = h.max_r > (uint32_t)kSm64Cap;
     if (presorted) {
         const uint32_t sort_grid = std::max<uint32_t>(1, std::min<uint32_t>(h.nrtasks, C.num_cus * 3));
@@ -235,6 +235,8 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
         }
         // the host vector must outlive the copy
         TBJ_HIP(hipStreamSynchronize(st));
+    } else {
+        sm64_sort_shared(st, Pb, Tmp, rtasks, h.nrtasks);
     }
     const ulonglong2* Rsorted = presorted ? Pb : Tmp;
     TBJ_TRY(pt.mark(kPhTasks));

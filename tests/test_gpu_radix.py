@@ -169,3 +169,23 @@ def test_profile_phases(tbj, oracle):
     res = assert_join(tbj, oracle, "RHO", R, S, profile=True, radix_bits=14, passes=2)
     ph = res.as_dict()["phase_ms"]
     assert ph["join"] > 0 and ph["scatter1"] > 0 and ph["scatter2"] > 0
+
+
+@pytest.mark.parametrize("part_tiles", [1, 31, 32, 33, 65])
+def test_segments_across_tile_groups(tbj, oracle, part_tiles):
+    # Output offsets come from sums over groups of 32 consecutive tiles; a
+    # second-pass segment (one first-pass partition) of about part_tiles
+    # tiles of 4096 tuples then starts and ends at varying places inside
+    # groups, and groups hold pieces of several segments.  16 first-pass
+    # partitions (4 + 4 bits); a skewed key mix leaves some partitions
+    # almost empty and others several groups long.
+    rng = np.random.default_rng(100 + part_tiles)
+    n = 16 * part_tiles * 4096
+    nR = n // 4
+    rk = rng.integers(0, 1 << 20, nR, dtype=np.uint32)
+    # bin p (low 4 bits) gets a share proportional to p + 1 of S's keys
+    w = np.arange(1, 17, dtype=np.float64)
+    bins = rng.choice(16, n - nR, p=w / w.sum())
+    sk = ((rng.integers(0, 1 << 16, n - nR, dtype=np.uint32) << 4) | bins).astype(np.uint32)
+    sk[::3] = rk[rng.integers(0, nR, sk[::3].shape[0])]
+    assert_join(tbj, oracle, "RHO", rel(rk), rel(sk), radix_bits=8, passes=2)

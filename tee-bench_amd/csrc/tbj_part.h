@@ -8,16 +8,14 @@ namespace tbj {
 // One contiguous input range partitioned independently (a relation in the
 // first pass, a first-pass partition of a relation in the second pass).
 //
-// 8-byte tuples: the histogram keeps one row of bin counts per tile (row t
-// = tile t of the pass, so every tile writes one contiguous row), and the
-// output offsets come from a scan over group sums.  Group g of a pass is
-// tiles [g * kGroupTiles, (g + 1) * kGroupTiles); a segment owns the
-// ngroups groups its tiles touch, and its group sums sit bin-major from
-// hist_base: (bin b, group g) at hist_base + b * ngroups + g - first_tile /
+// The histogram keeps one row of bin counts per tile (row t = tile t of the
+// pass, so every tile writes one contiguous row), and the output offsets
+// come from a scan over group sums.  Group g of a pass is tiles
+// [g * kGroupTiles, (g + 1) * kGroupTiles); a segment owns the ngroups
+// groups its tiles touch, and its group sums sit bin-major from hist_base:
+// (bin b, group g) at hist_base + b * ngroups + g - first_tile /
 // kGroupTiles.  Scanned in that order, the entry of bin b's first group is
-// where partition (segment, b) starts.
-// 16-byte tuples: hist_base locates the segment's bin-major per-tile counts
-// (bin b, tile slot k) at hist_base + b * ntiles + k; ngroups is unused.
+// where partition (segment, b) starts.  (8-byte and 16-byte tuples alike.)
 struct SegDesc {
     uint64_t hist_base;
     uint32_t in_off;      // first input tuple

@@ -83,8 +83,8 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
 }
 
 // ---------------------------------------------------------------------------
-// Histogram: one workgroup per tile, LDS-privatised counts, written
-// bin-major per segment.  With fine != NULL, a pseudo-random 1/16 of the
+// Histogram: one workgroup per tile, LDS-privatised counts, written as the
+// tile's row of u16 counts.  With fine != NULL, a pseudo-random 1/16 of the
 // tuples (by their index in the relation) also count into
 // fine[seg * nbins * nbins2 + bin * nbins2 + bin2(key)]: the sample from
 // which the second pass sizes its sub-partitions.

@@ -39,20 +39,34 @@ __global__ void k_fill_u4(uint4* __restrict__ p, uint64_t n, uint32_t v) {
         p[i] = make_uint4(v, v, v, v);
 }
 
+constexpr int kBuildItems = 4;  // inserts per thread, their atomics in flight together
+
 __global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, uint64_t nR,
                                                    PhtBucket* __restrict__ tab, uint32_t mask,
                                                    OvNode* __restrict__ pool,
                                                    uint32_t* __restrict__ pool_ctr) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nR;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint2 t = R[i];
-        PhtBucket* b = tab + (t.x & mask);
-        const uint32_t pos = atomicAdd(&b->count, 1u);
-        if (pos < 2) {
-            b->slot[pos] = t;
+    const uint64_t base = (uint64_t)blockIdx.x * 256 * kBuildItems + threadIdx.x;
+    uint2 t[kBuildItems];
+    uint32_t pos[kBuildItems];
+#pragma unroll
+    for (int k = 0; k < kBuildItems; k++) {
+        const uint64_t i = base + (uint64_t)k * 256;
+        t[k] = R[i < nR ? i : nR - 1];
+    }
+#pragma unroll
+    for (int k = 0; k < kBuildItems; k++) {
+        const uint64_t i = base + (uint64_t)k * 256;
+        pos[k] = i < nR ? atomicAdd(&tab[t[k].x & mask].count, 1u) : ~0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBuildItems; k++) {
+        if (pos[k] == ~0u) continue;
+        PhtBucket* b = tab + (t[k].x & mask);
+        if (pos[k] < 2) {
+            b->slot[pos[k]] = t[k];
         } else {
             const uint32_t n = atomicAdd(pool_ctr, 1u);
-            pool[n].t = t;
+            pool[n].t = t[k];
             pool[n].next = atomicExch(&b->ovf, n + 1);
         }
     }
@@ -421,8 +435,8 @@ size_t pht_node_bytes() { return sizeof(OvNode); }
 
 void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, uint32_t mask,
                       void* pool, uint32_t* pool_ctr) {
-    hipLaunchKernelGGL(k_pht_build, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR,
-                       (PhtBucket*)tab, mask, (OvNode*)pool, pool_ctr);
+    hipLaunchKernelGGL(k_pht_build, dim3((uint32_t)((nR + 256 * kBuildItems - 1) / (256 * kBuildItems))),
+                       dim3(256), 0, st, R, nR, (PhtBucket*)tab, mask, (OvNode*)pool, pool_ctr);
 }
 
 void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,

@@ -104,6 +104,10 @@ static int init_locked(int device) {
     // default priority; used to capture HIP graphs when the caller's stream
     // is the legacy default stream, which cannot be captured
     if (!g_ctx.own_stream) TBJ_HIP(hipStreamCreateWithFlags(&g_ctx.own_stream, hipStreamNonBlocking));
+    if (!g_ctx.side_stream)
+        TBJ_HIP(hipStreamCreateWithFlags(&g_ctx.side_stream, hipStreamNonBlocking));
+    if (!g_ctx.fork_ev) TBJ_HIP(hipEventCreateWithFlags(&g_ctx.fork_ev, hipEventDisableTiming));
+    if (!g_ctx.join_ev) TBJ_HIP(hipEventCreateWithFlags(&g_ctx.join_ev, hipEventDisableTiming));
     g_ctx.ready = true;
     return TBJ_OK;
 }

@@ -72,7 +72,10 @@ void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDes
 void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
                             const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles);
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
-void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nseg,
+// Sorts the m sampled keys in place (one workgroup).
+void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m);
+// The model from the sorted sample and R's exact key range.
+void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nseg,
                 uint32_t bits, const uint32_t* minmax, CdfModel* model);
 void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);

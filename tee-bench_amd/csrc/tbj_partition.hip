@@ -559,19 +559,11 @@ __global__ void k_sample_keys(const uint2* __restrict__ in, uint64_t n, uint32_t
     sample[i] = in[pos].x;
 }
 
-// Bitonic sort of m (power of two, <= kFitMax) keys in LDS, then knots.
-// Segment count nseg <= kLdsKnots.  Partition space: 2^bits partitions with
-// frac fixed-point bits (bits + frac == 32).
-__global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restrict__ sample,
-                                                       uint32_t m, uint32_t nseg_req,
-                                                       uint32_t bits,
-                                                       const uint32_t* __restrict__ minmax,
-                                                       CdfModel* __restrict__ model) {
+// Bitonic sort of m (power of two, <= kFitMax) sampled keys in LDS, in
+// place.  It needs only the sample, so it runs beside the exact key range.
+__global__ __launch_bounds__(kFitBlock) void k_fit_sort(uint32_t* __restrict__ sample,
+                                                        uint32_t m) {
     __shared__ uint32_t a[kFitMax];
-    __shared__ uint32_t knots[kLdsKnots + 1];
-    __shared__ uint64_t kx[kLdsKnots + 1];
-    __shared__ uint32_t ky[kLdsKnots + 1];
-    __shared__ uint32_t nk_s;
     for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) a[i] = sample[i];
     __syncthreads();
     for (uint32_t k = 2; k <= m; k <<= 1) {
@@ -590,13 +582,32 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
             __syncthreads();
         }
     }
+    for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) sample[i] = a[i];
+}
+
+// Knots and cell table from the sorted sample.  Segment count nseg <=
+// kLdsKnots.  Partition space: 2^bits partitions with frac fixed-point bits
+// (bits + frac == 32).
+__global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restrict__ sorted,
+                                                       uint32_t m, uint32_t nseg_req,
+                                                       uint32_t bits,
+                                                       const uint32_t* __restrict__ minmax,
+                                                       CdfModel* __restrict__ model) {
+    __shared__ uint32_t quant[kLdsKnots];
+    __shared__ uint32_t knots[kLdsKnots + 1];
+    __shared__ uint64_t kx[kLdsKnots + 1];
+    __shared__ uint32_t ky[kLdsKnots + 1];
+    __shared__ uint32_t nk_s;
+    for (uint32_t j = threadIdx.x; j < nseg_req; j += kFitBlock)
+        quant[j] = sorted[(uint64_t)j * m / nseg_req];
+    __syncthreads();
     const uint32_t kmin = minmax[0], kmax = minmax[1];
     if (threadIdx.x == 0) {
         // first knot at R's exact minimum, then distinct sample quantiles
         uint32_t nk = 0;
         knots[nk++] = kmin;
         for (uint32_t j = 1; j < nseg_req; j++) {
-            const uint32_t q = a[(uint64_t)j * m / nseg_req];
+            const uint32_t q = quant[j];
             if (q > knots[nk - 1] && q <= kmax) knots[nk++] = q;
         }
         nk_s = nk;
@@ -878,9 +889,13 @@ void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint
     hipLaunchKernelGGL(k_sample_keys, dim3((m + 255) / 256), dim3(256), 0, st, in, n, m, sample);
 }
 
-void launch_fit(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t nseg, uint32_t bits,
+void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m) {
+    hipLaunchKernelGGL(k_fit_sort, dim3(1), dim3(kFitBlock), 0, st, sample, m);
+}
+
+void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nseg, uint32_t bits,
                 const uint32_t* minmax, CdfModel* model) {
-    hipLaunchKernelGGL(k_fit_cdf, dim3(1), dim3(kFitBlock), 0, st, sample, m, nseg, bits, minmax,
+    hipLaunchKernelGGL(k_fit_cdf, dim3(1), dim3(kFitBlock), 0, st, sorted, m, nseg, bits, minmax,
                        model);
 }
 

@@ -183,8 +183,15 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
             // 64 samples per segment: knot noise of about 1/8 of a segment
             const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kLdsKnots, m / 64));
             launch_fill_u32(st, sm->minmax, 1, 0xFFFFFFFFu);
+            // the sample and its sort (one workgroup) run on the side stream
+            // while the exact key range reads all of R
+            TBJ_HIP(hipEventRecord(C.fork_ev, st));
+            TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
+            launch_sample(C.side_stream, Rb, nR, m, sample);
+            launch_fit_sort(C.side_stream, sample, m);
+            TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
             launch_minmax(st, Rb, nR, sm->minmax);
-            launch_sample(st, Rb, nR, m, sample);
+            TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
             launch_fit(st, sample, m, nseg, plan.B, sm->minmax, model);
             s1r = learned_spec(plan, 1, model, 0);
             s1s = learned_spec(plan, 1, model, 1);

@@ -33,6 +33,10 @@ struct Context {
     int device = -1;
     int num_cus = 0;
     hipStream_t own_stream = nullptr;
+    // a second stream for work beside the caller's (forked and joined
+    // through the two events)
+    hipStream_t side_stream = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     Workspace ws;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t pev[24] = {};

@@ -42,15 +42,21 @@ constexpr int kSm64MergeTile = kSm64Block * kSm64MergeItems;
 // outside the sample's range spread over the end partitions instead of
 // piling into the first and the last.  The clamps make the map monotone: partitions are key ranges
 // in key order.
+//
+// The partition kernels evaluate it through a cell table: cellv[c] is the
+// model at key kmin + (c << gshift) (c <= ncells), and a key inside
+// [kmin, kmax] interpolates linearly (integer arithmetic) between its cell's
+// two ends; keys outside take segment 0's or the last segment's line, the
+// latter never below ymax, the table's value at kmax.  The map stays
+// monotone.
 struct Model64 {
-    uint32_t nseg, gshift, ncells, pad;
-    uint64_t kmin, kmax;  // sample key range (cells of the guide table)
+    uint32_t nseg, gshift, ncells, ymax;
+    uint64_t kmin, kmax;  // sample key range (the cells span it)
     uint64_t x[kSm64Knots + 1];
     double base[kSm64Knots];
     double slope[kSm64Knots];
     uint32_t plo[kSm64Knots], phi[kSm64Knots];
-    // guide[c]: segment holding key kmin + (c << gshift); guide[ncells] = nseg - 1
-    uint16_t guide[kSm64Guide + 1];
+    uint32_t cellv[kSm64Guide + 1];
 };
 
 struct Bin64 {

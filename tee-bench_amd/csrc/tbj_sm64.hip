@@ -103,39 +103,64 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan_u64(
 // Partition function
 
 struct LdsModel64 {
-    uint64_t x[kSm64Knots + 1];
-    double base[kSm64Knots];
-    double slope[kSm64Knots];
-    uint32_t plo[kSm64Knots], phi[kSm64Knots];
-    uint16_t guide[kSm64Guide + 1];
+    uint32_t cellv[kSm64Guide + 1];
 };
+
+// The model's exact piecewise-linear value at a key (segment a's line,
+// clamped to the segment's partitions); below x[0] segment 0's line runs
+// down to partition 0.
+__device__ __forceinline__ uint32_t model64_exact(const uint64_t* x, const double* base,
+                                                  const double* slope, const uint32_t* plo,
+                                                  const uint32_t* phi, uint32_t nseg,
+                                                  uint64_t key) {
+    if (key < x[0]) {
+        const double v = base[0] - (double)(x[0] - key) * slope[0];
+        return min(v <= 0.0 ? 0u : (uint32_t)v, phi[0]);
+    }
+    uint32_t a = 0, b = nseg;  // last segment whose first knot is <= key
+    while (b - a > 1) {
+        const uint32_t mid = (a + b) >> 1;
+        if (x[mid] <= key) a = mid;
+        else b = mid;
+    }
+    const double v = fma((double)(key - x[a]), slope[a], base[a]);
+    uint32_t p = v <= 0.0 ? 0u : (v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v);
+    p = max(p, plo[a]);
+    return min(p, phi[a]);
+}
+
+// Linear interpolation inside cell c: cellv[c] + off * (cellv[c+1] - cellv[c]) / 2^gshift.
+__device__ __forceinline__ uint32_t cell_interp(uint32_t y0, uint32_t y1, uint64_t off,
+                                                uint32_t gshift) {
+    return gshift ? y0 + (uint32_t)__umul64hi(off << (64 - gshift), (uint64_t)(y1 - y0)) : y0;
+}
 
 struct Bin64Fn {
     uint32_t kind, pass_shift, pass_mask, radix_mask;
     const LdsModel64* lm;
-    uint32_t nseg, gshift, ncells;
-    uint64_t kmin, lo, hi;
+    uint32_t gshift, ymax;
+    uint64_t kmin, kmax, lo, hi;
+    // the two end segments, for keys outside [kmin, kmax]
+    uint64_t x0, xl;
+    double base0, slope0, basel, slopel;
+    uint32_t phi0, plol, phil;
     bool drop;
 
     __device__ __forceinline__ uint32_t full(uint64_t key) const {
         if (kind == kBinRadix) return (uint32_t)key & radix_mask;
-        if (key < lm->x[0]) {  // below the sample: segment 0's line, down to 0
-            const double v = lm->base[0] - (double)(lm->x[0] - key) * lm->slope[0];
-            return min(v <= 0.0 ? 0u : (uint32_t)v, lm->phi[0]);
+        if (key < kmin) {  // below the sample: segment 0's line, down to 0
+            const double v = base0 - (double)(x0 - key) * slope0;
+            return min(v <= 0.0 ? 0u : (uint32_t)v, phi0);
         }
-        uint64_t c = (key - kmin) >> gshift;
-        if (c >= ncells) c = ncells - 1;
-        uint32_t a = lm->guide[c], b = (uint32_t)lm->guide[c + 1] + 1;
-        if (b > nseg) b = nseg;
-        while (b - a > 1) {
-            const uint32_t mid = (a + b) >> 1;
-            if (lm->x[mid] <= key) a = mid;
-            else b = mid;
+        if (key > kmax) {  // above: the last segment's line, from ymax up
+            const double v = fma((double)(key - xl), slopel, basel);
+            uint32_t p = v <= 0.0 ? 0u : (v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v);
+            p = min(max(p, plol), phil);
+            return max(p, ymax);
         }
-        const double v = fma((double)(key - lm->x[a]), lm->slope[a], lm->base[a]);
-        uint32_t p = v <= 0.0 ? 0u : (v >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)v);
-        p = max(p, lm->plo[a]);
-        return min(p, lm->phi[a]);
+        const uint64_t d = key - kmin;
+        const uint32_t c = (uint32_t)(d >> gshift);
+        return cell_interp(lm->cellv[c], lm->cellv[c + 1], d & ((1ull << gshift) - 1), gshift);
     }
     __device__ __forceinline__ uint32_t operator()(uint64_t key) const {
         if (drop && (key < lo || key > hi)) return kNone;
@@ -151,27 +176,30 @@ __device__ __forceinline__ Bin64Fn make_bin64(const Bin64& b, LdsModel64* lm) {
     f.pass_mask = b.pass_mask;
     f.radix_mask = b.bits >= 32 ? 0xFFFFFFFFu : (1u << b.bits) - 1;
     f.lm = lm;
-    f.nseg = f.gshift = f.ncells = 0;
-    f.kmin = 0;
+    f.gshift = f.ymax = 0;
+    f.kmin = f.kmax = f.x0 = f.xl = 0;
+    f.base0 = f.slope0 = f.basel = f.slopel = 0.0;
+    f.phi0 = f.plol = f.phil = 0;
     f.drop = b.range != nullptr;
     f.lo = f.drop ? b.range[0] : 0;
     f.hi = f.drop ? b.range[1] : ~0ull;
     if (b.kind == kBinLearned) {
         const Model64* m = b.model;
-        for (uint32_t i = threadIdx.x; i <= m->nseg; i += blockDim.x) {
-            lm->x[i] = m->x[i];
-            if (i < m->nseg) {
-                lm->base[i] = m->base[i];
-                lm->slope[i] = m->slope[i];
-                lm->plo[i] = m->plo[i];
-                lm->phi[i] = m->phi[i];
-            }
-        }
-        for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->guide[i] = m->guide[i];
-        f.nseg = m->nseg;
+        for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->cellv[i] = m->cellv[i];
+        const uint32_t l = m->nseg - 1;
         f.gshift = m->gshift;
-        f.ncells = m->ncells;
+        f.ymax = m->ymax;
         f.kmin = m->kmin;
+        f.kmax = m->kmax;
+        f.x0 = m->x[0];
+        f.base0 = m->base[0];
+        f.slope0 = m->slope[0];
+        f.phi0 = m->phi[0];
+        f.xl = m->x[l];
+        f.basel = m->base[l];
+        f.slopel = m->slope[l];
+        f.plol = m->plo[l];
+        f.phil = m->phi[l];
     }
     return f;
 }
@@ -196,6 +224,9 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
     __shared__ uint64_t a[kSm64FitMax];
     __shared__ uint64_t knots[kSm64Knots];
     __shared__ uint32_t krank[kSm64Knots];
+    __shared__ double sbase[kSm64Knots], sslope[kSm64Knots];
+    __shared__ uint32_t splo[kSm64Knots], sphi[kSm64Knots];
+    __shared__ uint32_t scell[kSm64Guide + 1];
     __shared__ uint32_t nk_s;
     for (uint32_t i = threadIdx.x; i < m; i += 1024) a[i] = sample[i];
     __syncthreads();
@@ -244,34 +275,43 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
         const uint64_t x0 = knots[i];
         const uint64_t x1 = i + 1 < nseg ? knots[i + 1] : kmax;
         const double dx = x1 > x0 ? (double)(x1 - x0) : 1.0;
-        model->x[i] = x0;
-        model->base[i] = y0;
-        model->slope[i] = (y1 - y0) / dx;
+        const double sl = (y1 - y0) / dx;
         const uint32_t lo = i == 0 ? 0u : min((uint32_t)floor(y0), last);
         uint32_t hi = (uint32_t)ceil(y1);
         hi = hi > 0 ? hi - 1 : 0;
+        const uint32_t ph = i + 1 < nseg ? min(max(hi, lo), last) : last;
+        model->x[i] = x0;
+        model->base[i] = y0;
+        model->slope[i] = sl;
         model->plo[i] = lo;
-        model->phi[i] = i + 1 < nseg ? min(max(hi, lo), last) : last;
+        model->phi[i] = ph;
+        sbase[i] = y0;
+        sslope[i] = sl;
+        splo[i] = lo;
+        sphi[i] = ph;
     }
     if (threadIdx.x == 0) model->x[nseg] = kmax;
+    __syncthreads();
+    // cell table: the model at kmin + (c << gshift), c <= ncells (the last
+    // cell start lies above kmax; saturated at 2^64 - 1)
     uint32_t gshift = 0;
     while (((kmax - kmin) >> gshift) >= (uint64_t)kSm64Guide) gshift++;
     const uint32_t ncells = (uint32_t)(((kmax - kmin) >> gshift) + 1);
     for (uint32_t c = threadIdx.x; c <= ncells; c += 1024) {
-        uint32_t g = nseg - 1;
-        if (c < ncells) {
-            const uint64_t key = kmin + ((uint64_t)c << gshift);
-            uint32_t lo = 0, hi = nseg;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (knots[mid] <= key) lo = mid;
-                else hi = mid;
-            }
-            g = lo;
-        }
-        model->guide[c] = (uint16_t)g;
+        // c << gshift itself can pass 2^64 (the last cell start of a span
+        // near 2^64)
+        const bool wide = gshift > 0 && ((uint64_t)c >> (64 - gshift)) != 0;
+        const uint64_t off = (uint64_t)c << gshift;
+        const uint64_t key = wide || off > ~0ull - kmin ? ~0ull : kmin + off;
+        const uint32_t v = model64_exact(knots, sbase, sslope, splo, sphi, nseg, key);
+        scell[c] = v;
+        model->cellv[c] = v;
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
+        const uint64_t d = kmax - kmin;
+        const uint32_t c = (uint32_t)(d >> gshift);
+        model->ymax = cell_interp(scell[c], scell[c + 1], d & ((1ull << gshift) - 1), gshift);
         model->nseg = nseg;
         model->kmin = kmin;
         model->kmax = kmax;

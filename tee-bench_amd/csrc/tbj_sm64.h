@@ -27,7 +27,7 @@ constexpr int kSm64SItems = 8;
 constexpr int kSm64SChunk = kSm64Block * kSm64SItems;  // 4096
 // Learned model
 constexpr int kSm64Knots = 256;
-constexpr int kSm64Guide = 2048;
+constexpr int kSm64Cells = 2048;
 constexpr int kSm64FitMax = 16384;
 // Merge passes for R partitions larger than one chunk
 constexpr int kSm64MergeItems = 8;
@@ -56,7 +56,7 @@ struct Model64 {
     double base[kSm64Knots];
     double slope[kSm64Knots];
     uint32_t plo[kSm64Knots], phi[kSm64Knots];
-    uint32_t cellv[kSm64Guide + 1];
+    uint32_t cellv[kSm64Cells + 1];
 };
 
 struct Bin64 {

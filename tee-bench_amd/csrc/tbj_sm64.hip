@@ -103,7 +103,7 @@ __device__ __forceinline__ unsigned long long block_exclusive_scan_u64(
 // Partition function
 
 struct LdsModel64 {
-    uint32_t cellv[kSm64Guide + 1];
+    uint32_t cellv[kSm64Cells + 1];
 };
 
 // The model's exact piecewise-linear value at a key (segment a's line,
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
     __shared__ uint32_t krank[kSm64Knots];
     __shared__ double sbase[kSm64Knots], sslope[kSm64Knots];
     __shared__ uint32_t splo[kSm64Knots], sphi[kSm64Knots];
-    __shared__ uint32_t scell[kSm64Guide + 1];
+    __shared__ uint32_t scell[kSm64Cells + 1];
     __shared__ uint32_t nk_s;
     for (uint32_t i = threadIdx.x; i < m; i += 1024) a[i] = sample[i];
     __syncthreads();
@@ -295,7 +295,7 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
     // cell table: the model at kmin + (c << gshift), c <= ncells (the last
     // cell start lies above kmax; saturated at 2^64 - 1)
     uint32_t gshift = 0;
-    while (((kmax - kmin) >> gshift) >= (uint64_t)kSm64Guide) gshift++;
+    while (((kmax - kmin) >> gshift) >= (uint64_t)kSm64Cells) gshift++;
     const uint32_t ncells = (uint32_t)(((kmax - kmin) >> gshift) + 1);
     for (uint32_t c = threadIdx.x; c <= ncells; c += 1024) {
         // c << gshift itself can pass 2^64 (the last cell start of a span

@@ -213,3 +213,22 @@ def test_large_unique_r_zipf_s_properties(tbj):
     assert bool((ku[1:] >= ku[:-1]).all())
     sp = torch.sort(pairs[:, 1]).values
     assert bool((sp == torch.arange(n, device="cuda:0")).all())
+
+
+@pytest.mark.parametrize("case", ["top", "narrow", "bottom"])
+def test_learned_key_spans(tbj, oracle, case):
+    # The learned model's cell table spans the sample's key range: keys at
+    # the top of the 64-bit range (cell starts past 2^64 saturate), a span
+    # narrower than the cell count (cells one key wide) and keys at zero.
+    rng = np.random.default_rng({"top": 31, "narrow": 32, "bottom": 33}[case])
+    n = 120000
+    if case == "top":
+        rk = np.uint64(2**64 - 1) - rng.choice(1 << 40, n, replace=False).astype(np.uint64)
+    elif case == "narrow":
+        rk = (np.uint64(5000) + rng.permutation(1500).astype(np.uint64))
+    else:
+        rk = rng.choice(1 << 30, n, replace=False).astype(np.uint64)
+    sk = rk[rng.integers(0, len(rk), 3 * len(rk))]
+    R, S = rel64(rk), rel64(sk)
+    res, got = run64(tbj, "PSM", R, S, partitioner="learned")
+    check_pairs(oracle, R, S, got, res, ordered=True)

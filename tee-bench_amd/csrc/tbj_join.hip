@@ -55,9 +55,11 @@ __device__ __forceinline__ uint32_t tasks_of(uint2 r, uint2 s, uint32_t cap, uin
 // Per-partition task counts (task_cnt[nparts] is a zero sentinel so that the
 // exclusive scan leaves the total there).  stats[0] / stats[1] = largest R /
 // S partition.
-__global__ void k_task_counts(const uint2* __restrict__ bounds, uint32_t nparts, uint32_t cap,
-                              uint32_t chunk, uint32_t* __restrict__ task_cnt,
-                              uint32_t* __restrict__ stats) {
+__global__ __launch_bounds__(256) void k_task_counts(const uint2* __restrict__ bounds,
+                                                     uint32_t nparts, uint32_t cap,
+                                                     uint32_t chunk,
+                                                     uint32_t* __restrict__ task_cnt,
+                                                     uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t rn = 0, sn = 0;
     if (p < nparts) {
@@ -68,13 +70,27 @@ __global__ void k_task_counts(const uint2* __restrict__ bounds, uint32_t nparts,
     } else if (p == nparts) {
         task_cnt[p] = 0;
     }
+    // one atomic pair per block: per-wave atomics on the two words
+    // serialise (tens of microseconds at 2^16 partitions)
+    __shared__ uint32_t red[2][256 / TBJ_WAVE];
     for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
         rn = max(rn, (uint32_t)__shfl_xor(rn, d, TBJ_WAVE));
         sn = max(sn, (uint32_t)__shfl_xor(sn, d, TBJ_WAVE));
     }
-    if (lane_id() == 0 && (rn | sn)) {
-        atomicMax(&stats[0], rn);
-        atomicMax(&stats[1], sn);
+    if (lane_id() == 0) {
+        red[0][wave_id()] = rn;
+        red[1][wave_id()] = sn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / TBJ_WAVE; w++) {
+            rn = max(rn, red[0][w]);
+            sn = max(sn, red[1][w]);
+        }
+        if (rn | sn) {
+            atomicMax(&stats[0], rn);
+            atomicMax(&stats[1], sn);
+        }
     }
 }
 

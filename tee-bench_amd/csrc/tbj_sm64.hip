@@ -570,9 +570,11 @@ __global__ void k_bounds64(const uint32_t* __restrict__ scanned, const SegDesc* 
 
 // Per partition: R chunks and join tasks (R chunk x S chunk); word nparts is
 // zero (scan sentinel); max_r receives the largest R partition.
-__global__ void k_task_counts64(const uint2* __restrict__ bounds, uint32_t nparts,
-                                uint32_t* __restrict__ rcnt, uint32_t* __restrict__ jcnt,
-                                uint32_t* __restrict__ max_r) {
+__global__ __launch_bounds__(256) void k_task_counts64(const uint2* __restrict__ bounds,
+                                                       uint32_t nparts,
+                                                       uint32_t* __restrict__ rcnt,
+                                                       uint32_t* __restrict__ jcnt,
+                                                       uint32_t* __restrict__ max_r) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t nr = 0;
     if (p < nparts) {
@@ -586,8 +588,15 @@ __global__ void k_task_counts64(const uint2* __restrict__ bounds, uint32_t npart
         rcnt[p] = 0;
         jcnt[p] = 0;
     }
+    // one atomic per block (per-wave atomics on one word serialise)
+    __shared__ uint32_t red[256 / TBJ_WAVE];
     for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) nr = max(nr, (uint32_t)__shfl_xor(nr, d, TBJ_WAVE));
-    if (lane_id() == 0 && nr) atomicMax(max_r, nr);
+    if (lane_id() == 0) red[wave_id()] = nr;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (uint32_t w = 1; w < blockDim.x / TBJ_WAVE; w++) nr = max(nr, red[w]);
+        if (nr) atomicMax(max_r, nr);
+    }
 }
 
 // Task t of a list lies in the partition p with off[p] <= t < off[p + 1].

@@ -566,17 +566,17 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_sort(uint32_t* __restrict__ s
     __shared__ uint32_t a[kFitMax];
     for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) a[i] = sample[i];
     __syncthreads();
+    // one compare-exchange per pair index q < m / 2 (every lane busy)
     for (uint32_t k = 2; k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint32_t x = a[i], y = a[ixj];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
+            for (uint32_t q = threadIdx.x; q < m / 2; q += kFitBlock) {
+                const uint32_t i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const uint32_t ixj = i | j;
+                const bool up = (i & k) == 0;
+                const uint32_t x = a[i], y = a[ixj];
+                if ((x > y) == up) {
+                    a[i] = y;
+                    a[ixj] = x;
                 }
             }
             __syncthreads();

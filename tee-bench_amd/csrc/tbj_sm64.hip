@@ -230,17 +230,19 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
     __shared__ uint32_t nk_s;
     for (uint32_t i = threadIdx.x; i < m; i += 1024) a[i] = sample[i];
     __syncthreads();
+    // bitonic sort, one compare-exchange per pair index q < m / 2 (every
+    // lane busy): pair q joins i = its low bits below j and the rest shifted
+    // up one place, with i ^ j
     for (uint32_t k = 2; k <= m; k <<= 1) {
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t i = threadIdx.x; i < m; i += 1024) {
-                const uint32_t ixj = i ^ j;
-                if (ixj > i) {
-                    const bool up = (i & k) == 0;
-                    const uint64_t x = a[i], y = a[ixj];
-                    if ((x > y) == up) {
-                        a[i] = y;
-                        a[ixj] = x;
-                    }
+            for (uint32_t q = threadIdx.x; q < m / 2; q += 1024) {
+                const uint32_t i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                const uint32_t ixj = i | j;
+                const bool up = (i & k) == 0;
+                const uint64_t x = a[i], y = a[ixj];
+                if ((x > y) == up) {
+                    a[i] = y;
+                    a[ixj] = x;
                 }
             }
             __syncthreads();

@@ -654,16 +654,23 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
             }
             lds_sort_u64<RITEMS>(v, n, srt, cnt, scratch, mm, &big);
         }
+        // S groups are loaded one ahead (clamped, unconditional loads), so a
+        // group's loads fly during the previous group's searches and claim
+        const uint32_t slast = tk.s_len ? tk.s_len - 1 : 0;
+        uint2 sv[G];
+#pragma unroll
+        for (int k = 0; k < G; k++) sv[k] = Sb[tk.s_off + min(k * kJoinBlock + threadIdx.x, slast)];
         for (uint32_t base = 0; base < tk.s_len; base += kJoinBlock * G) {
-            uint2 sv[G];
+            uint2 sn[G];
+#pragma unroll
+            for (int k = 0; k < G; k++)
+                sn[k] = Sb[tk.s_off + min(base + kJoinBlock * G + k * kJoinBlock + threadIdx.x, slast)];
             uint32_t lo[G], hi[G];
             uint32_t c = 0;
 #pragma unroll
             for (int k = 0; k < G; k++) {
                 const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                const bool ok = j < tk.s_len;
-                sv[k] = ok ? Sb[tk.s_off + j] : make_uint2(0, 0);
-                lo[k] = ok ? 0 : n;  // first sorted entry with key >= S key
+                lo[k] = j < tk.s_len ? 0 : n;  // first sorted entry with key >= S key
                 hi[k] = n;
             }
             for (uint32_t w = n; w; w >>= 1) {
@@ -702,6 +709,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
                 }
                 __syncthreads();
             }
+#pragma unroll
+            for (int k = 0; k < G; k++) sv[k] = sn[k];
         }
         __syncthreads();
     }

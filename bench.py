@@ -226,9 +226,14 @@ def bench_join32(args, tb):
         b_alg = 16 * nR + 8 * nS + 8 * matches + pair_b
     else:
         kernels = None
+    if algo in ("RSM", "PSM"):
+        jsym = "k_join_sort"
+    else:
+        jsym = "k_join_mat" if mat else "k_join"
     kernels = kernels or {
-        # k_join / k_join_mat: reads every partitioned tuple once, writes the pairs
-        "join": (phases["join"], 8 * (nR + s_in) + pair_b, "k_join_mat" if mat else "k_join"),
+        # k_join / k_join_mat / k_join_sort: reads every partitioned tuple
+        # once, writes the pairs
+        "join": (phases["join"], 8 * (nR + s_in) + pair_b, jsym),
         # second-pass scatter: reads and writes every surviving tuple once
         "scatter2": (phases["scatter2"], 16 * (nR + s_in), "k_scatter"),
         # second-pass sizing: the per-tile histogram read

@@ -355,6 +355,9 @@ def bench_join64(args, tb):
         if part == "learned" and not np.all(keys[1:] >= keys[:-1]):
             raise SystemExit("pairs are not in key order")
     other = "radix" if part == "learned" else "learned"
+    # range partitions are key ranges in key order; radix partitions are not
+    order = {"learned": "key order over the whole output",
+             "radix": "key order inside each radix partition"}
     res2, el2 = timed(make_step(other), max(3, args.steps // 2), 1)
     if args.calibrate_copy:
         calibrate_copy()
@@ -383,7 +386,9 @@ def bench_join64(args, tb):
                    "phase_ms": {k: round(v, 4) for k, v in phases.items()},
                    "comparison": {"partitioner": other,
                                   "value": 2 * n * len(res2) / el2,
-                                  "ms_per_step": round(el2 / len(res2) * 1e3, 4)},
+                                  "ms_per_step": round(el2 / len(res2) * 1e3, 4),
+                                  "output_order": order[other]},
+                   "output_order": order[part],
                    "gen_seconds": round(gen_s, 1)},
         "roofline": roofline(kbytes, kms, ksym, "c5"),
         "pipeline_roofline": {"achieved": round(b_alg / (ms_per_step * 1e-3) / 1e9, 1),

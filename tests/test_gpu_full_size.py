@@ -131,3 +131,37 @@ def test_c5_psm_and_rsm_same_pairs_full_size(tbj):
         maps.append(m)
         del pairs
     assert torch.equal(maps[0], maps[1])
+
+
+def test_c4_full_batch_closed_form(tbj, oracle):
+    # The benchmark batch: 4096 generated joins (|R_i| log-uniform in
+    # [2^10, 2^22], |S_i| = k |R_i|, Zipf z over R_i's keys 1..|R_i|, R_i
+    # unique), so count_i = |S_i|; eager run, then the captured graph twice
+    import torch
+    torch.cuda.empty_cache()  # the arena alone is 170 GB
+    specs, total = tbj.gen_batch_specs(0xC4C4000000000000, 4096)
+    arena = torch.empty(total, dtype=torch.int64, device="cuda:0")
+    print(f"[case] gen_batch_device 4096 joins, {total} tuples", file=sys.stderr, flush=True)
+    tbj.gen_batch_device(arena, specs)
+    items = tbj.batch_items(list(specs))
+    counts = torch.zeros(len(items), dtype=torch.int64, device="cuda:0")
+    want = [s.nS for s in specs]
+    b = tbj.Batch(arena, items, counts, None)
+    print("[case] batch 4096 joins eager", file=sys.stderr, flush=True)
+    res = b.run()
+    torch.cuda.synchronize()
+    assert counts.cpu().tolist() == want and res.matches == sum(want)
+    b.capture()
+    for rep in range(2):
+        counts.fill_(0)
+        print(f"[case] batch 4096 joins graph replay {rep}", file=sys.stderr, flush=True)
+        res = b.run()
+        torch.cuda.synchronize()
+        assert counts.cpu().tolist() == want and res.matches == sum(want)
+    b.close()
+    # a few joins against the oracle on their generated contents
+    for i in (0, 1, 777, 2048, 4095):
+        s = specs[i]
+        R = arena[s.r_off:s.r_off + s.nR].cpu().numpy().view(np.uint32).reshape(-1, 2)
+        S = arena[s.s_off:s.s_off + s.nS].cpu().numpy().view(np.uint32).reshape(-1, 2)
+        assert oracle.count(R[:, 0], S[:, 0]) == s.nS

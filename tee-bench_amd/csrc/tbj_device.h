@@ -11,6 +11,43 @@ namespace tbj {
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (TBJ_WAVE - 1); }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / TBJ_WAVE; }
 
+// LDS counters: increments ctr[idx] for each active lane with idx !=
+// TBJ_DROP and returns the value before the lane's own increment (its rank).
+// Lanes sharing the idx of the wave's first such lane are served by one
+// atomic (same-address atomics of a wave serialise, and skewed keys put most
+// of a wave on one counter); the others add one each.  Every active lane of
+// the wave must call it.
+__device__ __forceinline__ uint32_t agg_inc(uint32_t* ctr, uint32_t idx) {
+    const uint64_t any = __ballot(idx != TBJ_DROP);
+    if (any == 0) return 0;
+    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
+    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
+    const uint64_t same = __ballot(idx == lead);
+    const uint32_t lane = lane_id();
+    uint32_t r = 0;
+    if (idx == lead) {
+        if (lane == fl) r = atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
+    } else if (idx != TBJ_DROP) {
+        r = atomicAdd(&ctr[idx], 1u);
+    }
+    const uint32_t b = __shfl(r, fl, TBJ_WAVE);
+    return idx == lead ? b + (uint32_t)__popcll(same & ((1ull << lane) - 1)) : r;
+}
+
+// agg_inc without the rank.
+__device__ __forceinline__ void agg_add(uint32_t* ctr, uint32_t idx) {
+    const uint64_t any = __ballot(idx != TBJ_DROP);
+    if (any == 0) return;
+    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
+    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
+    const uint64_t same = __ballot(idx == lead);
+    if (idx == lead) {
+        if (lane_id() == fl) atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
+    } else if (idx != TBJ_DROP) {
+        atomicAdd(&ctr[idx], 1u);
+    }
+}
+
 // Inclusive prefix sum across the 64 lanes of a wave, in DPP moves (no LDS
 // crossbar round trips): shifts by 1, 2, 4, 8 inside each row of 16 lanes
 // (lanes shifted in from outside the row read 0), then row 0's total is

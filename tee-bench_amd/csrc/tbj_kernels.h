@@ -54,10 +54,14 @@ void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* cou
                          uint32_t* offs, const TileDesc* tiles, const SegDesc* segs,
                          const uint32_t* ntiles_dev, uint32_t nbins, uint32_t* gsum,
                          uint64_t glen, uint32_t* partial, uint32_t* grand_total);
-// offs: the per-tile offsets from launch_tile_offsets.
+// offs: the per-tile offsets from launch_tile_offsets.  last_pass: the
+// plan's last pass, whose segments are narrow key ranges where one hot key can
+// fill most of a wave: lanes sharing a bin then take their ranks from one LDS
+// atomic (in a first pass of two, whose tiles mix the whole key range, the
+// plain per-tuple atomics are cheaper).
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins);
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,

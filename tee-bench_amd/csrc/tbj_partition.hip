@@ -280,7 +280,7 @@ __host__ __device__ constexpr size_t scatter_model_bytes(bool learned) {
 }
 
 // offs: row t = the output offset of each bin's run of tile t.
-template <bool GMODEL>
+template <bool GMODEL, bool AGG>
 __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
@@ -317,10 +317,11 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
             const uint32_t j = k * kPartBlock + threadIdx.x;
-            if (j < td.len) {
-                b[k] = bin(v[k].x);
-                if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
-            }
+            if (j < td.len) b[k] = bin(v[k].x);
+            if (AGG)
+                rank[k] = agg_inc(loff, b[k]);
+            else if (b[k] != TBJ_DROP)
+                rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
         __syncthreads();
         const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
@@ -416,10 +417,8 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter_est(
 #pragma unroll
     for (uint32_t k = 0; k < kPartItems; k++) {
         const uint32_t j = k * kPartBlock + threadIdx.x;
-        if (j < td.len) {
-            b[k] = bin(v[k].x);
-            if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
-        }
+        if (j < td.len) b[k] = bin(v[k].x);
+        rank[k] = agg_inc(loff, b[k]);
     }
     __syncthreads();
     // claim space: one atomic per non-empty bin, before loff becomes offsets
@@ -830,17 +829,16 @@ void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDes
 
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins) {
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
     const size_t lds = 2 * sizeof(uint32_t) * nbins;
-    if (bs.kind == kBinLearned)
-        hipLaunchKernelGGL(k_scatter<true>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
-                           tiles, ntiles_dev, tile_base, tile_base + ntiles, offs, bs, nbins);
-    else
-        hipLaunchKernelGGL(k_scatter<false>, dim3(ntiles), dim3(kPartBlock), lds, st, in, out,
-                           tiles, ntiles_dev, tile_base, tile_base + ntiles, offs, bs, nbins);
+    const bool learned = bs.kind == kBinLearned;
+    auto k = learned ? (last_pass ? k_scatter<true, true> : k_scatter<true, false>)
+                     : (last_pass ? k_scatter<false, true> : k_scatter<false, false>);
+    hipLaunchKernelGGL(k, dim3(ntiles), dim3(kPartBlock), lds, st, in, out, tiles, ntiles_dev,
+                       tile_base, tile_base + ntiles, offs, bs, nbins);
 }
 
 void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,

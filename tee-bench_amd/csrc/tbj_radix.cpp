@@ -239,8 +239,10 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
             launch_fill_u32(st, sm->stats, 2, 0);
         }
         TBJ_TRY(pt.mark(kPhScan1));
-        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, plan.F1);
-        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, plan.F1);
+        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, plan.F1,
+                       plan.passes == 1);
+        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, plan.F1,
+                       plan.passes == 1);
         TBJ_TRY(pt.mark(kPhScatter1));
 
         if (plan.passes == 1) {
@@ -269,7 +271,8 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                 launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, plan.F2,
                                     gs2, L2max, partial, &sm->total2);
                 TBJ_TRY(pt.mark(kPhSize2));
-                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, plan.F2);
+                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, plan.F2,
+                               true);
                 launch_part_bounds(st, gs2, segs2, plan.F1, plan.F2, 2, nR, nS, bounds);
             } else {
                 // regions sized from the 1/16 sample (ratio estimate inside each

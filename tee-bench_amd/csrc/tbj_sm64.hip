@@ -39,41 +39,6 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 
-// Increments ctr[idx] for each active lane with idx != kNone and returns
-// the value before the lane's own increment (its rank).  Lanes sharing the
-// idx of the wave's first such lane are served by one atomic; every active
-// lane of the wave must call it.
-__device__ __forceinline__ uint32_t agg_inc(uint32_t* ctr, uint32_t idx) {
-    const uint64_t any = __ballot(idx != kNone);
-    if (any == 0) return 0;
-    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
-    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
-    const uint64_t same = __ballot(idx == lead);
-    const uint32_t lane = lane_id();
-    uint32_t r = 0;
-    if (idx == lead) {
-        if (lane == fl) r = atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
-    } else if (idx != kNone) {
-        r = atomicAdd(&ctr[idx], 1u);
-    }
-    const uint32_t b = __shfl(r, fl, TBJ_WAVE);
-    return idx == lead ? b + (uint32_t)__popcll(same & ((1ull << lane) - 1)) : r;
-}
-
-// agg_inc without the rank.
-__device__ __forceinline__ void agg_add(uint32_t* ctr, uint32_t idx) {
-    const uint64_t any = __ballot(idx != kNone);
-    if (any == 0) return;
-    const uint32_t fl = (uint32_t)__ffsll((unsigned long long)any) - 1;
-    const uint32_t lead = __shfl(idx, fl, TBJ_WAVE);
-    const uint64_t same = __ballot(idx == lead);
-    if (idx == lead) {
-        if (lane_id() == fl) atomicAdd(&ctr[idx], (uint32_t)__popcll(same));
-    } else if (idx != kNone) {
-        atomicAdd(&ctr[idx], 1u);
-    }
-}
-
 __device__ __forceinline__ bool tuple_less(const ulonglong2& a, const ulonglong2& b) {
     return a.x < b.x || (a.x == b.x && a.y < b.y);
 }

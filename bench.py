@@ -122,7 +122,9 @@ def cpu_baseline(R, S, reps):
 
 def measured_traffic(kernel, workload_name):
     """HBM bytes per launch from the committed counter summary, if any."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload_name}.txt")
+    # a phase of several kernels ("a+b+c") is filed as a_b_c
+    path = os.path.join(ROOT, "profiles",
+                        f"traffic_{kernel.replace('+', '_')}_{workload_name}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):

@@ -38,6 +38,8 @@ namespace tbj {
 namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+// agg_inc / agg_add (tbj_device.h) skip lanes holding TBJ_DROP
+static_assert(kNone == TBJ_DROP, "one no-bin marker");
 
 __device__ __forceinline__ bool tuple_less(const ulonglong2& a, const ulonglong2& b) {
     return a.x < b.x || (a.x == b.x && a.y < b.y);

@@ -211,7 +211,7 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                             partial, &sm->total1);
 
         if (learned) {
-            // Judge the model on the first-pass R partitions; fall back to radix
+            // Check the model on the first-pass R partitions; fall back to radix
             // when one would overflow the per-partition bound.
             launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, bounds);
             launch_count_tasks(st, bounds, plan.F1, kJoinCap, 1u << 30, task_off, tpartial,

@@ -1,8 +1,9 @@
 """GPU parity of the sort-merge joins (RSM, PSM) against the oracle.
 
-The reference's own merges are unbounded and miscount (PSM +1, RSM under
-skew); the oracle's restatement uses a bounded merge, which is
-the true equi-join these tests hold the GPU to."""
+The reference's own merges are unbounded and miscount (PSM counts one extra
+match, Joins/psm/parallel_sortmerge_join.h:26-46; RSM miscounts under skew);
+the oracle's restatement uses a bounded merge, which is the true equi-join
+these tests hold the GPU to."""
 import numpy as np
 import pytest
 

@@ -522,16 +522,30 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // (key, payload) -- a bucket sort over the range's key span, each bucket
 // insertion-sorted by one thread, with a bitonic network when a bucket is
 // too large (clustered keys); every S tuple then finds its equal-key run with
-// a lower-bound search (several searches in lockstep per thread) and walks it.
+// a lower-bound search inside its bucket of the sort (several searches in
+// lockstep per thread) and walks it.
 // The merge is bounded on both sides (the reference's unbounded inner loops
 // overrun the arrays and miscount, parallel_sortmerge_join.h:26-46).  A task
 // sorts its R range once and probes its S range in sub-chunks.
 
 constexpr uint32_t kSortBucketMax = 32;
 
+// Bucket of key among n buckets spread evenly over [lo, lo + span):
+// floor((key - lo) * n / span), monotone in key.  The floor of the double
+// quotient is exact: (key - lo) * n < 2^44 and span <= 2^32 are exact
+// doubles, and a quotient below n <= 4096 that is not an integer lies at
+// least 1 / span >= 2^-32 below the next integer, while its rounding error
+// is at most 2^12 * 2^-53.
+__device__ __forceinline__ uint32_t sort_bucket(uint32_t key, uint32_t lo, uint32_t n,
+                                                uint64_t span) {
+    return (uint32_t)((double)((uint64_t)(key - lo) * n) / (double)span);
+}
+
 // Sorts the n values v (item k of the thread at k * kJoinBlock + threadIdx.x)
 // into srt[0, n) ascending; cnt (n words), scratch and mm/big are LDS work
-// space.  Every thread must call it; ends with a barrier.
+// space.  Every thread must call it; ends with a barrier.  Afterwards mm holds
+// the least and greatest key and cnt[b] the first sorted position of bucket b
+// (sort_bucket over span = mm[1] - mm[0] + 1): a directory of the sorted keys.
 template <int ITEMS>
 __device__ void lds_sort_u64(const unsigned long long (&v)[ITEMS], uint32_t n,
                              unsigned long long* srt, uint32_t* cnt, uint32_t* scratch,
@@ -568,7 +582,7 @@ __device__ void lds_sort_u64(const unsigned long long (&v)[ITEMS], uint32_t n,
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
         if (k * kJoinBlock + tid < n) {
-            bk[k] = (uint32_t)(((uint64_t)((uint32_t)(v[k] >> 32) - lo) * n) / span);
+            bk[k] = sort_bucket((uint32_t)(v[k] >> 32), lo, n, span);
             rk[k] = atomicAdd(&cnt[bk[k]], 1u);
         }
     }
@@ -654,6 +668,8 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
             }
             lds_sort_u64<RITEMS>(v, n, srt, cnt, scratch, mm, &big);
         }
+        const uint32_t kmin = mm[0], kmax = mm[1];
+        const uint64_t span = (uint64_t)kmax - kmin + 1;
         // S groups are loaded one ahead (clamped, unconditional loads), so a
         // group's loads fly during the previous group's searches and claim
         const uint32_t slast = tk.s_len ? tk.s_len - 1 : 0;
@@ -667,21 +683,32 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
                 sn[k] = Sb[tk.s_off + min(base + kJoinBlock * G + k * kJoinBlock + threadIdx.x, slast)];
             uint32_t lo[G], hi[G];
             uint32_t c = 0;
+            // first sorted entry with key >= S key, searched inside the S
+            // key's bucket [cnt[b], cnt[b + 1]) (keys outside [kmin, kmax]
+            // and S tuples past the range: n, no match)
 #pragma unroll
             for (int k = 0; k < G; k++) {
                 const uint32_t j = base + k * kJoinBlock + threadIdx.x;
-                lo[k] = j < tk.s_len ? 0 : n;  // first sorted entry with key >= S key
-                hi[k] = n;
+                const uint32_t x = sv[k].x;
+                lo[k] = hi[k] = n;
+                if (j < tk.s_len && x >= kmin && x <= kmax) {
+                    const uint32_t bk = sort_bucket(x, kmin, n, span);
+                    lo[k] = cnt[bk];
+                    hi[k] = bk + 1 < n ? cnt[bk + 1] : n;
+                }
             }
-            for (uint32_t w = n; w; w >>= 1) {
+            for (;;) {
+                bool more = false;
 #pragma unroll
                 for (int k = 0; k < G; k++) {
                     if (lo[k] < hi[k]) {
                         const uint32_t m = (lo[k] + hi[k]) >> 1;
                         if (srt[m] < ((unsigned long long)sv[k].x << 32)) lo[k] = m + 1;
                         else hi[k] = m;
+                        more |= lo[k] < hi[k];
                     }
                 }
+                if (!more) break;
             }
 #pragma unroll
             for (int k = 0; k < G; k++) {

@@ -45,7 +45,19 @@ enum tbj_algo {
     TBJ_PSM = 6,        /* sort both relations + merge (psm/parallel_sortmerge_join.h:61) */
     TBJ_RHO_ATOMIC = 7, /* RHO with lock-free task indices (rho_atomic/radix_join_atomic.cpp:1278) */
     TBJ_NPO_ST = 8,     /* single-table variant of PHT (no_partitioning_join.h:474) */
-    TBJ_ALGO_COUNT = 9
+    TBJ_MWAY = 9,       /* partition, sort, bounded merge join (mway/sortmergejoin_multiway.c:327,
+                           merge semantics of mway/joincommon.c:241-316) */
+    TBJ_ALGO_COUNT = 10
+};
+
+/* What a materialising join writes per match (tbj_options.output). */
+enum tbj_output {
+    TBJ_OUT_PAIRS = 0,    /* tbj_pair {R.payload, S.payload} */
+    TBJ_OUT_RELATION = 1  /* tbj_tuple {key, R.payload}: the tuple the reference's
+                             to_relation copies out of each output_list_t node
+                             (Enclave/TrustedJoins/secure_joins.cpp:277-301), so the
+                             output is a relation that feeds the next join on the
+                             device (radix-family algorithms only) */
 };
 
 enum tbj_partitioner {
@@ -79,7 +91,8 @@ typedef struct tbj_options {
                                    per-tile histograms (falls back to exact on
                                    overflow) */
     int32_t profile;            /* 1: record per-phase device times in phase_ms */
-    int32_t reserved[3];
+    int32_t output;             /* enum tbj_output */
+    int32_t reserved[2];
 } tbj_options;
 
 typedef struct tbj_result {
@@ -104,10 +117,18 @@ typedef struct tbj_result {
     float phase_ms[8];
 } tbj_result;
 
-/* Initialise on a HIP device (0-based).  Safe to call more than once. */
+/* Initialise on a HIP device (0-based).  Safe to call more than once.
+ *
+ * Concurrency: every caller stream gets its own session (device workspace,
+ * events, a side stream), so calls on different streams, from different host
+ * threads, run at the same time; calls on one stream serialise.  Each
+ * session keeps its workspace between calls. */
 int tbj_init(int device);
-/* Release cached device workspace. */
+/* Release every session's cached device workspace. */
 void tbj_shutdown(void);
+/* Release the session (and workspace) of one stream; waits for a call still
+ * running on it. */
+int tbj_release_stream(void* stream);
 void tbj_default_options(tbj_options* opt);
 const char* tbj_status_str(int status);
 /* Number of compiled join algorithms and their names (the -a strings). */
@@ -127,6 +148,20 @@ int tbj_join(int algo,
              const tbj_tuple* S, uint64_t nS,
              tbj_pair* pairs, uint64_t pairs_cap,
              const tbj_options* opt, tbj_result* res, void* stream);
+
+/*
+ * The three-way join (R join S) join T of the reference's
+ * ecall_three_way_join_sealed_tables (Enclave/TrustedJoins/secure_joins.cpp:
+ * 348-399) without its sealing: out1 = to_relation(join(R, S)), the
+ * (key, R.payload) tuple of every match, then join(T, out1) written as
+ * (key, T.payload) tuples into out (device, out_cap tuples).  out1 stays in
+ * HBM between the two joins (no host round trip).  res describes the second
+ * join (ms_total covers both).  algo: a radix-family selector (the
+ * reference uses RHO for both joins).
+ */
+int tbj_join3(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+              const tbj_tuple* T, uint64_t nT, tbj_tuple* out, uint64_t out_cap,
+              const tbj_options* opt, tbj_result* res, void* stream);
 
 /* Same with host pointers: copies R and S to HBM, joins, copies pairs back. */
 int tbj_join_host(int algo,

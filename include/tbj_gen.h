@@ -40,6 +40,18 @@ int tbj_gen_fk_sel(uint32_t seed, uint64_t n, int64_t maxid, tbj_tuple* out, int
 int tbj_gen_zipf(uint32_t seed, uint64_t n, uint32_t alphabet, double z, tbj_tuple* out,
                  int payload_mode, int nthreads);
 
+/* create_relation_from_file (generator.cpp:290-351), as native.cpp:78-84 runs
+ * it for -t/-u: one integer key per line (atoi), as many tuples as the file
+ * has newlines, then knuth_shuffle with the rand() stream of seed.  Returns
+ * the tuple count; out (cap tuples) is filled only when the count fits, so a
+ * call with out = NULL sizes the relation.  TBJ_ERR_INVALID when the file
+ * cannot be opened. */
+int64_t tbj_gen_from_file(uint32_t seed, const char* path, tbj_tuple* out, uint64_t cap,
+                          int payload_mode);
+/* The same shuffle of n given keys (the keys of a loaded file). */
+int tbj_gen_from_keys(uint32_t seed, const uint32_t* keys, uint64_t n, tbj_tuple* out,
+                      int payload_mode);
+
 /* Remaps every odd-indexed tuple's key to key + offset (keys outside R's
  * domain when offset >= max R key): the out-of-domain half of the skewed
  * 50%-selectivity workload.  No reference counterpart (the reference CLI

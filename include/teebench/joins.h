@@ -4,7 +4,8 @@
  * Same names, signature and C++ linkage as the reference's join entry
  * points (Joins/radix_join.h:17-24, Joins/no_partitioning_join.h:474,543,
  * Joins/radix_sortmerge_join.h, Joins/psm/parallel_sortmerge_join.h:61,
- * native.cpp:26-32 for CHT, rho_atomic/radix_join_atomic.cpp:1278), so an
+ * native.cpp:26-32 for CHT, rho_atomic/radix_join_atomic.cpp:1278,
+ * mway/sortmergejoin_multiway.h for MWAY), so an
  * algorithm_t table like native.cpp:34-46 links against this library
  * instead of the reference's Joins objects.  Each call copies the relations
  * to HBM, runs the join through the C ABI (tbj.h) and returns a malloc'd
@@ -14,8 +15,8 @@
  * Materialisation: the reference compiles it in with -DJOIN_MATERIALIZE
  * (radix_join.c:231-233); here teebench_set_materialize(1) turns it on at
  * run time.  resultlist then holds one entry whose list has exactly
- * totalresults output_list_t nodes (key, Rpayload, Spayload); free it with
- * teebench_free_result().
+ * totalresults output_list_t nodes (key, Rpayload, Spayload), allocated as
+ * one block in list order; free it with teebench_free_result().
  *
  * Errors: a failing device call logs and exits with EXIT_FAILURE, which is
  * what the reference does on allocation or thread failures (util.c:11-18).
@@ -33,6 +34,7 @@ result_t* NPO_st(relation_t* relR, relation_t* relS, int nthreads);
 result_t* CHT(relation_t* relR, relation_t* relS, int nthreads);
 result_t* RSM(relation_t* relR, relation_t* relS, int nthreads);
 result_t* PSM(relation_t* relR, relation_t* relS, int nthreads);
+result_t* MWAY(relation_t* relR, relation_t* relS, int nthreads);
 
 /* Runtime switches of this implementation (no reference counterpart; C
  * linkage so that FFI callers find them by name). */

@@ -490,29 +490,6 @@ void orc_sort_pairs32(uint32_t* a, uint32_t* b, uint64_t n) {
     free(w);
 }
 
-typedef struct { uint64_t a, b; } orc_p64;
-static int cmp_p64(const void* x, const void* y) {
-    const orc_p64* p = (const orc_p64*)x;
-    const orc_p64* q = (const orc_p64*)y;
-    if (p->a != q->a) return p->a < q->a ? -1 : 1;
-    if (p->b != q->b) return p->b < q->b ? -1 : 1;
-    return 0;
-}
-
-void orc_sort_pairs64(uint64_t* a, uint64_t* b, uint64_t n) {
-    orc_p64* w = (orc_p64*)malloc(n * sizeof(orc_p64) + 16);
-    for (uint64_t i = 0; i < n; i++) {
-        w[i].a = a[i];
-        w[i].b = b[i];
-    }
-    qsort(w, n, sizeof(orc_p64), cmp_p64);
-    for (uint64_t i = 0; i < n; i++) {
-        a[i] = w[i].a;
-        b[i] = w[i].b;
-    }
-    free(w);
-}
-
 /* ------------------------------------------------------------------------ */
 /* Sort-merge restatement (PSM parallel_sortmerge_join.h:18-111 and RSM
  * radix_sortmerge_join.c:40-105): sort both relations by key, then merge.
@@ -539,6 +516,93 @@ int64_t orc_sortmerge(const uint32_t* rk, uint64_t nr, const uint32_t* sk, uint6
             j = j2;
         }
     }
+    free(r);
+    free(s);
+    return m;
+}
+
+/* MWAY's merge join (mway/joincommon.c:241-316) after MWAY has sorted both
+ * relations (sortmergejoin_multiway.c): while both cursors are in range,
+ * advance the smaller key; on equal keys, for every R tuple of the key walk
+ * the S run of that key (bounded by |S|), then continue after the run. */
+int64_t orc_mway(const uint32_t* rk, uint64_t nr, const uint32_t* sk, uint64_t ns) {
+    uint64_t* r = (uint64_t*)malloc(nr * sizeof(uint64_t) + 8);
+    uint64_t* s = (uint64_t*)malloc(ns * sizeof(uint64_t) + 8);
+    for (uint64_t i = 0; i < nr; i++) r[i] = rk[i];
+    for (uint64_t i = 0; i < ns; i++) s[i] = sk[i];
+    radix_sort_u64(r, nr);
+    radix_sort_u64(s, ns);
+    int64_t m = 0;
+    uint64_t i = 0, j = 0;
+    while (i < nr && j < ns) {
+        if (r[i] < s[j]) {
+            i++;
+        } else if (r[i] > s[j]) {
+            j++;
+        } else {
+            uint64_t jj;
+            do {
+                jj = j;
+                do {
+                    m++;
+                    jj++;
+                } while (jj < ns && r[i] == s[jj]);
+                i++;
+            } while (i < nr && r[i] == s[j]);
+            j = jj;
+        }
+    }
+    free(r);
+    free(s);
+    return m;
+}
+
+/* The join written as a relation: for every match one (key, R.payload)
+ * tuple, the fields to_relation copies out of each output_list_t node
+ * (Enclave/TrustedJoins/secure_joins.cpp:277-301, nodes as insert_output
+ * fills them, radix_join.c:231-233).  Tuples come out sorted by (key,
+ * payload).  Return = matches; at most cap tuples are written. */
+static int cmp_u64(const void* x, const void* y) {
+    const uint64_t a = *(const uint64_t*)x, b = *(const uint64_t*)y;
+    return a < b ? -1 : a > b;
+}
+
+int64_t orc_join_relation(const uint32_t* rk, const uint32_t* rp, uint64_t nr,
+                          const uint32_t* sk, uint64_t ns, uint32_t* out_key,
+                          uint32_t* out_pay, uint64_t cap) {
+    uint64_t* r = (uint64_t*)malloc(nr * sizeof(uint64_t) + 8);
+    uint64_t* s = (uint64_t*)malloc(ns * sizeof(uint64_t) + 8);
+    for (uint64_t i = 0; i < nr; i++) r[i] = ((uint64_t)rk[i] << 32) | (rp ? rp[i] : 0);
+    for (uint64_t i = 0; i < ns; i++) s[i] = sk[i];
+    radix_sort_u64(r, nr);
+    radix_sort_u64(s, ns);
+    int64_t m = 0;
+    uint64_t i = 0, j = 0;
+    uint64_t* w = (uint64_t*)malloc(cap * sizeof(uint64_t) + 8);
+    while (i < nr && j < ns) {
+        const uint64_t k = r[i] >> 32;
+        if (k < s[j]) i++;
+        else if (k > s[j]) j++;
+        else {
+            uint64_t i2 = i, j2 = j;
+            while (i2 < nr && (r[i2] >> 32) == k) i2++;
+            while (j2 < ns && s[j2] == k) j2++;
+            for (uint64_t a = i; a < i2; a++)
+                for (uint64_t b = j; b < j2; b++) {
+                    if ((uint64_t)m < cap) w[m] = r[a];
+                    m++;
+                }
+            i = i2;
+            j = j2;
+        }
+    }
+    const uint64_t n = (uint64_t)m < cap ? (uint64_t)m : cap;
+    qsort(w, n, sizeof(uint64_t), cmp_u64);
+    for (uint64_t q = 0; q < n; q++) {
+        out_key[q] = (uint32_t)(w[q] >> 32);
+        out_pay[q] = (uint32_t)w[q];
+    }
+    free(w);
     free(r);
     free(s);
     return m;

@@ -2,7 +2,8 @@
  * ORACLE — test infrastructure only.
  *
  * A plain-C restatement of the reference's equi-join path (generators, RHO,
- * RHT, PHT, CHT, PSM/RSM) used as the checker for the HIP product.  Only
+ * RHT, PHT, CHT, PSM/RSM, MWAY, the join written as a relation) used as the
+ * checker for the HIP product.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * liboracle.so, and only as the checker.  The product never calls it.
  *
@@ -45,6 +46,14 @@ int64_t orc_cht(const uint32_t* rk, const uint32_t* rp, uint64_t nr,
                 const uint32_t* sk, const uint32_t* sp, uint64_t ns,
                 uint64_t* checksum, int64_t* unresolved);
 int64_t orc_sortmerge(const uint32_t* rk, uint64_t nr, const uint32_t* sk, uint64_t ns);
+/* MWAY's merge join over the sorted relations (mway/joincommon.c:241-316). */
+int64_t orc_mway(const uint32_t* rk, uint64_t nr, const uint32_t* sk, uint64_t ns);
+/* The join as a relation of (key, R.payload) tuples, to_relation's format
+ * (secure_joins.cpp:277-301), sorted; the building block of the three-way
+ * join (R join S) join T (secure_joins.cpp:348-399). */
+int64_t orc_join_relation(const uint32_t* rk, const uint32_t* rp, uint64_t nr,
+                          const uint32_t* sk, uint64_t ns, uint32_t* out_key,
+                          uint32_t* out_pay, uint64_t cap);
 
 /* Textbook equi-join (sort both sides, multiply equal-key run lengths). */
 int64_t orc_textbook_count(const uint32_t* rk, uint64_t nr, const uint32_t* sk, uint64_t ns);
@@ -58,7 +67,6 @@ int64_t orc_textbook_pairs64(const uint64_t* rk, const uint64_t* rp, uint64_t nr
 
 /* Sorts (a[i], b[i]) pairs lexicographically, in place. */
 void orc_sort_pairs32(uint32_t* a, uint32_t* b, uint64_t n);
-void orc_sort_pairs64(uint64_t* a, uint64_t* b, uint64_t n);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@
 //
 // Exports (all extern "C"):
 //   ref_gen_pk / ref_gen_fk / ref_gen_fk_sel / ref_gen_zipf   -> keys only
+//   ref_gen_from_file(seed, path, ...)                          -> keys only
 //   ref_join_count(name, ...)                                  -> matches
 //   ref_join_pairs(...)   (JOIN_MATERIALIZE build only)         -> pairs
 //   ref_rand_stream(seed, n, out)                              -> glibc rand()
@@ -25,6 +26,7 @@
 #include "CHTJoinWrapper.hpp"
 #include "radix_sortmerge_join.h"
 #include "parallel_sortmerge_join.h"
+#include "sortmergejoin_multiway.h"
 
 // The reference's Logger reads this start stamp (Logger.cpp:6); native.cpp:23
 // normally defines it.
@@ -82,6 +84,28 @@ result_t* CHT_wrap(relation_t* R, relation_t* S, int n, uint64_t* checksum) {
     return res;
 }
 
+// MWAY's thread start (mway/joincommon.c:51,137) passes a pthread_attr_t it
+// never initialises.  Zeroing the stack the call is about to use makes that
+// attribute read as all-zero (glibc: default stack and guard size) instead of
+// garbage, on which pthread_create fails with EAGAIN.
+__attribute__((noinline)) void scrub_stack() {
+    volatile uint8_t pad[1 << 20];
+    for (size_t i = 0; i < sizeof(pad); i++) pad[i] = 0;
+}
+
+__attribute__((noinline)) result_t* MWAY_clean_stack(relation_t* R, relation_t* S, int n) {
+    scrub_stack();
+    return MWAY(R, S, n);
+}
+
+// create_relation_from_file hands getline an uninitialised buffer pointer
+// with size 0 (generator.cpp:298-300,329); on a zeroed stack that pointer is
+// NULL and getline allocates, as the reference intends.
+__attribute__((noinline)) int from_file_clean_stack(table_t* t, char* path) {
+    scrub_stack();
+    return create_relation_from_file(t, path, 0);
+}
+
 }  // namespace
 
 extern "C" {
@@ -132,6 +156,22 @@ int ref_gen_zipf(uint32_t seed, uint64_t n, int64_t alphabet, double z, uint32_t
     return 0;
 }
 
+// create_relation_from_file (generator.cpp:290-351) after seed_generator(seed),
+// as native.cpp:78-84 does for -t/-u.  Returns the tuple count (keys in out,
+// at most cap), or -1 when the file cannot be read.
+int64_t ref_gen_from_file(uint32_t seed, const char* path, uint32_t* out, uint64_t cap,
+                          int quiet) {
+    StdoutSilencer s(quiet);
+    table_t t;
+    seed_generator(seed);
+    if (from_file_clean_stack(&t, (char*)path)) return -1;
+    const uint64_t n = t.num_tuples < cap ? t.num_tuples : cap;
+    for (uint64_t i = 0; i < n; i++) out[i] = t.tuples[i].key;
+    const int64_t total = (int64_t)t.num_tuples;
+    free(t.tuples);
+    return total;
+}
+
 // Runs one of the reference's join entry points on copies of the given
 // relations.  Returns the match count, or -1 for an unknown name.
 // checksum_out receives CHT's payload checksum (left untouched otherwise).
@@ -156,6 +196,7 @@ int64_t ref_join_count(const char* name,
     else if (!strcmp(name, "PHT")) res = PHT(&R, &S, nthreads);
     else if (!strcmp(name, "NPO_st")) res = NPO_st(&R, &S, nthreads);
     else if (!strcmp(name, "CHT")) res = CHT_wrap(&R, &S, nthreads, checksum_out);
+    else if (!strcmp(name, "MWAY")) res = MWAY_clean_stack(&R, &S, nthreads);
     clock_gettime(CLOCK_MONOTONIC, &t1);
     if (elapsed_us_out)
         *elapsed_us_out = (t1.tv_sec - t0.tv_sec) * 1e6 + (t1.tv_nsec - t0.tv_nsec) * 1e-3;

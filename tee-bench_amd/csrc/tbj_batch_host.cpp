@@ -305,7 +305,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     TBJ_HIP(hipMemcpy(P->small.p, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     TBJ_HIP(hipEventCreate(&P->e0));
     TBJ_HIP(hipEventCreate(&P->e1));
-    P->grid = (uint32_t)tbj::ctx().num_cus * 4;
+    P->grid = (uint32_t)tbj::dev().num_cus * 4;
     return TBJ_OK;
 }
 
@@ -319,7 +319,7 @@ int tbj_batch_create(int algo, const tbj_tuple* arena, const tbj_batch_item* ite
     *out = nullptr;
     if (algo != TBJ_RHO) return TBJ_ERR_UNSUPPORTED;
     if (n == 0 || !items || !arena || !counts) return TBJ_ERR_INVALID;
-    int s = tbj_init(tbj::ctx().ready ? tbj::ctx().device : 0);
+    int s = tbj_init(tbj::dev().ready ? tbj::dev().device : 0);
     if (s != TBJ_OK) return s;
     tbj_batch_plan* P = new tbj_batch_plan();
     P->algo = algo;
@@ -339,7 +339,7 @@ int tbj_batch_create(int algo, const tbj_tuple* arena, const tbj_batch_item* ite
 int tbj_batch_capture(tbj_batch_plan* P, void* stream) {
     if (!P) return TBJ_ERR_INVALID;
     // the legacy default stream cannot be captured: record on the library's own
-    hipStream_t st = stream ? (hipStream_t)stream : tbj::ctx().own_stream;
+    hipStream_t st = stream ? (hipStream_t)stream : tbj::dev().own_stream;
     if (P->exec) {
         TBJ_HIP(hipGraphExecDestroy(P->exec));
         P->exec = nullptr;
@@ -470,7 +470,7 @@ int tbj_gen_batch_device(tbj_tuple* arena, const tbj_batch_spec* specs, uint32_t
                          void* stream) {
     if ((!arena && n) || (!specs && n)) return TBJ_ERR_INVALID;
     if (n == 0) return TBJ_OK;
-    int s = tbj_init(tbj::ctx().ready ? tbj::ctx().device : 0);
+    int s = tbj_init(tbj::dev().ready ? tbj::dev().device : 0);
     if (s != TBJ_OK) return s;
     std::vector<tbj::GenJob> jobs(n);
     uint64_t start = 0;

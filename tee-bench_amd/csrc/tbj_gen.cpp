@@ -5,6 +5,8 @@
 // boundary, then independent blocks are shuffled (or Zipf ranks looked up)
 // on separate threads from their snapshot.
 #include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -244,6 +246,45 @@ int tbj_gen_zipf(uint32_t seed, uint64_t n, uint32_t alphabet, double z, tbj_tup
         }
     });
     return TBJ_OK;
+}
+
+int tbj_gen_from_keys(uint32_t seed, const uint32_t* keys, uint64_t n, tbj_tuple* out,
+                      int payload_mode) {
+    if (n && (!keys || !out)) return TBJ_ERR_INVALID;
+    for (uint64_t i = 0; i < n; i++) out[i].key = keys[i];
+    set_payloads(out, n, payload_mode, 1);
+    Rand3 g(seed);
+    shuffle_keys(g, out, n);
+    return TBJ_OK;
+}
+
+int64_t tbj_gen_from_file(uint32_t seed, const char* path, tbj_tuple* out, uint64_t cap,
+                          int payload_mode) {
+    if (!path) return TBJ_ERR_INVALID;
+    FILE* f = fopen(path, "r");
+    if (!f) return TBJ_ERR_INVALID;
+    // the tuple count is the number of newlines (generator.cpp:308-314)
+    uint64_t lines = 0;
+    for (int c = getc(f); c != EOF; c = getc(f))
+        if (c == '\n') lines++;
+    if (!out || lines > cap) {
+        fclose(f);
+        return (int64_t)lines;
+    }
+    rewind(f);
+    std::vector<uint32_t> keys;
+    keys.reserve(lines);
+    char buf[256];
+    while (keys.size() < lines && fgets(buf, sizeof(buf), f)) {
+        keys.push_back((uint32_t)atoi(buf));
+        // a line longer than the buffer: skip its rest
+        while (!strchr(buf, '\n') && fgets(buf, sizeof(buf), f)) {
+        }
+    }
+    fclose(f);
+    keys.resize(lines, 0);
+    tbj_gen_from_keys(seed, keys.data(), lines, out, payload_mode);
+    return (int64_t)lines;
 }
 
 void tbj_remap_odd(tbj_tuple* rel, uint64_t n, uint32_t offset) {

@@ -94,30 +94,46 @@ __global__ __launch_bounds__(256) void k_task_counts(const uint2* __restrict__ b
     }
 }
 
-// One thread per task: task t lies in the partition p with task_off[p] <= t
-// < task_off[p + 1] (binary search), as R chunk li / nsc and S chunk li %
-// nsc of it (li = t - task_off[p], nsc = S chunks of p).  A partition's
-// tasks are contiguous in partition order.
-__global__ void k_fill_tasks(const uint2* __restrict__ bounds, uint32_t nparts, uint32_t cap,
-                             uint32_t chunk, const uint32_t* __restrict__ task_off,
-                             uint32_t ntasks, JoinTask* __restrict__ tasks) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntasks) return;
-    uint32_t lo = 0, hi = nparts;
+// Task t lies in the partition p with task_off[p] <= t < task_off[p + 1],
+// as R chunk li / nsc and S chunk li % nsc of it (li = t - task_off[p], nsc =
+// S chunks of p).  A partition's tasks are contiguous in partition order.
+__device__ __forceinline__ JoinTask task_in_part(const TaskSrc& ts, uint32_t p, uint32_t t) {
+    const uint2 r = ts.bounds[p], s = ts.bounds[ts.nparts + p];
+    const uint32_t nsc = (s.y - s.x + ts.chunk - 1) / ts.chunk;
+    const uint32_t li = t - ts.task_off[p];
+    JoinTask tk;
+    tk.r_off = r.x + (li / nsc) * ts.cap;
+    tk.r_len = min(ts.cap, r.y - tk.r_off);
+    tk.s_off = s.x + (li % nsc) * ts.chunk;
+    tk.s_len = min(ts.chunk, s.y - tk.s_off);
+    return tk;
+}
+
+// Partition of task t: binary search over task_off.
+__device__ __forceinline__ uint32_t task_part(const TaskSrc& ts, uint32_t t) {
+    uint32_t lo = 0, hi = ts.nparts;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if (task_off[mid] <= t) lo = mid;
+        if (ts.task_off[mid] <= t) lo = mid;
         else hi = mid;
     }
-    const uint2 r = bounds[lo], s = bounds[nparts + lo];
-    const uint32_t nsc = (s.y - s.x + chunk - 1) / chunk;
-    const uint32_t li = t - task_off[lo];
-    JoinTask tk;
-    tk.r_off = r.x + (li / nsc) * cap;
-    tk.r_len = min(cap, r.y - tk.r_off);
-    tk.s_off = s.x + (li % nsc) * chunk;
-    tk.s_len = min(chunk, s.y - tk.s_off);
-    tasks[t] = tk;
+    return lo;
+}
+
+// The descriptor of task t < ntasks: from the task list below task_cap, else
+// decoded from the per-partition task offsets (only partitions whose R side
+// spans several builds and whose S side several chunks at once can push the
+// count past the list, which the host sizes without knowing the count).
+__device__ __forceinline__ JoinTask get_task(const TaskSrc& ts, uint32_t t) {
+    if (t < ts.task_cap) return ts.tasks[t];
+    return task_in_part(ts, task_part(ts, t), t);
+}
+
+// One thread per list entry t < min(ntasks, task_cap).
+__global__ void k_fill_tasks(TaskSrc ts) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= min(*ts.ntasks_dev, ts.task_cap)) return;
+    ts.tasks[t] = task_in_part(ts, task_part(ts, t), t);
 }
 
 // ---------------------------------------------------------------------------
@@ -125,20 +141,23 @@ __global__ void k_fill_tasks(const uint2* __restrict__ bounds, uint32_t nparts, 
 // (R.payload + S.payload), [2] = output cursor.
 
 // Looks one S tuple up in the LDS table; EMIT=false counts matches and sums
-// the checksum, EMIT=true writes them from output slot o on.
+// the checksum, EMIT=true writes them from output slot o on: (R.payload,
+// S.payload) pairs, or with rel the (key, R.payload) tuples of the
+// to_relation format.
 template <int MODE, bool EMIT>
 __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hmask,
                                           const uint32_t* rkey, const uint32_t* rpay,
                                           const uint32_t* head, const uint16_t* nxt,
                                           uint32_t& c, uint64_t& cks, uint32_t& first,
-                                          uint2* __restrict__ pairs, uint64_t cap, uint64_t& o) {
+                                          uint2* __restrict__ pairs, uint64_t cap, uint64_t& o,
+                                          bool rel = false) {
     const uint32_t key = sv.x;
     const uint32_t h = (key >> hshift) & hmask;
     if (MODE == kModeChain) {
         for (uint32_t e = head[h]; e; e = nxt[e - 1]) {
             if (rkey[e - 1] == key) {
                 if (EMIT) {
-                    if (o < cap) pairs[o] = make_uint2(rpay[e - 1], sv.y);
+                    if (o < cap) pairs[o] = rel ? make_uint2(key, rpay[e - 1]) : make_uint2(rpay[e - 1], sv.y);
                     o++;
                 } else {
                     first = rpay[e - 1];
@@ -152,7 +171,7 @@ __device__ __forceinline__ void lds_probe(uint2 sv, uint32_t hshift, uint32_t hm
         for (uint32_t e = head[h]; e < en; e++) {
             if (rkey[e] == key) {
                 if (EMIT) {
-                    if (o < cap) pairs[o] = make_uint2(rpay[e], sv.y);
+                    if (o < cap) pairs[o] = rel ? make_uint2(key, rpay[e]) : make_uint2(rpay[e], sv.y);
                     o++;
                 } else {
                     first = rpay[e];
@@ -307,9 +326,8 @@ __device__ __forceinline__ void flush_counters(uint64_t matches, uint64_t cks,
 // kJoinBlock tuples.
 template <int MODE, int SITEMS = kJoinSItems>
 __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join(
-    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, const JoinTask* __restrict__ tasks,
-    const uint32_t* __restrict__ ntasks_dev, uint32_t* __restrict__ task_ctr, uint32_t hshift,
-    unsigned long long* __restrict__ counters) {
+    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, TaskSrc ts,
+    uint32_t* __restrict__ task_ctr, uint32_t hshift, unsigned long long* __restrict__ counters) {
     constexpr int RITEMS = kJoinCap / kJoinBlock;
     constexpr int NHEAD = MODE == kModeChain ? kJoinCap : kJoinCap / 2 + 1;
     __shared__ uint32_t rkey[kJoinCap];
@@ -321,11 +339,11 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
     __shared__ JoinTask tk_s;
 
-    const uint32_t ntasks = *ntasks_dev;
+    const uint32_t ntasks = *ts.ntasks_dev;
     uint64_t matches = 0, cks = 0;
     if (threadIdx.x == 0) {
         task_s = atomicAdd(task_ctr, 1u);
-        if (task_s < ntasks) tk_s = tasks[task_s];
+        if (task_s < ntasks) tk_s = get_task(ts, task_s);
     }
     for (uint32_t i = threadIdx.x; i < NHEAD; i += kJoinBlock) head[i] = 0;
     __syncthreads();
@@ -347,7 +365,7 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
         lds_build<MODE>(rv, n, hshift, hmask, nb, rkey, rpay, head, nxt, scratch);
         __syncthreads();
 
-        if (threadIdx.x == 0 && next_t < ntasks) nd = tasks[next_t];
+        if (threadIdx.x == 0 && next_t < ntasks) nd = get_task(ts, next_t);
         for (uint32_t s0 = 0; s0 < tk.s_len; s0 += SITEMS * kJoinBlock) {
             if (s0) load_range(sv, Sb, tk.s_off, tk.s_len, s0);
             constexpr int G = MODE == kModeChain ? 8 : 4;
@@ -379,11 +397,12 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // before the claim is awaited, so they travel while the claim returns and the
 // pairs are written.  Output slots go in item-major order (all threads' item
 // 0, then item 1, ...) so that a wave's stores of one item are contiguous.
+// rel: (key, R.payload) tuples instead of (R.payload, S.payload) pairs.
 template <int MODE>
 __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_join_mat(
-    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, const JoinTask* __restrict__ tasks,
-    const uint32_t* __restrict__ ntasks_dev, uint32_t* __restrict__ task_ctr, uint32_t hshift,
-    uint2* __restrict__ pairs, uint64_t cap, unsigned long long* __restrict__ counters) {
+    const uint2* __restrict__ Rb, const uint2* __restrict__ Sb, TaskSrc ts,
+    uint32_t* __restrict__ task_ctr, uint32_t hshift, uint2* __restrict__ pairs, uint64_t cap,
+    bool rel, unsigned long long* __restrict__ counters) {
     constexpr int SI = kJoinSItems / 2;
     constexpr int SUB = SI * kJoinBlock;
     constexpr int RITEMS = kJoinCap / kJoinBlock;
@@ -404,13 +423,13 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     __shared__ unsigned long long obase_s;
     __shared__ unsigned long long red_m[NW], red_c[NW];
 
-    const uint32_t ntasks = *ntasks_dev;
+    const uint32_t ntasks = *ts.ntasks_dev;
     const uint32_t lane = lane_id(), w = wave_id();
     uint64_t matches = 0, cks = 0;
     if (threadIdx.x == 0) {
         const uint32_t t = atomicAdd(task_ctr, 1u);
         first_s = t;
-        if (t < ntasks) tk_s = tasks[t];
+        if (t < ntasks) tk_s = get_task(ts, t);
     }
     for (uint32_t i = threadIdx.x; i < NHEAD; i += kJoinBlock) head[i] = 0;
     __syncthreads();
@@ -436,7 +455,7 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
         __syncthreads();
         const uint32_t ntu = next_s;
         const bool has_next = ntu < ntasks;
-        const JoinTask nd = tasks[has_next ? ntu : 0];
+        const JoinTask nd = get_task(ts, has_next ? ntu : 0);
         for (uint32_t s0 = 0;; s0 += SUB) {
             uint32_t ck[SI], fp[SI];
 #pragma unroll
@@ -458,11 +477,11 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
             }
             __syncthreads();
             const bool last = s0 + SUB >= tk.s_len;
-            // the payloads are kept for the writes; an item with several
-            // matches re-reads its tuple (rare: R keys repeat)
+            // the S payloads (keys for rel) are kept for the writes; an item
+            // with several matches re-reads its tuple (rare: R keys repeat)
             uint32_t cpay[SI];
 #pragma unroll
-            for (int k = 0; k < SI; k++) cpay[k] = sv[k].y;
+            for (int k = 0; k < SI; k++) cpay[k] = rel ? sv[k].x : sv[k].y;
             if (!last) {
                 load_range(sv, Sb, tk.s_off, tk.s_len, s0 + SUB);
             } else if (has_next) {
@@ -494,13 +513,13 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
                 const uint32_t incl = wave_inclusive_scan(ck[q]);
                 uint64_t o = obase + woff[ph][w][q] + incl - ck[q];
                 if (ck[q] == 1) {
-                    if (o < cap) pairs[o] = make_uint2(fp[q], cpay[q]);
+                    if (o < cap) pairs[o] = rel ? make_uint2(cpay[q], fp[q]) : make_uint2(fp[q], cpay[q]);
                 } else if (ck[q] > 1) {
                     uint32_t c2 = 0, f2 = 0;
                     uint64_t k2 = 0;
                     const uint2 v = Sb[tk.s_off + s0 + q * kJoinBlock + threadIdx.x];
                     lds_probe<MODE, true>(v, hshift, hmask, rkey, rpay, head, nxt, c2, k2, f2,
-                                          pairs, cap, o);
+                                          pairs, cap, o, rel);
                 }
             }
             ph ^= 1;
@@ -633,12 +652,10 @@ __device__ void lds_sort_u64(const unsigned long long (&v)[ITEMS], uint32_t n,
 }
 
 __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restrict__ Rb,
-                                                          const uint2* __restrict__ Sb,
-                                                          const JoinTask* __restrict__ tasks,
-                                                          const uint32_t* __restrict__ ntasks_dev,
+                                                          const uint2* __restrict__ Sb, TaskSrc ts,
                                                           uint32_t* __restrict__ task_ctr,
                                                           uint2* __restrict__ pairs, uint64_t cap,
-                                                          bool mat,
+                                                          bool mat, bool rel,
                                                           unsigned long long* __restrict__ counters) {
     constexpr int RITEMS = kJoinCap / kJoinBlock;
     constexpr int G = kJoinProbeItems;  // S tuples searched in lockstep per thread
@@ -647,16 +664,20 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
     __shared__ uint32_t scratch[kJoinBlock / TBJ_WAVE + 1];
     __shared__ uint32_t mm[2], big;
     __shared__ uint32_t task_s;
+    __shared__ JoinTask tk_s;
     __shared__ unsigned long long obase_s;
     __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
-    const uint32_t ntasks = *ntasks_dev;
+    const uint32_t ntasks = *ts.ntasks_dev;
     uint64_t matches = 0, cks = 0;
     for (;;) {
-        if (threadIdx.x == 0) task_s = atomicAdd(task_ctr, 1u);
+        if (threadIdx.x == 0) {
+            task_s = atomicAdd(task_ctr, 1u);
+            if (task_s < ntasks) tk_s = get_task(ts, task_s);
+        }
         __syncthreads();
         const uint32_t t = task_s;
         if (t >= ntasks) break;
-        const JoinTask tk = tasks[t];
+        const JoinTask tk = tk_s;
         const uint32_t n = tk.r_len;
         {
             unsigned long long v[RITEMS];
@@ -729,7 +750,9 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
 #pragma unroll
                     for (int k = 0; k < G; k++) {
                         for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
-                            if (o < cap) pairs[o] = make_uint2((uint32_t)srt[e], sv[k].y);
+                            if (o < cap)
+                                pairs[o] = rel ? make_uint2(sv[k].x, (uint32_t)srt[e])
+                                               : make_uint2((uint32_t)srt[e], sv[k].y);
                             o++;
                         }
                     }
@@ -763,42 +786,37 @@ void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, ui
     launch_scan(st, task_off, (uint64_t)nparts + 1, partial, ntasks_dev);
 }
 
-void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
-                       uint32_t s_chunk, const uint32_t* task_off, uint32_t ntasks,
-                       JoinTask* tasks) {
-    if (ntasks)
-        hipLaunchKernelGGL(k_fill_tasks, dim3((ntasks + 255) / 256), dim3(256), 0, st, bounds,
-                           nparts, cap, s_chunk, task_off, ntasks, tasks);
+void launch_fill_tasks(hipStream_t st, const TaskSrc& ts) {
+    if (ts.task_cap)
+        hipLaunchKernelGGL(k_fill_tasks, dim3((ts.task_cap + 255) / 256), dim3(256), 0, st, ts);
 }
 
 template <int MODE>
-static void launch_join_t(hipStream_t st, bool mat, uint32_t grid, const uint2* Rb,
-                          const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
-                          uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
-                          unsigned long long* counters) {
+static void launch_join_t(hipStream_t st, bool mat, bool rel, uint32_t grid, const uint2* Rb,
+                          const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr, uint32_t hshift,
+                          uint2* pairs, uint64_t cap, unsigned long long* counters) {
     if (mat)
-        hipLaunchKernelGGL((k_join_mat<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
-                           ntasks_dev, task_ctr, hshift, pairs, cap, counters);
+        hipLaunchKernelGGL((k_join_mat<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts,
+                           task_ctr, hshift, pairs, cap, rel, counters);
     else
-        hipLaunchKernelGGL((k_join<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
-                           ntasks_dev, task_ctr, hshift, counters);
+        hipLaunchKernelGGL((k_join<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts,
+                           task_ctr, hshift, counters);
 }
 
-void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, const uint2* Rb,
-                 const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
-                 uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
-                 unsigned long long* counters) {
+void launch_join(hipStream_t st, int mode, bool materialize, bool rel, uint32_t grid,
+                 const uint2* Rb, const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr,
+                 uint32_t hshift, uint2* pairs, uint64_t cap, unsigned long long* counters) {
     if (mode == kModeSort) {
-        hipLaunchKernelGGL(k_join_sort, dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, tasks,
-                           ntasks_dev, task_ctr, pairs, cap, materialize, counters);
+        hipLaunchKernelGGL(k_join_sort, dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts, task_ctr,
+                           pairs, cap, materialize, rel, counters);
         return;
     }
     if (mode == kModeChain)
-        launch_join_t<kModeChain>(st, materialize, grid, Rb, Sb, tasks, ntasks_dev, task_ctr,
-                                  hshift, pairs, cap, counters);
+        launch_join_t<kModeChain>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, pairs,
+                                  cap, counters);
     else
-        launch_join_t<kModeHist>(st, materialize, grid, Rb, Sb, tasks, ntasks_dev, task_ctr,
-                                 hshift, pairs, cap, counters);
+        launch_join_t<kModeHist>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, pairs,
+                                 cap, counters);
 }
 
 template <typename K>

@@ -110,13 +110,15 @@ void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* 
 void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
                         uint32_t* ntasks_dev, uint32_t* stats);
-void launch_fill_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
-                       uint32_t s_chunk, const uint32_t* task_off, uint32_t ntasks,
-                       JoinTask* tasks);
-void launch_join(hipStream_t st, int mode, bool materialize, uint32_t grid, const uint2* Rb,
-                 const uint2* Sb, const JoinTask* tasks, const uint32_t* ntasks_dev,
-                 uint32_t* task_ctr, uint32_t hshift, uint2* pairs, uint64_t cap,
-                 unsigned long long* counters);
+// Lists the first ts.task_cap tasks (the grid covers task_cap; the count
+// itself stays on the device).
+void launch_fill_tasks(hipStream_t st, const TaskSrc& ts);
+// Persistent join over the tasks of ts (grid: resident workgroups; those that
+// find no task left exit).  rel: materialise (key, R.payload) tuples instead
+// of (R.payload, S.payload) pairs.
+void launch_join(hipStream_t st, int mode, bool materialize, bool rel, uint32_t grid,
+                 const uint2* Rb, const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr,
+                 uint32_t hshift, uint2* pairs, uint64_t cap, unsigned long long* counters);
 int join_occupancy(int mode, bool materialize);
 
 // No-partitioning joins (tbj_nopart.hip)

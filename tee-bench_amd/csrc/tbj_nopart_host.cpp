@@ -44,9 +44,9 @@ int finish(Context& C, hipStream_t st, Small* sm, bool mat, uint64_t cap, tbj_re
 
 // PHT (no_partitioning_join.h:543-623) and NPO_st (474-541): one shared
 // table of nextpow2(|R| / 2) buckets (allocate_hashtable, 119-151).
-int run_pht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
-            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st) {
-    Context& C = ctx();
+int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+            tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
+            hipStream_t st) {
     Workspace& W = C.ws;
     const bool mat = opt.materialize && pairs && pairs_cap;
     if (nR >= (1ull << 32) || nS >= (1ull << 40)) return TBJ_ERR_TOO_LARGE;
@@ -75,9 +75,9 @@ int run_pht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tb
 
 // CHT (CHTJoinWrapper.cpp:56-114 with CHTJoin<128 partitions>): bitmap of
 // nextpow2(|R|) bits, partitions of bitmap/128 bits.
-int run_cht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
-            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st) {
-    Context& C = ctx();
+int run_cht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+            tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
+            hipStream_t st) {
     Workspace& W = C.ws;
     const bool mat = opt.materialize && pairs && pairs_cap;
     if (nR >= (1ull << 31) || nS >= (1ull << 40)) return TBJ_ERR_TOO_LARGE;

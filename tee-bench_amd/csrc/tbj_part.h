@@ -45,6 +45,20 @@ struct JoinTask {
     uint32_t r_off, r_len, s_off, s_len;
 };
 
+// Where the join kernels find their tasks: partition p (bounds[p] = R range,
+// bounds[nparts + p] = S range) holds tasks [task_off[p], task_off[p + 1]),
+// *ntasks_dev in all; the first task_cap of them are also listed in tasks.
+struct TaskSrc {
+    JoinTask* tasks;
+    const uint32_t* task_off;
+    const uint2* bounds;
+    const uint32_t* ntasks_dev;
+    uint32_t task_cap;  // entries of tasks
+    uint32_t nparts;
+    uint32_t cap;       // R tuples per build
+    uint32_t chunk;     // S tuples per task
+};
+
 // Piecewise-linear key-CDF model (learned range partitioner).  Knot i
 // (key x[i]) has cumulative value y[i] in units of 2^-frac partitions; the
 // knots sit at equal-mass sample quantiles between R's exact minimum kmin

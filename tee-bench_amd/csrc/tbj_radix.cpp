@@ -98,13 +98,15 @@ struct Job {
     uint2* pairs;
     uint64_t cap;
     bool mat;
+    bool rel;  // materialise (key, R.payload) tuples
     int mode;
 };
 
-// One full run; *overflow_out reports a too-small sample-sized region.
-int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
+// One full run; *overflow_out reports a too-small sample-sized region.  The
+// host waits for the device once, at the end (and once more for the learned
+// partitioner's judge).
+int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
              hipStream_t st, bool* overflow_out) {
-    Context& C = ctx();
     Workspace& W = C.ws;
     PhaseTimer pt(C, st, opt.profile != 0);
     *overflow_out = false;
@@ -308,25 +310,35 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
                                  ? (uint32_t)opt.s_chunk : max_chunk;
     launch_count_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, tpartial, &sm->ntasks,
                        sm->stats);
-    Small h;
-    TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small), hipMemcpyDeviceToHost, st));
-    TBJ_HIP(hipStreamSynchronize(st));
-    res->max_r_partition = h.stats[0];
-    JoinTask* tasks = nullptr;
-    TBJ_TRY(W.get("tasks", sizeof(JoinTask) * std::max<uint32_t>(1, h.ntasks), (void**)&tasks));
-    launch_fill_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, h.ntasks, tasks);
+    // the task list is sized without the count: every partition's tasks are
+    // listed unless one spans several builds and several S chunks at once
+    // (then the join decodes the rest from task_off)
+    TaskSrc ts{};
+    ts.task_off = task_off;
+    ts.bounds = bounds;
+    ts.ntasks_dev = &sm->ntasks;
+    ts.nparts = nparts;
+    ts.cap = kJoinCap;
+    ts.chunk = s_chunk;
+    const uint64_t tcap = (uint64_t)nparts + (nR + kJoinCap - 1) / kJoinCap +
+                          (nS + s_chunk - 1) / s_chunk + 64;
+    ts.task_cap = (uint32_t)std::min<uint64_t>(tcap, 1ull << 31);
+    TBJ_TRY(W.get("tasks", sizeof(JoinTask) * ts.task_cap, (void**)&ts.tasks));
+    launch_fill_tasks(st, ts);
     TBJ_TRY(pt.mark(kPhTasks));
     const uint32_t hshift = learned ? 0 : plan.B;
     const uint32_t occ = (uint32_t)join_occupancy(J.mode, J.mat);
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(h.ntasks, C.num_cus * occ));
-    launch_join(st, J.mode, J.mat, grid, Rb, Sb, tasks, &sm->ntasks, &sm->task_ctr, hshift,
-                J.pairs, J.mat ? J.cap : 0, sm->counters);
+    const uint32_t grid = (uint32_t)C.num_cus * occ;
+    launch_join(st, J.mode, J.mat, J.rel, grid, Rb, Sb, ts, &sm->task_ctr, hshift, J.pairs,
+                J.mat ? J.cap : 0, sm->counters);
     TBJ_HIP(hipGetLastError());
     TBJ_TRY(pt.mark(kPhJoin));
     TBJ_HIP(hipEventRecord(C.ev[2], st));
+    Small h;
     TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small), hipMemcpyDeviceToHost, st));
     TBJ_HIP(hipStreamSynchronize(st));
     TBJ_TRY(pt.finish(res));
+    res->max_r_partition = h.stats[0];
     *overflow_out = h.overflow != 0;
     res->matches = h.counters[0];
     res->checksum = h.counters[1];
@@ -343,8 +355,8 @@ int run_once(const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
 
 }  // namespace
 
-int run_radix_join(int algo, const tbj_tuple* R, uint64_t nR64, const tbj_tuple* S,
-                   uint64_t nS64, tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt,
+int run_radix_join(Context& C, int algo, const tbj_tuple* R, uint64_t nR64, const tbj_tuple* S,
+                   uint64_t nS64, void* pairs, uint64_t pairs_cap, const tbj_options& opt,
                    tbj_result* res, hipStream_t st) {
     if (nR64 + nS64 >= (1ull << 32) - (1ull << 20)) return TBJ_ERR_TOO_LARGE;
     Job J;
@@ -356,18 +368,21 @@ int run_radix_join(int algo, const tbj_tuple* R, uint64_t nR64, const tbj_tuple*
     J.pairs = (uint2*)pairs;
     J.cap = pairs_cap;
     J.mat = opt.materialize && pairs && pairs_cap;
-    J.mode = algo == TBJ_RHT ? kModeHist : (algo == TBJ_RSM || algo == TBJ_PSM) ? kModeSort : kModeChain;
+    J.rel = opt.output == TBJ_OUT_RELATION;
+    J.mode = algo == TBJ_RHT ? kModeHist
+             : (algo == TBJ_RSM || algo == TBJ_PSM || algo == TBJ_MWAY) ? kModeSort
+                                                                       : kModeChain;
     if (J.nR == 0 || J.nS == 0) {
         res->partitioner_used = opt.partitioner == TBJ_PART_LEARNED ? TBJ_PART_LEARNED : TBJ_PART_RADIX;
         return TBJ_OK;
     }
     bool overflow = false;
-    TBJ_TRY(run_once(J, opt, opt.sampled_regions == 0, res, st, &overflow));
+    TBJ_TRY(run_once(C, J, opt, opt.sampled_regions == 0, res, st, &overflow));
     if (overflow) {
         // a sample-sized region was too small: redo with exact histograms
         const float first_ms = res->ms_total;
         memset(res, 0, sizeof(*res));
-        TBJ_TRY(run_once(J, opt, true, res, st, &overflow));
+        TBJ_TRY(run_once(C, J, opt, true, res, st, &overflow));
         res->redone_exact = 1;
         res->ms_total += first_ms;
     }

@@ -1,7 +1,10 @@
-// Host runtime: device context, cached workspace, error plumbing.
+// Host runtime: device state, per-stream sessions with cached workspace,
+// error plumbing.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <mutex>
 #include <string>
 #include <unordered_map>
 
@@ -14,8 +17,9 @@ struct Buf {
     size_t bytes = 0;
 };
 
-// Device workspace, grown on demand and kept between calls.  `cap` (bytes,
-// 0 = none) bounds the total and turns an over-cap request into TBJ_ERR_OOM.
+// Device workspace of one session: buffers are grown on demand with
+// hipMalloc and kept between calls, keyed by name; `cap` (bytes, 0 = none)
+// bounds the total and turns an over-cap request into TBJ_ERR_OOM.
 class Workspace {
 public:
     int get(const char* name, size_t bytes, void** out);
@@ -28,11 +32,22 @@ private:
     size_t total_ = 0;
 };
 
-struct Context {
+// Process-wide state of the device the library runs on.
+struct Device {
     bool ready = false;
     int device = -1;
     int num_cus = 0;
+    // default priority; used to capture HIP graphs when the caller's stream
+    // is the legacy default stream, which cannot be captured
     hipStream_t own_stream = nullptr;
+};
+
+// One session per caller stream: its own workspace, events and side stream,
+// so that calls on different streams (from different host threads) run
+// concurrently; calls on one stream serialise on `mu`.
+struct Context {
+    std::mutex mu;
+    int num_cus = 0;
     // a second stream for work beside the caller's (forked and joined
     // through the two events)
     hipStream_t side_stream = nullptr;
@@ -40,6 +55,10 @@ struct Context {
     Workspace ws;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t pev[24] = {};
+    // creates the streams and events (every event, the per-phase ones too,
+    // so that a call allocates nothing on the device afterwards)
+    int setup(int num_cus);
+    void destroy();
 };
 
 // Per-phase device timing (tbj_options.profile): mark(phase) records an event
@@ -59,7 +78,7 @@ private:
     int phase_[24] = {};
 };
 
-Context& ctx();
+Device& dev();
 
 #define TBJ_HIP(call)                                            \
     do {                                                         \
@@ -77,17 +96,20 @@ Context& ctx();
     } while (0)
 
 void set_last_error(const char* what, hipError_t e, const char* file, int line);
+void set_last_message(const char* msg);
 
-// Algorithms (implemented in the tbj_*.cpp files).
-int run_radix_join(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
-                   tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
-                   hipStream_t st);
-int run_pht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
-            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
-int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR, const tbj_tuple64* S, uint64_t nS,
-             tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
-             hipStream_t st);
-int run_cht(const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS, tbj_pair* pairs,
-            uint64_t pairs_cap, const tbj_options& opt, tbj_result* res, hipStream_t st);
+// Algorithms (implemented in the tbj_*.cpp files), each on one session.
+int run_radix_join(Context& C, int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S,
+                   uint64_t nS, void* out, uint64_t out_cap, const tbj_options& opt,
+                   tbj_result* res, hipStream_t st);
+int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+            tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
+            hipStream_t st);
+int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR, const tbj_tuple64* S,
+             uint64_t nS, tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt,
+             tbj_result* res, hipStream_t st);
+int run_cht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+            tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
+            hipStream_t st);
 
 }  // namespace tbj

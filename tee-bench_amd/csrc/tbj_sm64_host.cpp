@@ -49,9 +49,9 @@ Bin64 bin64(uint32_t kind, uint32_t B, uint32_t shift, uint32_t fanout, const Mo
 
 }  // namespace
 
-int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S, uint64_t nS64,
-             tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
-             hipStream_t st) {
+int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S,
+             uint64_t nS64, tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt,
+             tbj_result* res, hipStream_t st) {
     if (nR64 + nS64 >= (1ull << 32) - (1ull << 20)) return TBJ_ERR_TOO_LARGE;
     bool learned = opt.partitioner == TBJ_PART_LEARNED ||
                    (opt.partitioner == TBJ_PART_AUTO && algo == TBJ_PSM);
@@ -61,7 +61,6 @@ int run_sm64(int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S
         res->partitioner_used = learned ? TBJ_PART_LEARNED : TBJ_PART_RADIX;
         return TBJ_OK;
     }
-    Context& C = ctx();
     Workspace& W = C.ws;
     PhaseTimer pt(C, st, opt.profile != 0);
 

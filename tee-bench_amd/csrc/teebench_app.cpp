@@ -3,7 +3,7 @@
 // generators and seeds, runs the selected join on the MI355X and prints the
 // reference's log lines, ending with "Matches = N".
 //
-//   -a ALG   RHO|RHT|RJ|RHO_atomic|PHT|NPO_st|CHT|RSM|PSM (default PHT)
+//   -a ALG   RHO|RHT|RJ|RHO_atomic|PHT|NPO_st|CHT|RSM|PSM|MWAY (default PHT)
 //   -n N     threads (accepted; the GPU path does not use it)
 //   -r/-s N  tuples of R / S          -x/-y MB  size of R / S in MB
 //   -d cache-fit|cache-exceed|L       -l SEL    selectivity of S in %
@@ -17,7 +17,6 @@
 #include <string.h>
 
 #include <algorithm>
-#include <fstream>
 #include <string>
 #include <vector>
 
@@ -135,23 +134,17 @@ Args parse(int argc, char** argv) {
     return a;
 }
 
-// create_relation_from_file (generator.cpp:290-351): one integer key per
-// line, payload 0, then shuffled with the seeded rand() stream.
+// create_relation_from_file (generator.cpp:290-351) through the library's
+// loader: one integer key per line, payload 0, shuffled with the seeded
+// rand() stream.
 bool from_file(const std::string& path, uint32_t seed, std::vector<tbj_tuple>& out) {
-    std::ifstream f(path);
-    if (!f) {
+    const int64_t n = tbj_gen_from_file(seed, path.c_str(), nullptr, 0, TBJ_PAYLOAD_ZERO);
+    if (n < 0) {
         teebench_log(TB_ERROR, "Could not open file %s", path.c_str());
         return false;
     }
-    std::string line;
-    std::vector<uint32_t> keys;
-    while (std::getline(f, line)) keys.push_back((uint32_t)atoi(line.c_str()));
-    out.resize(keys.size());
-    // shuffle exactly like knuth_shuffle: fill a pk-style relation with the
-    // keys permuted by the same swap sequence
-    std::vector<tbj_tuple> perm(keys.size());
-    tbj_gen_pk(seed, keys.size(), perm.data(), TBJ_PAYLOAD_ZERO, 1);
-    for (size_t i = 0; i < keys.size(); i++) out[i] = {keys[perm[i].key - 1], 0};
+    out.resize((size_t)n);
+    tbj_gen_from_file(seed, path.c_str(), out.data(), out.size(), TBJ_PAYLOAD_ZERO);
     return true;
 }
 

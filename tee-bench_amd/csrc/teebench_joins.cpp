@@ -7,6 +7,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/tbj.h"
@@ -41,11 +42,12 @@ result_t* run(int algo, relation_t* R, relation_t* S, int nthreads) {
         std::vector<tbj_tuple> r(R->num_tuples), s(S->num_tuples);
         for (uint64_t i = 0; i < R->num_tuples; i++) r[i] = {R->tuples[i].key, (uint32_t)i};
         for (uint64_t i = 0; i < S->num_tuples; i++) s[i] = {S->tuples[i].key, (uint32_t)i};
-        tbj_options cnt = opt;
-        cnt.materialize = 0;
-        rc = tbj_join_host(algo, r.data(), r.size(), s.data(), s.size(), nullptr, 0, &cnt, &res);
-        std::vector<tbj_pair> pairs;
-        if (rc == TBJ_OK && res.matches) {
+        // room for one match per S tuple (every foreign-key join); a join
+        // with more matches runs once more with the exact room
+        std::vector<tbj_pair> pairs(std::max<uint64_t>(1, S->num_tuples));
+        rc = tbj_join_host(algo, r.data(), r.size(), s.data(), s.size(), pairs.data(),
+                           pairs.size(), &opt, &res);
+        if (rc == TBJ_OK && res.matches > res.pairs_written) {
             pairs.resize(res.matches);
             rc = tbj_join_host(algo, r.data(), r.size(), s.data(), s.size(), pairs.data(),
                                pairs.size(), &opt, &res);
@@ -53,23 +55,26 @@ result_t* run(int algo, relation_t* R, relation_t* S, int nthreads) {
         if (rc == TBJ_OK) {
             out->resultlist = (threadresult_t*)malloc(sizeof(threadresult_t));
             out->nthreads = 1;
-            out->resultlist[0].nresults = (int64_t)res.matches;
+            out->resultlist[0].nresults = (int64_t)res.pairs_written;
             out->resultlist[0].threadid = 0;
-            output_list_t* head = nullptr;
-            for (uint64_t k = res.pairs_written; k-- > 0;) {
-                output_list_t* node = (output_list_t*)malloc(sizeof(output_list_t));
-                if (!node) {
+            // one block of nodes, linked in order (teebench_free_result frees
+            // the block through the head)
+            output_list_t* nodes = nullptr;
+            if (res.pairs_written) {
+                nodes = (output_list_t*)malloc(sizeof(output_list_t) * res.pairs_written);
+                if (!nodes) {
                     teebench_log(TB_ERROR, "Failed to allocate memory");
                     exit(EXIT_FAILURE);
                 }
-                const tbj_pair& p = pairs[k];
-                node->key = R->tuples[p.r_payload].key;
-                node->Rpayload = R->tuples[p.r_payload].payload;
-                node->Spayload = S->tuples[p.s_payload].payload;
-                node->next = head;
-                head = node;
             }
-            out->resultlist[0].results = head;
+            for (uint64_t k = 0; k < res.pairs_written; k++) {
+                const tbj_pair& p = pairs[k];
+                nodes[k].key = R->tuples[p.r_payload].key;
+                nodes[k].Rpayload = R->tuples[p.r_payload].payload;
+                nodes[k].Spayload = S->tuples[p.s_payload].payload;
+                nodes[k].next = k + 1 < res.pairs_written ? &nodes[k + 1] : nullptr;
+            }
+            out->resultlist[0].results = nodes;
         }
     }
     if (rc != TBJ_OK) {
@@ -100,6 +105,7 @@ result_t* NPO_st(relation_t* R, relation_t* S, int n) { return run(TBJ_NPO_ST, R
 result_t* CHT(relation_t* R, relation_t* S, int n) { return run(TBJ_CHT, R, S, n); }
 result_t* RSM(relation_t* R, relation_t* S, int n) { return run(TBJ_RSM, R, S, n); }
 result_t* PSM(relation_t* R, relation_t* S, int n) { return run(TBJ_PSM, R, S, n); }
+result_t* MWAY(relation_t* R, relation_t* S, int n) { return run(TBJ_MWAY, R, S, n); }
 
 void teebench_set_materialize(int on) { g_materialize = on ? 1 : 0; }
 void teebench_set_partitioner(int learned) { g_learned = learned ? 1 : 0; }
@@ -107,22 +113,15 @@ void teebench_set_partitioner(int learned) { g_learned = learned ? 1 : 0; }
 void teebench_free_result(result_t* res) {
     if (!res) return;
     if (res->resultlist) {
-        for (int t = 0; t < res->nthreads; t++) {
-            output_list_t* node = res->resultlist[t].results;
-            for (int64_t i = 0; i < res->resultlist[t].nresults && node; i++) {
-                output_list_t* nxt = node->next;
-                free(node);
-                node = nxt;
-            }
-        }
+        for (int t = 0; t < res->nthreads; t++) free(res->resultlist[t].results);
         free(res->resultlist);
     }
     free(res);
 }
 
 // native.cpp:34-46 with the GPU implementations (the reference's CPU-only
-// NL, INL and MWAY selectors are not part of this library).
+// nested-loop joins NL and INL are not part of this library).
 algorithm_t teebench_algorithms[] = {
     {"PHT", PHT}, {"NPO_st", NPO_st}, {"RJ", RJ},   {"PSM", PSM},
     {"RHO", RHO}, {"RHT", RHT},       {"RSM", RSM}, {"CHT", CHT},
-    {"RHO_atomic", RHO_atomic},       {"", nullptr}};
+    {"MWAY", MWAY}, {"RHO_atomic", RHO_atomic}, {"", nullptr}};

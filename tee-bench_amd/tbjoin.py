@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtbj.so")
 
 ALGOS = {"RHO": 0, "RHT": 1, "RJ": 2, "PHT": 3, "CHT": 4, "RSM": 5, "PSM": 6,
-         "RHO_atomic": 7, "NPO_st": 8}
+         "RHO_atomic": 7, "NPO_st": 8, "MWAY": 9}
+OUT_PAIRS, OUT_RELATION = 0, 1
 PART_AUTO, PART_RADIX, PART_LEARNED = 0, 1, 2
 PARTITIONERS = {"auto": PART_AUTO, "radix": PART_RADIX, "learned": PART_LEARNED}
 
@@ -30,7 +31,7 @@ class Options(ctypes.Structure):
                 ("workspace_cap", ctypes.c_uint64), ("learned_bound", ctypes.c_uint32),
                 ("learned_samples", ctypes.c_int32), ("s_chunk", ctypes.c_int32),
                 ("sampled_regions", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 3)]
+                ("output", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
 
 
 class Result(ctypes.Structure):
@@ -90,6 +91,11 @@ def lib():
         L.tbj_join64.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64,
                                  ctypes.POINTER(Options), ctypes.POINTER(Result), vp]
         L.tbj_join64.restype = ctypes.c_int
+        L.tbj_join3.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64, vp, u64,
+                                ctypes.POINTER(Options), ctypes.POINTER(Result), vp]
+        L.tbj_join3.restype = ctypes.c_int
+        L.tbj_release_stream.argtypes = [vp]
+        L.tbj_release_stream.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -105,7 +111,7 @@ def init(device=0):
 
 def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
                  workspace_cap=0, learned_bound=0, learned_samples=0, s_chunk=0,
-                 sampled_regions=False, profile=False):
+                 sampled_regions=False, profile=False, output=OUT_PAIRS):
     o = Options()
     lib().tbj_default_options(ctypes.byref(o))
     o.partitioner = PARTITIONERS[partitioner] if isinstance(partitioner, str) else int(partitioner)
@@ -115,6 +121,7 @@ def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
     o.learned_samples, o.s_chunk = learned_samples, s_chunk
     o.sampled_regions = 1 if sampled_regions else 0
     o.profile = 1 if profile else 0
+    o.output = int(output)
     return o
 
 
@@ -155,6 +162,29 @@ def join(algo, R, S, pairs=None, **opts):
     stream = torch.cuda.current_stream(R.device).cuda_stream
     return join_ptr(algo, R.data_ptr(), nR, S.data_ptr(), nS,
                     pairs.data_ptr() if pairs is not None else 0, cap, o, stream)
+
+
+def join3(R, S, T, out, algo="RHO", **opts):
+    """Three-way join (R join S) join T on the device: out receives the
+    (key, T.payload) tuples of T joined with out1 = the (key, R.payload)
+    tuples of R join S (int64 [cap] or uint32 [cap, 2])."""
+    for t in (R, S, T, out):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("relations must be contiguous device tensors")
+    n = [x.numel() * x.element_size() // 8 for x in (R, S, T, out)]
+    o = make_options(**opts)
+    import torch
+    stream = torch.cuda.current_stream(R.device).cuda_stream
+    res = Result()
+    check(lib().tbj_join3(_algo(algo), R.data_ptr(), n[0], S.data_ptr(), n[1], T.data_ptr(),
+                          n[2], out.data_ptr(), n[3], ctypes.byref(o), ctypes.byref(res),
+                          stream or None))
+    return res
+
+
+def release_stream(stream):
+    """Frees the library's session (workspace) of a raw hipStream_t."""
+    check(lib().tbj_release_stream(stream or None))
 
 
 def join64_ptr(algo, r_ptr, nR, s_ptr, nS, pairs_ptr=0, pairs_cap=0, options=None, stream=0):

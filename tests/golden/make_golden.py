@@ -9,6 +9,11 @@ the reference's outputs on its own seeds:
   * match counts of every -a selector, quirks included (native.cpp:34-46)
   * CHT's payload checksum with payload = row index    (CHTJoin.hpp:287-302)
   * RHO's materialised pairs with payload = row index  (radix_join.c:231-233)
+  * MWAY counts, duplicate keys on both sides too      (mway/joincommon.c:241-316)
+  * create_relation_from_file on the committed key files (generator.cpp:290-351)
+  * the three-way join (R join S) join T of secure_joins.cpp:348-399 built from
+    the reference RHO's materialised output and to_relation's tuple copy
+    (secure_joins.cpp:277-301)
 
 Usage: python3 tests/golden/make_golden.py
 """
@@ -36,6 +41,9 @@ def load(name):
                                   ctypes.c_int]
     L.ref_gen_zipf.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64, ctypes.c_double,
                                u32p, ctypes.c_int]
+    L.ref_gen_from_file.restype = ctypes.c_int64
+    L.ref_gen_from_file.argtypes = [ctypes.c_uint32, ctypes.c_char_p, u32p, ctypes.c_uint64,
+                                    ctypes.c_int]
     L.ref_join_count.restype = ctypes.c_int64
     L.ref_join_count.argtypes = [ctypes.c_char_p, u32p, vp, ctypes.c_uint64, u32p, vp,
                                  ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
@@ -137,7 +145,7 @@ def main():
         rp = np.arange(nR, dtype=np.uint32)
         sp = np.arange(nS, dtype=np.uint32)
         counts = {}
-        for algo in ("RHO", "RHT", "RJ", "PHT", "NPO_st", "PSM", "RSM", "CHT"):
+        for algo in ("RHO", "RHT", "RJ", "PHT", "NPO_st", "PSM", "RSM", "CHT", "MWAY"):
             # CHT aborts on a probe miss in its unbatched tail (CHT.hpp:194-206)
             if algo == "CHT" and nS % 16 and skind == "fk_sel":
                 counts[algo] = None
@@ -175,6 +183,69 @@ def main():
             order = np.lexsort((b, a))
             entry["pairs"] = [[int(x), int(y)] for x, y in zip(a[order], b[order])]
         g["pairs"].append(entry)
+
+    # MWAY with duplicate keys on both sides: R = fk over 2^14 (4 copies of
+    # each key), S = fk over 2^14 (16 copies); true count 2^14 * 4 * 16
+    rk = gen(C, "fk", 11111, 1 << 16, 1 << 14)
+    sk = gen(C, "fk", 22222, 1 << 18, 1 << 14)
+    rp = np.arange(1 << 16, dtype=np.uint32)
+    sp = np.arange(1 << 18, dtype=np.uint32)
+    g["mway_dups"] = {"nR": 1 << 16, "R": "fk", "pR": 1 << 14, "nS": 1 << 18, "S": "fk",
+                      "pS": 1 << 14, "r_sha256": sha(rk), "s_sha256": sha(sk), "counts": {}}
+    for algo in ("MWAY", "RHO", "PHT"):
+        g["mway_dups"]["counts"][algo] = in_child(
+            lambda algo=algo: C.ref_join_count(algo.encode(), rk, rp.ctypes.data, len(rk), sk,
+                                               sp.ctypes.data, len(sk), 4, 1, None, None))
+
+    # relations from files (-t / -u): keys as the reference loads and
+    # shuffles them, and the join counts on them
+    g["files"] = []
+    for rfile, sfile in (("r_keys.txt", "s_keys.txt"),):
+        ent = {"r_file": rfile, "s_file": sfile, "r_seed": 11111, "s_seed": 22222}
+        keys = {}
+        for side, fname, seed in (("r", rfile, 11111), ("s", sfile, 22222)):
+            path = os.path.join(HERE, fname)
+            n = sum(1 for _ in open(path))
+            out = np.zeros(n, np.uint32)
+            got = in_child(lambda: [C.ref_gen_from_file(seed, path.encode(), out, n, 1),
+                                    out.tolist()])
+            assert got != "crash" and got[0] == n, got
+            keys[side] = np.array(got[1], np.uint32)
+            ent[side + "_n"] = n
+            ent[side + "_sha256"] = sha(keys[side])
+            ent[side + "_head"] = keys[side][:16].tolist()
+        ent["counts"] = {}
+        zr = np.zeros(len(keys["r"]), np.uint32)
+        zs = np.zeros(len(keys["s"]), np.uint32)
+        for algo in ("RHO", "PHT", "RSM", "MWAY"):
+            ent["counts"][algo] = in_child(
+                lambda algo=algo: C.ref_join_count(algo.encode(), keys["r"], zr.ctypes.data,
+                                                   len(keys["r"]), keys["s"], zs.ctypes.data,
+                                                   len(keys["s"]), 2, 1, None, None))
+        g["files"].append(ent)
+
+    # three-way join: out1 = to_relation(RHO(R, S)) = (key, R.payload) per
+    # match; out2 = to_relation(RHO(T, out1)) = (key, T.payload) per match
+    g["three_way"] = []
+    for name, nR, nS, nT, tkind in (("fk T", 1 << 14, 1 << 16, 1 << 15, "fk"),
+                                    ("zipf T", 1 << 14, 1 << 16, 1 << 15, "zipf")):
+        rk = gen(C, "pk", 11111, nR)
+        sk = gen(C, "fk", 22222, nS, nR)
+        tk = gen(C, "fk", 33333, nT, nR) if tkind == "fk" else gen(C, "zipf", 33333, nT, nR, 1.0)
+        rp = np.arange(nR, dtype=np.uint32)
+        sp = np.arange(nS, dtype=np.uint32)
+        tp = np.arange(nT, dtype=np.uint32)
+        cap = 4 * (nS + nT) + (1 << 16)
+        a, b, k = (np.zeros(cap, np.uint32) for _ in range(3))
+        n1 = M.ref_join_pairs(b"RHO", rk, rp, nR, sk, sp, nS, 4, 1, a, b, k, cap)
+        o1k, o1p = k[:n1].copy(), a[:n1].copy()  # (key, R.payload)
+        a2, b2, k2 = (np.zeros(cap, np.uint32) for _ in range(3))
+        n2 = M.ref_join_pairs(b"RHO", tk, tp, nT, o1k, o1p, n1, 4, 1, a2, b2, k2, cap)
+        assert n2 >= 0
+        o2k, o2p = k2[:n2], a2[:n2]  # (key, T.payload)
+        g["three_way"].append({"name": name, "nR": nR, "nS": nS, "nT": nT, "T": tkind,
+                               "t_seed": 33333, "join1": int(n1), "matches": int(n2),
+                               "sorted_out2_sha256": sorted_pairs_sha(o2k, o2p)})
 
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(g, f, indent=1)

@@ -51,7 +51,10 @@ class Oracle:
         L.orc_textbook_pairs64.argtypes = [_u64, _vp, _U64, _u64, _vp, _U64, _vp, _vp, _U64]
         L.orc_textbook_pairs64.restype = ctypes.c_int64
         L.orc_sort_pairs32.argtypes = [_u32, _u32, _U64]
-        L.orc_sort_pairs64.argtypes = [_u64, _u64, _U64]
+        L.orc_mway.argtypes = [_u32, _U64, _u32, _U64]
+        L.orc_mway.restype = ctypes.c_int64
+        L.orc_join_relation.argtypes = [_u32, _vp, _U64, _u32, _U64, _vp, _vp, _U64]
+        L.orc_join_relation.restype = ctypes.c_int64
         self.L = L
 
     # -- generators (keys only, like the reference)
@@ -116,6 +119,28 @@ class Oracle:
     def sortmerge(self, rk, sk):
         return self.L.orc_sortmerge(np.ascontiguousarray(rk, np.uint32), len(rk),
                                     np.ascontiguousarray(sk, np.uint32), len(sk))
+
+    def mway(self, rk, sk):
+        return self.L.orc_mway(np.ascontiguousarray(rk, np.uint32), len(rk),
+                               np.ascontiguousarray(sk, np.uint32), len(sk))
+
+    def join_relation(self, rk, rp, sk):
+        """The join as sorted (key, R.payload) tuples (to_relation's format)."""
+        rk, sk = np.ascontiguousarray(rk, np.uint32), np.ascontiguousarray(sk, np.uint32)
+        rp = np.ascontiguousarray(rp, np.uint32)
+        n = self.L.orc_join_relation(rk, rp.ctypes.data, len(rk), sk, len(sk), None, None, 0)
+        k, p = np.zeros(max(n, 1), np.uint32), np.zeros(max(n, 1), np.uint32)
+        self.L.orc_join_relation(rk, rp.ctypes.data, len(rk), sk, len(sk), k.ctypes.data,
+                                 p.ctypes.data, n)
+        return n, k[:n], p[:n]
+
+    def three_way(self, rk, rp, sk, tk, tp):
+        """(R join S) join T as secure_joins.cpp:348-399 runs it: out1 = the
+        (key, R.payload) relation of R join S, out2 = the (key, T.payload)
+        relation of T join out1.  Returns (|out1|, sorted out2 keys, payloads)."""
+        n1, k1, _ = self.join_relation(rk, rp, sk)
+        n2, k2, p2 = self.join_relation(tk, tp, k1)
+        return n1, k2, p2
 
     def count(self, rk, sk):
         return self.L.orc_textbook_count(np.ascontiguousarray(rk, np.uint32), len(rk),

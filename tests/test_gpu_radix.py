@@ -51,20 +51,38 @@ def test_pass_shapes(tbj, oracle, passes, bits):
     assert res.passes == passes and res.radix_bits == bits
 
 
-def test_duplicates_and_extreme_keys(tbj, oracle):
+# every join family that partitions: bucket chains, histogram ranges, and the
+# LDS sort with its bucket-directory search (RSM radix, PSM learned ranges,
+# MWAY = RSM's kernels with the bounded merge)
+PART_ALGOS = ["RHO", "RHT", "RSM", "PSM", "MWAY"]
+
+
+@pytest.mark.parametrize("algo", PART_ALGOS)
+def test_duplicates_and_extreme_keys(tbj, oracle, algo):
+    # keys 0 and 0xFFFFFFFF make a sort task's key span 2^32 (the edge of the
+    # sort bucket's exact floor)
     rng = np.random.default_rng(1)
     rk = np.concatenate([rng.integers(0, 5000, 20000), [0, 0, 0xFFFFFFFF, 0xFFFFFFFE]])
     sk = np.concatenate([rng.integers(0, 6000, 30000), [0, 0xFFFFFFFF, 0xFFFFFFFF]])
     R, S = rel(rk), rel(sk)
     n = oracle.count(R[:, 0], S[:, 0])
-    for algo in ("RHO", "RHT"):
-        res, got = gu.run(tbj, algo, R, S, cap=n)
-        _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
-        ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
-        assert res.matches == n and np.array_equal(ga, ra) and np.array_equal(gb, sa)
+    res, got = gu.run(tbj, algo, R, S, cap=n)
+    _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert res.matches == n and np.array_equal(ga, ra) and np.array_equal(gb, sa)
 
 
-@pytest.mark.parametrize("algo", ["RHO", "RHT"])
+@pytest.mark.parametrize("algo", PART_ALGOS)
+def test_one_partition_spanning_key_range(tbj, oracle, algo):
+    # a single task whose R range holds 0 and 0xFFFFFFFF (span 2^32) and a
+    # repeated key (a sort bucket far above 32 entries: the bitonic path)
+    rk = np.concatenate([[0, 0xFFFFFFFF, 7, 7], np.full(300, 12345), np.arange(100, 1100)])
+    sk = np.concatenate([[0, 0xFFFFFFFF, 0xFFFFFFFF, 7], np.full(5, 12345),
+                         np.arange(50, 1500)])
+    assert_join(tbj, oracle, algo, rel(rk), rel(sk))
+
+
+@pytest.mark.parametrize("algo", PART_ALGOS)
 @pytest.mark.parametrize("ns", [1, 4095, 4096, 4097, 16384, 16385, 40000])
 def test_materialising_join_sub_chunk_edges(tbj, oracle, algo, ns):
     # one partition (|R| below one table): the task list splits S into tasks of
@@ -80,17 +98,33 @@ def test_materialising_join_sub_chunk_edges(tbj, oracle, algo, ns):
     assert res.checksum == gu.checksum(ra, sa)
 
 
-def test_oversized_partition_is_chunked(tbj, oracle):
-    # one key repeated far beyond the LDS table capacity on the build side
+@pytest.mark.parametrize("algo", PART_ALGOS)
+def test_oversized_partition_is_chunked(tbj, oracle, algo):
+    # one key repeated far beyond the LDS table capacity on the build side:
+    # its partition splits into several R builds (and, for the sort join, a
+    # bucket of 4096 equal keys goes through the bitonic path)
     rk = np.concatenate([np.full(10000, 42), np.arange(1, 3000)])
     sk = np.concatenate([np.full(50, 42), np.arange(1, 9000)])
     R, S = rel(rk), rel(sk)
     n = oracle.count(R[:, 0], S[:, 0])
-    res, got = gu.run(tbj, "RHO", R, S, cap=n)
+    res, got = gu.run(tbj, algo, R, S, cap=n)
     _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
     ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
     assert res.matches == n and res.max_r_partition > 4096
     assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+
+
+@pytest.mark.parametrize("algo", ["RHO", "RHT", "RSM"])
+def test_tasks_beyond_the_task_list(tbj, oracle, algo):
+    # The host sizes the task list without the count.  One partition whose R
+    # side spans 12 builds and whose S side 12 chunks holds 144 tasks, more
+    # than the list (partitions + R builds + S chunks + 64 = 32 + 12 + 12 + 64):
+    # the join decodes the rest from the per-partition task offsets.
+    nr, ns = 12 * 4096, 12 * 1024
+    R, S = rel(np.full(nr, 42)), rel(np.full(ns, 42))
+    res, _ = gu.run(tbj, algo, R, S, materialize=False, s_chunk=1024, radix_bits=5)
+    assert res.matches == nr * ns
+    assert res.checksum == (ns * (nr * (nr - 1) // 2) + nr * (ns * (ns - 1) // 2)) % (1 << 64)
 
 
 def test_empty_and_tiny(tbj, oracle):

@@ -49,7 +49,7 @@ def test_c_abi_exports_every_declared_symbol(tb):
 def test_reference_api_keeps_cpp_names():
     # the reference's entry points have C++ linkage (_Z3RHOP7table_tS0_i etc.)
     syms = exported(os.path.join(PKG, "libteebench.so"))
-    for name in ("RHO", "RHT", "PHT", "CHT", "RSM", "PSM"):
+    for name in ("RHO", "RHT", "PHT", "CHT", "RSM", "PSM", "MWAY"):
         assert f"_Z{len(name)}{name}P7table_tS0_i" in syms
     assert "_Z2RJP7table_tS0_i" in syms
     assert "_Z6NPO_stP7table_tS0_i" in syms
@@ -87,7 +87,8 @@ def test_status_and_names(tb):
     for name, idx in tb.ALGOS.items():
         assert L.tbj_algo_from_name(name.encode()) == idx
         assert L.tbj_algo_name(idx).decode() == name
-    assert L.tbj_algo_from_name(b"MWAY") == tb.ERR_INVALID
+    assert L.tbj_algo_from_name(b"MWAY") == tb.ALGOS["MWAY"]
+    assert L.tbj_algo_from_name(b"NL") == tb.ERR_INVALID
     assert L.tbj_status_str(tb.ERR_OOM) == b"out of device memory"
 
 
@@ -152,6 +153,31 @@ def test_cli_unknown_algorithm_exits_success():
 def test_cli_bad_selectivity_exits_failure():
     p = subprocess.run([APP, "-a", "RHO", "-l", "200"], capture_output=True, text=True)
     assert p.returncode != 0 and "Selectivity should be between 0 and 100" in p.stdout
+
+
+def test_file_relations_match_reference(tb):
+    # create_relation_from_file + knuth_shuffle (generator.cpp:290-351) on the
+    # committed key files, against the reference build's keys
+    import ctypes as C
+    gdir = os.path.join(ROOT, "tests", "golden")
+    for f in GOLDEN["files"]:
+        for side in ("r", "s"):
+            path = os.path.join(gdir, f[side + "_file"]).encode()
+            seed = f[side + "_seed"]
+            L = tb.lib()
+            L.tbj_gen_from_file.restype = C.c_int64
+            L.tbj_gen_from_file.argtypes = [C.c_uint32, C.c_char_p, C.c_void_p, C.c_uint64,
+                                            C.c_int]
+            n = L.tbj_gen_from_file(seed, path, None, 0, 0)
+            assert n == f[side + "_n"]
+            out = np.zeros((n, 2), np.uint32)
+            assert L.tbj_gen_from_file(seed, path, out.ctypes.data, n, 0) == n
+            assert sha(out[:, 0]) == f[side + "_sha256"]
+            assert out[:16, 0].tolist() == f[side + "_head"] and not out[:, 1].any()
+            # the shuffle is a permutation of the file's keys
+            keys = np.loadtxt(path.decode(), dtype=np.uint32)
+            assert np.array_equal(np.sort(keys), np.sort(out[:, 0]))
+    assert tb.lib().tbj_gen_from_file(1, b"/nonexistent/keys.txt", None, 0, 0) == tb.ERR_INVALID
 
 
 def test_batch_specs_follow_the_batch_recipe(tb):

@@ -115,6 +115,8 @@ def test_join_counts_match_reference(oracle, case):
     assert oracle.sortmerge(rk, sk) == true
     # PSM's unbounded merge may overrun by one; RSM's drifts (reference quirks)
     assert c["PSM"] in (true, true + 1, "crash")
+    # MWAY's merge join is bounded: the true join
+    assert c["MWAY"] == true == oracle.mway(rk, sk)
     if c["CHT"] is not None:
         rp = np.arange(case["nR"], dtype=np.uint32)
         sp = np.arange(case["nS"], dtype=np.uint32)
@@ -145,3 +147,53 @@ def test_cht_restatement_flags_nondense_drops(oracle):
     m, _, unresolved = oracle.cht(rk, None, sk, None)
     assert oracle.count(rk, sk) == 3
     assert unresolved >= 1 and m < 3
+
+
+def test_mway_duplicates_both_sides(oracle):
+    g = GOLDEN["mway_dups"]
+    rk = oracle.gen_fk(11111, g["nR"], g["pR"])
+    sk = oracle.gen_fk(22222, g["nS"], g["pS"])
+    assert sha(rk) == g["r_sha256"] and sha(sk) == g["s_sha256"]
+    # every key appears nR/pR times in R and nS/pS times in S
+    want = g["pR"] * (g["nR"] // g["pR"]) * (g["nS"] // g["pS"])
+    c = g["counts"]
+    assert c["MWAY"] == c["RHO"] == c["PHT"] == want
+    assert oracle.mway(rk, sk) == want == oracle.count(rk, sk)
+
+
+def test_mway_brute_force(oracle):
+    rng = np.random.default_rng(3)
+    for n_r, n_s, hi in ((1, 1, 2), (17, 40, 5), (300, 500, 40), (1000, 20, 1000)):
+        rk = rng.integers(0, hi, n_r).astype(np.uint32)
+        sk = rng.integers(0, hi, n_s).astype(np.uint32)
+        assert oracle.mway(rk, sk) == int((rk[:, None] == sk[None, :]).sum())
+    assert oracle.mway(np.zeros(0, np.uint32), np.ones(3, np.uint32)) == 0
+
+
+def test_join_relation_brute_force(oracle):
+    rng = np.random.default_rng(4)
+    for n_r, n_s, hi in ((5, 9, 3), (200, 300, 50), (1000, 1000, 2000)):
+        rk = rng.integers(0, hi, n_r).astype(np.uint32)
+        rp = rng.integers(0, 1 << 32, n_r, dtype=np.uint64).astype(np.uint32)
+        sk = rng.integers(0, hi, n_s).astype(np.uint32)
+        n, k, p = oracle.join_relation(rk, rp, sk)
+        ii, jj = np.nonzero(rk[:, None] == sk[None, :])
+        want = sorted(zip(rk[ii].tolist(), rp[ii].tolist()))
+        assert n == len(ii) and list(zip(k.tolist(), p.tolist())) == want
+
+
+@pytest.mark.parametrize("case", GOLDEN["three_way"], ids=lambda c: c["name"])
+def test_three_way_matches_reference(oracle, case):
+    # golden: the reference RHO's materialised (key, Rpayload) output fed back
+    # into the reference RHO as the probe side, as secure_joins.cpp:376-386
+    rk = oracle.gen_pk(11111, case["nR"])
+    sk = oracle.gen_fk(22222, case["nS"], case["nR"])
+    tk = (oracle.gen_fk(case["t_seed"], case["nT"], case["nR"]) if case["T"] == "fk"
+          else oracle.gen_zipf(case["t_seed"], case["nT"], case["nR"], 1.0))
+    rp = np.arange(case["nR"], dtype=np.uint32)
+    tp = np.arange(case["nT"], dtype=np.uint32)
+    n1, k2, p2 = oracle.three_way(rk, rp, sk, tk, tp)
+    assert n1 == case["join1"] and len(k2) == case["matches"]
+    w = (k2.astype(np.uint64) << np.uint64(32)) | p2.astype(np.uint64)
+    assert hashlib.sha256(np.sort(w).astype("<u8").tobytes()).hexdigest() == \
+        case["sorted_out2_sha256"]

@@ -86,10 +86,12 @@ typedef struct tbj_options {
                                    radix fallback; 0 = the LDS capacity */
     int32_t learned_samples;    /* keys sampled to fit the CDF model; 0 = auto */
     int32_t s_chunk;            /* S tuples per join task; 0 = auto */
-    int32_t sampled_regions;    /* 1: size second-pass regions from a 1/16 sample
-                                   taken by the first histogram instead of exact
-                                   per-tile histograms (falls back to exact on
-                                   overflow) */
+    int32_t sampled_regions;    /* two-pass plans: size the output regions of
+                                   both passes from a 1/16 sample of the input
+                                   instead of per-tile histograms, so that no
+                                   pass reads its input twice (redone exactly
+                                   when a region overflows).  0 = auto (for
+                                   |R| + |S| >= 2^24), 1 = always, -1 = never */
     int32_t profile;            /* 1: record per-phase device times in phase_ms */
     int32_t output;             /* enum tbj_output */
     int32_t reserved[2];

@@ -243,7 +243,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
                     d.ntiles = (d.n + kTile - 1) / kTile;
                     d.first_tile = tiles;
                     d.ngroups = seg_groups(tiles, d.ntiles);
-                    d.pad = 0;
+                    d.aux = 0;
                     d.hist_base = groups * g.F1;
                     tiles += d.ntiles;
                     groups += d.ngroups;

@@ -46,27 +46,54 @@ __global__ void k_part_bounds(const uint32_t* __restrict__ scanned,
     bounds[i] = make_uint2(st, en);
 }
 
-__device__ __forceinline__ uint32_t tasks_of(uint2 r, uint2 s, uint32_t cap, uint32_t chunk) {
-    const uint32_t nr = r.y - r.x, ns = s.y - s.x;
-    if (!nr || !ns) return 0;
-    return ((nr + cap - 1) / cap) * ((ns + chunk - 1) / chunk);
+// Builds (or chunks) of one partition side: its npieces ranges from `b`,
+// each cut into pieces of at most `cap` tuples; *len = its tuple count.
+__device__ __forceinline__ uint32_t side_chunks(const uint2* __restrict__ b, uint32_t npieces,
+                                                uint32_t cap, uint32_t* len) {
+    uint32_t c = 0, n = 0;
+    for (uint32_t x = 0; x < npieces; x++) {
+        const uint2 r = b[x];
+        const uint32_t l = r.y - r.x;
+        c += (l + cap - 1) / cap;
+        n += l;
+    }
+    *len = n;
+    return c;
+}
+
+// Range of chunk k of a side (k < its chunk count).
+__device__ __forceinline__ void side_chunk(const uint2* __restrict__ b, uint32_t npieces,
+                                           uint32_t cap, uint32_t k, uint32_t* off,
+                                           uint32_t* len) {
+    for (uint32_t x = 0; x < npieces; x++) {
+        const uint2 r = b[x];
+        const uint32_t c = (r.y - r.x + cap - 1) / cap;
+        if (k < c) {
+            *off = r.x + k * cap;
+            *len = min(cap, r.y - *off);
+            return;
+        }
+        k -= c;
+    }
+    *off = 0;
+    *len = 0;
 }
 
 // Per-partition task counts (task_cnt[nparts] is a zero sentinel so that the
-// exclusive scan leaves the total there).  stats[0] / stats[1] = largest R /
-// S partition.
+// exclusive scan leaves the total there): builds times chunks, zero when a
+// side is empty.  stats[0] / stats[1] = largest R / S partition.
 __global__ __launch_bounds__(256) void k_task_counts(const uint2* __restrict__ bounds,
-                                                     uint32_t nparts, uint32_t cap,
-                                                     uint32_t chunk,
+                                                     uint32_t nparts, uint32_t npieces,
+                                                     uint32_t cap, uint32_t chunk,
                                                      uint32_t* __restrict__ task_cnt,
                                                      uint32_t* __restrict__ stats) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t rn = 0, sn = 0;
     if (p < nparts) {
-        const uint2 r = bounds[p], s = bounds[nparts + p];
-        task_cnt[p] = tasks_of(r, s, cap, chunk);
-        rn = r.y - r.x;
-        sn = s.y - s.x;
+        const uint32_t rc = side_chunks(bounds + (uint64_t)p * npieces, npieces, cap, &rn);
+        const uint32_t sc =
+            side_chunks(bounds + ((uint64_t)nparts + p) * npieces, npieces, chunk, &sn);
+        task_cnt[p] = rc * sc;
     } else if (p == nparts) {
         task_cnt[p] = 0;
     }
@@ -94,18 +121,18 @@ __global__ __launch_bounds__(256) void k_task_counts(const uint2* __restrict__ b
     }
 }
 
-// Task t lies in the partition p with task_off[p] <= t < task_off[p + 1],
-// as R chunk li / nsc and S chunk li % nsc of it (li = t - task_off[p], nsc =
-// S chunks of p).  A partition's tasks are contiguous in partition order.
+// Task t of partition p (task_off[p] <= t < task_off[p + 1]): build li / nsc
+// and chunk li % nsc of it (li = t - task_off[p], nsc = S chunks of p).  A
+// partition's tasks are contiguous in partition order.
 __device__ __forceinline__ JoinTask task_in_part(const TaskSrc& ts, uint32_t p, uint32_t t) {
-    const uint2 r = ts.bounds[p], s = ts.bounds[ts.nparts + p];
-    const uint32_t nsc = (s.y - s.x + ts.chunk - 1) / ts.chunk;
+    const uint2* rb = ts.bounds + (uint64_t)p * ts.npieces;
+    const uint2* sb = ts.bounds + ((uint64_t)ts.nparts + p) * ts.npieces;
+    uint32_t sn;
+    const uint32_t nsc = side_chunks(sb, ts.npieces, ts.chunk, &sn);
     const uint32_t li = t - ts.task_off[p];
     JoinTask tk;
-    tk.r_off = r.x + (li / nsc) * ts.cap;
-    tk.r_len = min(ts.cap, r.y - tk.r_off);
-    tk.s_off = s.x + (li % nsc) * ts.chunk;
-    tk.s_len = min(ts.chunk, s.y - tk.s_off);
+    side_chunk(rb, ts.npieces, ts.cap, li / nsc, &tk.r_off, &tk.r_len);
+    side_chunk(sb, ts.npieces, ts.chunk, li % nsc, &tk.s_off, &tk.s_len);
     return tk;
 }
 
@@ -780,9 +807,9 @@ void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* 
 
 void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
-                        uint32_t* ntasks_dev, uint32_t* stats) {
+                        uint32_t* ntasks_dev, uint32_t* stats, uint32_t npieces) {
     hipLaunchKernelGGL(k_task_counts, dim3((nparts + 1 + 255) / 256), dim3(256), 0, st, bounds,
-                       nparts, cap, s_chunk, task_off, stats);
+                       nparts, npieces, cap, s_chunk, task_off, stats);
     launch_scan(st, task_off, (uint64_t)nparts + 1, partial, ntasks_dev);
 }
 

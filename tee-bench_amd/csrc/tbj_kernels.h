@@ -39,11 +39,9 @@ enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 // Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev); tile t's
 // bin counts go to hist[t * nbins, (t + 1) * nbins) (u16: a tile holds at
 // most kTile tuples).
-// fine (optional): sampled (bin, bin2) histogram per segment, see k_hist.
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
-                 const BinSpec& bs, uint32_t nbins, uint32_t* fine = nullptr,
-                 const BinSpec* bs2 = nullptr, uint32_t nbins2 = 1);
+                 const BinSpec& bs, uint32_t nbins);
 // Per-tile counts (u16 rows of nbins) -> per-tile output offsets (u32 rows
 // in offs), for the tiles below *ntiles_dev (at most max_tiles).  gsum (glen
 // words, the last one zero and glen - 1 >= nbins * the groups of all
@@ -54,14 +52,28 @@ void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* cou
                          uint32_t* offs, const TileDesc* tiles, const SegDesc* segs,
                          const uint32_t* ntiles_dev, uint32_t nbins, uint32_t* gsum,
                          uint64_t glen, uint32_t* partial, uint32_t* grand_total);
-// offs: the per-tile offsets from launch_tile_offsets.  last_pass: the
+// Sample-sized regions of a claiming scatter: tile t's run of bin i goes to
+// region r = cb0 + tiles[t].aux * cmul + i * cstride + xmul * (the XCD eighth
+// the tile runs in), which owns
+// [starts[r], starts[r + 1]) and fills from cursors[r] (zeroed beforehand);
+// a run that does not fit sets *overflow.
+struct ClaimSpec {
+    const uint32_t* starts;
+    uint32_t* cursors;
+    uint32_t* overflow;
+    uint32_t cb0, cmul, cstride;
+    uint32_t xmul;  // r += xmul * the XCD eighth the tile runs in
+};
+// offs: the per-tile offsets from launch_tile_offsets (claims == NULL), or
+// sample-sized regions claimed per (tile, bin) (claims).  last_pass: the
 // plan's last pass, whose segments are narrow key ranges where one hot key can
 // fill most of a wave: lanes sharing a bin then take their ranks from one LDS
 // atomic (in a first pass of two, whose tiles mix the whole key range, the
 // plain per-tuple atomics are cheaper).
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass);
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
+                    const ClaimSpec* claims = nullptr);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
@@ -83,16 +95,47 @@ void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nse
                 uint32_t bits, const uint32_t* minmax, CdfModel* model);
 void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
-// Sample-sized second pass: caps (nseg2 * nbins2 + 1 words), then an
-// exclusive scan of caps gives region starts; cursors start at zero.
-void launch_region_caps(hipStream_t st, const uint32_t* fine, const SegDesc* segs2,
-                        uint32_t nseg2, uint32_t nbins2, uint32_t* caps);
-void launch_scatter_est(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
-                        const TileDesc* tiles, const uint32_t* ntiles_dev, const uint32_t* starts,
-                        uint32_t* cursors, uint32_t* overflow, const BinSpec& bs,
-                        uint32_t nbins);
+// Sample-sized passes (no histogram read).  k_sample_rows: per first-pass
+// tile row the sampled counts of each first-pass bin (cnt1[pbase(rel) + row *
+// F1 + b1], pbase(S) = F1 * nrowR) and per relation the sampled (b1, b2)
+// counts (fine, 2 * F1 * F2 words, zeroed; F1 * F2 <= 65536).
+void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
+                        uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
+                        const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
+                        uint32_t* cnt1, uint32_t* fine);
+// caps[i] from 16 * cnt[i] for i < n, caps[n] = 0 (scan to region starts).
+void launch_row_caps(hipStream_t st, const uint32_t* cnt, uint32_t n, uint32_t* caps);
+// Exact first-pass partition sizes (2 * F1 words) from the filled regions;
+// their sum is added to *total.
+void launch_part_sizes(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                       uint32_t F1, uint32_t nrowR, uint32_t nrowS, uint32_t* npart,
+                       uint32_t* total);
+// Estimated largest first-pass R partition (from the row counts) into
+// stats[0] (zeroed).
+void launch_est_part_max(hipStream_t st, const uint32_t* cnt1, uint32_t F1, uint32_t nrowR,
+                         uint32_t* stats);
+// Second-pass region caps, 8 per-XCD sub-regions per (partition, bin):
+// 8 * nseg2 * nbins2 + 1 words, from fine, npart and the partitions' tile
+// spans (first: scanned tile counts in partition order; ntiles2: the total).
+void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* npart,
+                        uint32_t nseg2, uint32_t nbins2, const uint32_t* first,
+                        const uint32_t* ntiles2, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
+                        uint32_t* caps);
+// Second-pass tiles over the n filled first-pass regions, visited in
+// partition order: ntiles (n + 1 words), scanned into first tiles, then the
+// descriptors.
+void launch_region_tiles(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                         uint32_t n, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
+                         uint32_t* ntiles);
+void launch_region_tile_descs(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                              const uint32_t* first, uint32_t n, uint32_t F1, uint32_t nrowR,
+                              uint32_t nrowS, TileDesc* tiles);
 void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                        uint32_t n, uint2* bounds);
+// Join bounds (8 pieces per partition side) of the second pass's per-XCD
+// sub-regions, laid out ((rel * F1 + b1) * 8 + x) * F2 + b2.
+void launch_bounds_x8(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                      uint32_t F1, uint32_t F2, uint2* bounds);
 void launch_make_tiles1(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                         uint32_t nbins, uint32_t* ntiles_dev);
 // Zeroes hist[mult * (*count_dev) .. n); max_tail bounds the zeroed length.
@@ -106,10 +149,10 @@ void launch_part_bounds(hipStream_t st, const uint32_t* scanned, const SegDesc* 
                         uint32_t nR, uint32_t nS, uint2* bounds);
 // Task offsets (task_off needs nparts + 1 words, partial scan_partials(nparts + 1)),
 // their total in *ntasks_dev, partition maxima folded into stats[0..1]
-// (stats must start at zero).
+// (stats must start at zero).  npieces: ranges per partition side in bounds.
 void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, uint32_t cap,
                         uint32_t s_chunk, uint32_t* task_off, uint32_t* partial,
-                        uint32_t* ntasks_dev, uint32_t* stats);
+                        uint32_t* ntasks_dev, uint32_t* stats, uint32_t npieces = 1);
 // Lists the first ts.task_cap tasks (the grid covers task_cap; the count
 // itself stays on the device).
 void launch_fill_tasks(hipStream_t st, const TaskSrc& ts);

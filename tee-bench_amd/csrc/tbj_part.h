@@ -23,7 +23,8 @@ struct SegDesc {
     uint32_t ntiles;      // tiles of the segment
     uint32_t first_tile;  // global index of its first tile
     uint32_t ngroups;
-    uint32_t pad;
+    uint32_t aux;         // sample-sized second pass: the segment's first-pass
+                          // partition (relation * F1 + bin); 0 otherwise
 };
 
 constexpr uint32_t kGroupTiles = 32;
@@ -37,17 +38,52 @@ struct TileDesc {
     uint32_t in_off;      // first input tuple of the tile
     uint32_t len;         // tuples in the tile
     uint32_t seg;         // owning segment
-    uint32_t pad;
+    uint32_t aux;         // sample-sized passes: first pass, the tile's row
+                          // (tile_row); second pass, its segment's aux
 };
+
+// Sample-sized passes.  A first-pass launch hands its n tiles to the 8 XCDs
+// in contiguous eighths (xcd_tile); each eighth is cut into rows of
+// kRowTiles tiles, and every (bin, row) of a relation owns its own output
+// region, so that a region is written by one XCD.  A row's regions average
+// kRowTiles * kTile / F1 tuples (8 tiles at 256 bins), so that the second
+// pass, which reads them region by region, sees mostly full tiles.  The
+// sample is counted in chunks of kSampleTiles tiles of a row (the 16-bit
+// LDS counters of one chunk cannot overflow).
+constexpr uint32_t kRowTiles = 2048;
+constexpr uint32_t kSampleTiles = 224;
+constexpr uint32_t kChunksPerRow = (kRowTiles + kSampleTiles - 1) / kSampleTiles;
+
+inline __host__ __device__ uint32_t eighth_start(uint32_t x, uint32_t n) {
+    const uint32_t q = n / 8, r = n % 8;
+    return x * q + (x < r ? x : r);
+}
+
+inline __host__ __device__ uint32_t rows_per_eighth(uint32_t n) {
+    const uint32_t longest = n / 8 + (n % 8 ? 1 : 0);
+    return (longest + kRowTiles - 1) / kRowTiles;
+}
+
+// Row of tile u of a launch of n tiles: eighth x (the first n % 8 eighths hold
+// n / 8 + 1 tiles, the others n / 8), then the kRowTiles-tile row inside it.
+inline __host__ __device__ uint32_t tile_row(uint32_t u, uint32_t n) {
+    const uint32_t q = n / 8, r = n % 8;
+    const uint32_t x = u < r * (q + 1) ? u / (q + 1) : r + (u - r * (q + 1)) / q;
+    return x * rows_per_eighth(n) + (u - eighth_start(x, n)) / kRowTiles;
+}
 
 // One build/probe task: an R range (at most the LDS capacity) against an S range.
 struct JoinTask {
     uint32_t r_off, r_len, s_off, s_len;
 };
 
-// Where the join kernels find their tasks: partition p (bounds[p] = R range,
-// bounds[nparts + p] = S range) holds tasks [task_off[p], task_off[p + 1]),
-// *ntasks_dev in all; the first task_cap of them are also listed in tasks.
+// Where the join kernels find their tasks: side rel (0 = R, 1 = S) of
+// partition p is the npieces ranges bounds[(rel * nparts + p) * npieces + x]
+// (one range, or the per-XCD pieces of sample-sized regions); its R ranges
+// are cut into builds of at most cap tuples and its S ranges into chunks of
+// at most chunk tuples, and every (build, chunk) pair is a task.  Partition
+// p holds tasks [task_off[p], task_off[p + 1]), *ntasks_dev in all; the first
+// task_cap of them are also listed in tasks.
 struct TaskSrc {
     JoinTask* tasks;
     const uint32_t* task_off;
@@ -57,6 +93,7 @@ struct TaskSrc {
     uint32_t nparts;
     uint32_t cap;       // R tuples per build
     uint32_t chunk;     // S tuples per task
+    uint32_t npieces;   // ranges per partition side (1 or 8)
 };
 
 // Piecewise-linear key-CDF model (learned range partitioner).  Knot i

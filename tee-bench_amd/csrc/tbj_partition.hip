@@ -84,14 +84,7 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
 
 // ---------------------------------------------------------------------------
 // Histogram: one workgroup per tile, LDS-privatised counts, written as the
-// tile's row of u16 counts.  With fine != NULL, a pseudo-random 1/16 of the
-// tuples (by their index in the relation) also count into
-// fine[seg * nbins * nbins2 + bin * nbins2 + bin2(key)]: the sample from
-// which the second pass sizes its sub-partitions.
-
-__device__ __forceinline__ bool sampled16(uint32_t idx) {
-    return ((idx * 2654435761u) >> 28) == 0;
-}
+// tile's row of u16 counts.
 
 // Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
 // ... below tile_end and *ntiles_dev, loading the partition function (the
@@ -102,14 +95,11 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
                                                      const uint32_t* __restrict__ ntiles_dev,
                                                      uint32_t tile_base, uint32_t tile_end,
                                                      uint16_t* __restrict__ hist, BinSpec bs,
-                                                     uint32_t nbins, uint32_t* __restrict__ fine,
-                                                     BinSpec bs2, uint32_t nbins2) {
+                                                     uint32_t nbins) {
     __shared__ uint32_t cnt[kMaxBins];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
     BinFn bin = make_binfn(bs, &lm);
-    BinFn bin2 = bin;
-    bin2.bs = bs2;
     const uint32_t end = min(tile_end, *ntiles_dev);
     for (uint32_t t = tile_base + blockIdx.x; t < end; t += gridDim.x) {
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
@@ -126,12 +116,7 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
             const uint32_t j = k * kPartBlock + threadIdx.x;
             if (j < td.len) {
                 const uint32_t b = bin(key[k]);
-                if (b != TBJ_DROP) {
-                    atomicAdd(&cnt[b], 1u);
-                    if (fine && sampled16(td.in_off + j))
-                        atomicAdd(&fine[((uint64_t)td.seg * nbins + b) * nbins2 + bin2(key[k])],
-                                  1u);
-                }
+                if (b != TBJ_DROP) atomicAdd(&cnt[b], 1u);
             }
         }
         __syncthreads();
@@ -279,15 +264,35 @@ __host__ __device__ constexpr size_t scatter_model_bytes(bool learned) {
     return learned ? (sizeof(LdsModel) + 15) / 16 * 16 : 0;
 }
 
-// offs: row t = the output offset of each bin's run of tile t.
-template <bool GMODEL, bool AGG>
+// Where a claiming scatter puts tile t's run of bin i: region r = cb0 +
+// tiles[t].aux * cmul + i * cstride owns [starts[r], starts[r + 1]); the run
+// takes its place with one atomicAdd on cursors[r].  A run that would pass
+// the region's end sets *overflow and writes only what fits (the caller then
+// redoes the join with exact histograms).
+struct Claims {
+    const uint32_t* starts;
+    uint32_t* cursors;
+    uint32_t* overflow;
+    uint32_t cb0, cmul, cstride;
+    uint32_t xmul;  // r += xmul * the tile's XCD eighth (block index mod 8)
+};
+
+// CLAIM = false: offs row t = the output offset of each bin's run of tile t.
+// CLAIM = true: the runs claim space in sample-sized regions (cl).
+// The global words a tile needs per bin (its offset row, or its regions'
+// bounds) are loaded together with the tile, and the claims are issued
+// before the LDS scan and staging and consumed after them, so that neither
+// round trip sits between the tile's load and its writes.
+constexpr int kScatterBinsPerThread = kMaxBins / kPartBlock;
+
+template <bool GMODEL, bool AGG, bool CLAIM>
 __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles_dev,
                                                         uint32_t tile_base, uint32_t tile_end,
                                                         const uint32_t* __restrict__ offs,
-                                                        BinSpec bs, uint32_t nbins) {
+                                                        Claims cl, BinSpec bs, uint32_t nbins) {
     __shared__ uint2 stage[kTile];
     __shared__ uint16_t sbin[kTile];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
@@ -296,6 +301,7 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
     uint32_t* loff = reinterpret_cast<uint32_t*>(
         dyn_lds + scatter_model_bytes(!GMODEL && bs.kind == kBinLearned));
     uint32_t* goff = loff + nbins;
+    uint32_t* glim = goff + nbins;  // CLAIM only
     BinFn bin = make_binfn<GMODEL>(bs, &lm);
     // the launch's tiles that exist (the grid may be an upper bound)
     const uint32_t end = min(tile_end, *ntiles_dev);
@@ -303,7 +309,6 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
     if (blockIdx.x < n) {
         const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
-        __syncthreads();
         const TileDesc td = tiles[t];
         uint2 v[kPartItems];
         uint32_t b[kPartItems];
@@ -314,6 +319,26 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
             b[k] = TBJ_DROP;
             if (j < td.len) v[k] = in[td.in_off + j];
         }
+        // per bin of this thread: the tile's offset (exact) or its region's
+        // start and end (claims)
+        const uint32_t r0 =
+            CLAIM ? cl.cb0 + td.aux * cl.cmul + cl.xmul * (blockIdx.x % kNumXcd) : 0;
+        uint32_t w0[kScatterBinsPerThread], w1[kScatterBinsPerThread];
+#pragma unroll
+        for (int q = 0; q < kScatterBinsPerThread; q++) {
+            const uint32_t i = threadIdx.x + q * kPartBlock;
+            w0[q] = w1[q] = 0;
+            if (i < nbins) {
+                if (CLAIM) {
+                    const uint32_t r = r0 + i * cl.cstride;
+                    w0[q] = cl.starts[r];
+                    w1[q] = cl.starts[r + 1];
+                } else {
+                    w0[q] = offs[(uint64_t)t * nbins + i];
+                }
+            }
+        }
+        __syncthreads();
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
             const uint32_t j = k * kPartBlock + threadIdx.x;
@@ -324,9 +349,17 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
                 rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
         __syncthreads();
+        // claims: one atomic per non-empty bin, issued now, awaited after the
+        // scan and the staging
+        uint32_t cnt[kScatterBinsPerThread], base[kScatterBinsPerThread];
+#pragma unroll
+        for (int q = 0; q < kScatterBinsPerThread; q++) {
+            const uint32_t i = threadIdx.x + q * kPartBlock;
+            cnt[q] = i < nbins ? loff[i] : 0;
+            base[q] = 0;
+            if (CLAIM && cnt[q]) base[q] = atomicAdd(&cl.cursors[r0 + i * cl.cstride], cnt[q]);
+        }
         const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
-        for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock)
-            goff[i] = offs[(uint64_t)t * nbins + i] - loff[i];
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
             if (b[k] != TBJ_DROP) {
@@ -335,27 +368,175 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
                 sbin[slot] = (uint16_t)b[k];
             }
         }
+#pragma unroll
+        for (int q = 0; q < kScatterBinsPerThread; q++) {
+            const uint32_t i = threadIdx.x + q * kPartBlock;
+            if (i < nbins) {
+                if (CLAIM) {
+                    const uint32_t cap = w1[q] - w0[q];
+                    if (cnt[q] && base[q] + cnt[q] > cap) atomicOr(cl.overflow, 1u);
+                    goff[i] = w0[q] + base[q] - loff[i];
+                    glim[i] = w0[q] + cap;
+                } else {
+                    goff[i] = w0[q] - loff[i];
+                }
+            }
+        }
         __syncthreads();
         for (uint32_t j = threadIdx.x; j < nvalid; j += kPartBlock) {
             const uint32_t bb = sbin[j];
-            out[goff[bb] + j] = stage[j];
+            const uint32_t pos = goff[bb] + j;
+            if (!CLAIM || pos < glim[bb]) out[pos] = stage[j];
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Second pass sized from the sample instead of per-tile histograms.  Every
-// sub-partition (segment s = rel * nbins1 + p1, bin p2) owns the region
-// [start[i], start[i+1]) of the output, i = s * nbins2 + p2; tiles claim
-// space in it with one atomicAdd per (tile, bin).  A region that turns out
-// too small sets *overflow; the caller then redoes the pass with exact
-// histograms.
+// Sample-sized passes: no histogram read.  One kernel reads 1/16 of the
+// input's 16-tuple groups (exactly one group in every 16, at a hashed
+// position) and counts, per first-pass tile row, the sampled tuples of each
+// first-pass bin (sizing that row's regions of the first pass) and, per
+// relation, the sampled tuples of each (first-pass bin, second-pass bin)
+// pair (sizing the second pass's regions inside each first-pass partition).
 
-// caps[i] for every sub-partition (one workgroup per segment); caps[n] = 0
-// is the scan sentinel.
+__device__ __forceinline__ uint32_t sampled_group(uint32_t block) {
+    return (block * 2654435761u) >> 28;  // which of the block's 16 groups
+}
+
+// One workgroup per sample chunk: chunk c of tile row `row` (rows of R first,
+// then S; kChunksPerRow chunks per row).  cnt1 (zeroed) accumulates the
+// row's sampled counts per first-pass bin at cnt1[pbase[rel] + row * F1 +
+// b1] (the first-pass regions are laid out row by row, so that a tile's
+// claims touch consecutive words); fine (zeroed, 2 * F1 * F2 words)
+// accumulates (rel, b1, b2) counts.  A chunk's (b1, b2) counts live in LDS as
+// 16-bit halves of 32-bit words: a chunk holds at most kSampleTiles * kTile /
+// 16 < 2^16 sampled tuples.
+constexpr uint32_t kFineLds = 1u << 16;  // F1 * F2 at most
+constexpr int kSampleBlock = 512;
+
+__global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
+    const uint2* __restrict__ R, uint32_t nR, const uint2* __restrict__ S, uint32_t nS,
+    uint32_t nrowR, uint32_t nrowS, BinSpec bsr, BinSpec bss, BinSpec bs2, uint32_t F1,
+    uint32_t F2, uint32_t* __restrict__ cnt1, uint32_t* __restrict__ fine) {
+    __shared__ uint32_t h[kFineLds / 2];
+    __shared__ uint32_t c1[kMaxBins];
+    const uint32_t rowg = blockIdx.x / kChunksPerRow, chunk = blockIdx.x % kChunksPerRow;
+    const uint32_t rel = rowg < nrowR ? 0 : 1;
+    const uint32_t row = rel ? rowg - nrowR : rowg;
+    const uint2* in = rel ? S : R;
+    const uint32_t n = rel ? nS : nR;
+    const uint32_t ntiles = (n + kTile - 1) / kTile;
+    const uint32_t rpe = rows_per_eighth(ntiles);
+    const uint32_t x = row / rpe, c = row % rpe;
+    const uint32_t e0 = eighth_start(x, ntiles), e1 = eighth_start(x + 1, ntiles);
+    const uint32_t r0 = min(e0 + c * kRowTiles, e1), r1 = min(r0 + kRowTiles, e1);
+    const uint32_t t0 = min(r0 + chunk * kSampleTiles, r1), t1 = min(t0 + kSampleTiles, r1);
+    if (t0 == t1) return;  // whole workgroup
+    const uint32_t nf = F1 * F2;
+    for (uint32_t i = threadIdx.x; i < nf / 2 + 1 && i < kFineLds / 2; i += kSampleBlock) h[i] = 0;
+    for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock) c1[i] = 0;
+    __syncthreads();
+    BinFn b1f = make_binfn<true>(rel ? bss : bsr, nullptr);
+    BinFn b2f = make_binfn<true>(bs2, nullptr);
+    const uint32_t lo = t0 * kTile, hi = min(t1 * kTile, n);
+    // blocks of 16 groups = 256 tuples; one thread per block
+    const uint32_t blo = lo / 256, bhi = (hi + 255) / 256;
+    for (uint32_t blk = blo + threadIdx.x; blk < bhi; blk += kSampleBlock) {
+        const uint32_t first = blk * 256 + sampled_group(blk) * 16;
+#pragma unroll 4
+        for (uint32_t k = 0; k < 16; k++) {
+            const uint32_t i = first + k;
+            if (i >= lo && i < hi) {
+                const uint32_t key = in[i].x;
+                const uint32_t b1 = b1f(key), b2 = b2f(key);
+                if (b1 != TBJ_DROP && b2 != TBJ_DROP) {
+                    const uint32_t f = b1 * F2 + b2;
+                    atomicAdd(&h[f >> 1], 1u << ((f & 1) * 16));
+                    atomicAdd(&c1[b1], 1u);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t pbase = rel ? F1 * nrowR : 0;
+    for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock)
+        if (c1[i]) atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
+    uint32_t* fr = fine + (uint64_t)rel * nf;
+    for (uint32_t f = threadIdx.x; f < nf; f += kSampleBlock) {
+        const uint32_t v = (h[f >> 1] >> ((f & 1) * 16)) & 0xFFFFu;
+        if (v) atomicAdd(&fr[f], v);
+    }
+}
+
+// Region capacity (tuples) from an estimate e = 16 * sampled count: room for
+// 10 % over e, 12 standard deviations of a 1/16 sample of independent tuples
+// (48 sqrt(e)), and 640 more -- a region of 640 tuples shows no sampled tuple
+// with probability (15/16)^640 < 1e-17, so an empty sample is no evidence of
+// an empty region.
+constexpr double kCapFloor = 640.0;
+__device__ __forceinline__ uint32_t region_cap(double e) {
+    return (uint32_t)min(4.0e9, e * 1.1 + 48.0 * sqrt(e) + kCapFloor);
+}
+
+// caps[i] = region_cap(16 * cnt[i]) for the n first-pass regions; caps[n] =
+// 0 (the scan's sentinel).
+__global__ void k_row_caps(const uint32_t* __restrict__ cnt, uint32_t n,
+                           uint32_t* __restrict__ caps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) caps[i] = region_cap(16.0 * cnt[i]);
+    else if (i == n) caps[i] = 0;
+}
+
+// Exact size of each first-pass partition p = rel * F1 + b1 (the sum of its
+// rows' filled regions): one thread per partition; *total (zeroed) receives
+// the sum over all partitions.
+__global__ void k_part_sizes(const uint32_t* __restrict__ starts,
+                             const uint32_t* __restrict__ cursors, uint32_t F1, uint32_t nrowR,
+                             uint32_t nrowS, uint32_t* __restrict__ npart,
+                             uint32_t* __restrict__ total) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= 2 * F1) return;
+    const uint32_t rel = p / F1, b1 = p % F1;
+    const uint32_t nrow = rel ? nrowS : nrowR;
+    const uint32_t r0 = (rel ? F1 * nrowR : 0) + b1;
+    uint32_t tot = 0;
+    for (uint32_t k = 0; k < nrow; k++) {
+        const uint32_t r = r0 + k * F1;
+        tot += min(cursors[r], starts[r + 1] - starts[r]);
+    }
+    npart[p] = tot;
+    if (tot) atomicAdd(total, tot);
+}
+
+// Estimated largest first-pass R partition (16 times its sampled count over
+// the rows) into stats[0] (zeroed): one thread per R bin.
+__global__ void k_est_part_max(const uint32_t* __restrict__ cnt1, uint32_t F1, uint32_t nrowR,
+                               uint32_t* __restrict__ stats) {
+    const uint32_t b1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b1 >= F1) return;
+    uint32_t tot = 0;
+    for (uint32_t r = 0; r < nrowR; r++) tot += cnt1[r * F1 + b1];
+    atomicMax(&stats[0], 16 * tot);
+}
+
+// Second-pass region caps inside first-pass partition s (one workgroup per
+// partition): the ratio estimate of bin b2's share of the exact partition
+// size from the sampled (b1, b2) counts; bins with few samples are shrunk
+// toward an even split by 8 pseudo-samples, the larger estimate wins.  Each
+// region is split into 8 sub-regions, one per XCD eighth of the second pass's
+// tiles (so that the claims of a partition whose tiles span several XCDs --
+// a hot key's -- are spread over several counters and never cross an XCD),
+// each sized by the share of the partition's tiles in that eighth: region
+// (s, b2) sub-region x is caps[(s * 8 + x) * nbins2 + b2] (a tile's claims
+// then hit consecutive words).  first: the first tile of each first-pass
+// region in partition order (scanned), *ntiles2 the tile count.
+// caps[8 * nsub] = 0 is the scan's sentinel.
 __global__ __launch_bounds__(256) void k_region_caps(const uint32_t* __restrict__ fine,
-                                                     const SegDesc* __restrict__ segs2,
+                                                     const uint32_t* __restrict__ npart,
                                                      uint32_t nbins2,
+                                                     const uint32_t* __restrict__ first,
+                                                     const uint32_t* __restrict__ ntiles2,
+                                                     uint32_t F1, uint32_t nrowR, uint32_t nrowS,
                                                      uint32_t* __restrict__ caps,
                                                      uint32_t nsub) {
     __shared__ uint32_t red[256 / TBJ_WAVE];
@@ -368,89 +549,93 @@ __global__ __launch_bounds__(256) void k_region_caps(const uint32_t* __restrict_
     __syncthreads();
     tot = 0;
     for (int w = 0; w < 256 / TBJ_WAVE; w++) tot += red[w];
-    const double n = (double)segs2[s].n;
-    for (uint32_t b = threadIdx.x; b < nbins2; b += 256) {
-        uint32_t cap = 0;
-        if (n > 0) {
-            // ratio estimate of the bin's share of the exact segment size;
-            // bins with few samples are shrunk toward an even split by 8
-            // pseudo-samples, the larger estimate wins; room for 6 standard
-            // deviations of a 1/16 sample's estimate
-            const double raw = tot ? n * (double)f[b] / (double)tot : 0.0;
-            const double shrunk = n * ((double)f[b] + 8.0) / ((double)tot + 8.0 * nbins2);
-            const double e = raw > shrunk ? raw : shrunk;
-            cap = (uint32_t)min(n, e * 1.1 + 6.0 * sqrt(16.0 * e) + 64.0);
-        }
-        caps[(uint64_t)s * nbins2 + b] = cap;
+    // the partition's tiles [ft, et) and their share in each eighth
+    const uint32_t rel = s / F1, b1 = s % F1;
+    const uint32_t nrow = rel ? nrowS : nrowR;
+    const uint32_t pos0 = (rel ? F1 * nrowR : 0) + b1 * nrow;
+    const uint32_t ft = first[pos0], et = first[pos0 + nrow], T = *ntiles2;
+    double share[8];
+#pragma unroll
+    for (uint32_t x = 0; x < 8; x++) {
+        const uint32_t a = max(ft, eighth_start(x, T)), b = min(et, eighth_start(x + 1, T));
+        share[x] = et > ft && b > a ? (double)(b - a) / (double)(et - ft) : 0.0;
     }
-    if (s == 0 && threadIdx.x == 0) caps[nsub] = 0;
+    const double n = (double)npart[s];
+    for (uint32_t b = threadIdx.x; b < nbins2; b += 256) {
+        const double raw = tot ? n * (double)f[b] / (double)tot : 0.0;
+        const double shrunk = n * ((double)f[b] + 8.0) / ((double)tot + 8.0 * nbins2);
+        const double e = raw > shrunk ? raw : shrunk;
+#pragma unroll
+        for (uint32_t x = 0; x < 8; x++) {
+            const uint32_t cap = n > 0 && share[x] > 0 ? (uint32_t)min(n, (double)region_cap(e * share[x])) : 0;
+            caps[((uint64_t)s * 8 + x) * nbins2 + b] = cap;
+        }
+    }
+    if (s == 0 && threadIdx.x == 0) caps[8 * (uint64_t)nsub] = 0;
 }
 
-__global__ __launch_bounds__(kPartBlock) void k_scatter_est(
-    const uint2* __restrict__ in, uint2* __restrict__ out, const TileDesc* __restrict__ tiles,
-    const uint32_t* __restrict__ ntiles_dev, const uint32_t* __restrict__ starts,
-    uint32_t* __restrict__ cursors, uint32_t* __restrict__ overflow, BinSpec bs,
-    uint32_t nbins) {
-    const uint32_t t = blockIdx.x;
-    if (t >= *ntiles_dev) return;
-    __shared__ uint2 stage[kTile];
-    __shared__ uint16_t sbin[kTile];
-    __shared__ uint32_t loff[kMaxBins];
-    __shared__ uint32_t goff[kMaxBins];
-    __shared__ uint32_t glim[kMaxBins];
-    __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
-    extern __shared__ uint8_t dyn_lds[];
-    LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
-    BinFn bin = make_binfn(bs, &lm);
-    __syncthreads();
-    const TileDesc td = tiles[t];
-    uint2 v[kPartItems];
-    uint32_t b[kPartItems];
-    uint32_t rank[kPartItems];
-#pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        const uint32_t j = k * kPartBlock + threadIdx.x;
-        b[k] = TBJ_DROP;
-        if (j < td.len) v[k] = in[td.in_off + j];
+// Second-pass tiles over the first pass's filled regions.  Region i is
+// (rel, row, b1) in memory order; the second pass visits the regions in
+// partition order (rel, b1, row), so that consecutive tiles -- one XCD's
+// share -- fill the regions of one partition.  ntiles[pos] = region i's tile
+// count at its position pos in that order (scanned afterwards into first
+// tiles); ntiles[n] = 0.
+__device__ __forceinline__ uint32_t region_order(uint32_t i, uint32_t F1, uint32_t nrowR,
+                                                 uint32_t nrowS, uint32_t* part) {
+    const uint32_t nR1 = F1 * nrowR;
+    const uint32_t rel = i < nR1 ? 0 : 1, j = rel ? i - nR1 : i;
+    const uint32_t nrow = rel ? nrowS : nrowR;
+    const uint32_t row = j / F1, b1 = j % F1;
+    *part = rel * F1 + b1;
+    return (rel ? nR1 : 0) + b1 * nrow + row;
+}
+
+__global__ void k_region_tiles(const uint32_t* __restrict__ starts,
+                               const uint32_t* __restrict__ cursors, uint32_t n, uint32_t F1,
+                               uint32_t nrowR, uint32_t nrowS, uint32_t* __restrict__ ntiles) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint32_t len = min(cursors[i], starts[i + 1] - starts[i]);
+        uint32_t part;
+        ntiles[region_order(i, F1, nrowR, nrowS, &part)] = (len + kTile - 1) / kTile;
+    } else if (i == n) {
+        ntiles[i] = 0;
     }
-#pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        const uint32_t j = k * kPartBlock + threadIdx.x;
-        if (j < td.len) b[k] = bin(v[k].x);
-        rank[k] = agg_inc(loff, b[k]);
+}
+
+// One thread per region: its tiles (from first, in partition order), each
+// tagged with the region's first-pass partition rel * F1 + b1.
+__global__ void k_region_tile_descs(const uint32_t* __restrict__ starts,
+                                    const uint32_t* __restrict__ cursors,
+                                    const uint32_t* __restrict__ first, uint32_t n, uint32_t F1,
+                                    uint32_t nrowR, uint32_t nrowS,
+                                    TileDesc* __restrict__ tiles) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t st = starts[i], len = min(cursors[i], starts[i + 1] - st);
+    uint32_t part;
+    uint32_t t = first[region_order(i, F1, nrowR, nrowS, &part)];
+    for (uint32_t k = 0; k < len; k += kTile, t++) {
+        TileDesc d;
+        d.in_off = st + k;
+        d.len = min((uint32_t)kTile, len - k);
+        d.seg = i;
+        d.aux = part;
+        tiles[t] = d;
     }
-    __syncthreads();
-    // claim space: one atomic per non-empty bin, before loff becomes offsets
-    const uint64_t sub0 = (uint64_t)td.seg * nbins;
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) {
-        const uint32_t c = loff[i];
-        uint32_t base = 0;
-        const uint32_t st = starts[sub0 + i], cap = starts[sub0 + i + 1] - st;
-        if (c) {
-            base = atomicAdd(&cursors[sub0 + i], c);
-            if (base + c > cap) atomicOr(overflow, 1u);
-        }
-        goff[i] = st + base;
-        glim[i] = st + cap;
-    }
-    __syncthreads();
-    const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) goff[i] -= loff[i];
-#pragma unroll
-    for (uint32_t k = 0; k < kPartItems; k++) {
-        if (b[k] != TBJ_DROP) {
-            const uint32_t slot = loff[b[k]] + rank[k];
-            stage[slot] = v[k];
-            sbin[slot] = (uint16_t)b[k];
-        }
-    }
-    __syncthreads();
-    for (uint32_t j = threadIdx.x; j < nvalid; j += kPartBlock) {
-        const uint32_t bb = sbin[j];
-        const uint32_t pos = goff[bb] + j;
-        if (pos < glim[bb]) out[pos] = stage[j];
-    }
+}
+
+// Join bounds of the per-XCD sub-regions: sub-region ((rel * F1 + b1) * 8 +
+// x) * F2 + b2 is piece x of side rel of partition p = b1 * F2 + b2,
+// bounds[(rel * F1 * F2 + p) * 8 + x] = {start, start + min(cursor, cap)}.
+__global__ void k_bounds_x8(const uint32_t* __restrict__ starts,
+                            const uint32_t* __restrict__ cursors, uint32_t F1, uint32_t F2,
+                            uint2* __restrict__ bounds) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= 16 * F1 * F2) return;
+    const uint32_t b2 = r % F2, x = (r / F2) % 8, s = r / (8 * F2);  // s = rel * F1 + b1
+    const uint32_t st = starts[r], cap = starts[r + 1] - st;
+    bounds[((uint64_t)s * F2 + b2) * 8 + x] = make_uint2(st, st + min(cursors[r], cap));
 }
 
 // bounds[i] = {start, start + min(cursor, cap)} for the 2 * nparts partitions.
@@ -499,7 +684,7 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments(
             d.ntiles = nt;
             d.first_tile = ft;
             d.ngroups = ng;
-            d.pad = 0;
+            d.aux = 0;
             d.hist_base = (uint64_t)(gcarry_s + gex) * nbins2;
             segs2[s2] = d;
         }
@@ -539,7 +724,7 @@ __global__ void k_make_tiles2(const SegDesc* __restrict__ segs2, uint32_t nseg2,
     td.in_off = sd.in_off + k * kTile;
     td.len = min((uint32_t)kTile, sd.n - k * kTile);
     td.seg = lo;
-    td.pad = 0;
+    td.aux = 0;
     tiles2[t] = td;
 }
 
@@ -673,7 +858,7 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
         d.in_off = i * kTile;
         d.len = min((uint32_t)kTile, nR - i * kTile);
         d.seg = 0;
-        d.pad = 0;
+        d.aux = tile_row(i, tR);
         tiles[i] = d;
     } else if (i < tR + tS) {
         const uint32_t k = i - tR;
@@ -681,7 +866,7 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
         d.in_off = k * kTile;
         d.len = min((uint32_t)kTile, nS - k * kTile);
         d.seg = 1;
-        d.pad = 0;
+        d.aux = tile_row(k, tS);
         tiles[i] = d;
     }
     if (i == 0) {
@@ -692,14 +877,14 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
         a.ntiles = tR;
         a.first_tile = 0;
         a.ngroups = seg_groups(0, tR);
-        a.pad = 0;
+        a.aux = 0;
         b.hist_base = (uint64_t)nbins * a.ngroups;
         b.in_off = 0;
         b.n = nS;
         b.ntiles = tS;
         b.first_tile = tR;
         b.ngroups = seg_groups(tR, tS);
-        b.pad = 0;
+        b.aux = 0;
         segs[0] = a;
         segs[1] = b;
         *ntiles_dev = tR + tS;
@@ -815,30 +1000,44 @@ static uint32_t persistent_grid(uint32_t ntiles) {
 
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
-                 const BinSpec& bs, uint32_t nbins, uint32_t* fine, const BinSpec* bs2,
-                 uint32_t nbins2) {
+                 const BinSpec& bs, uint32_t nbins) {
     if (!ntiles) return;
-    const BinSpec b2 = bs2 ? *bs2 : bs;
     // the learned model is loaded into LDS once per workgroup: persistent
     // grid; radix: one tile per workgroup
     const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
     hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles,
-                       ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins, fine, b2,
-                       nbins2);
+                       ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins);
 }
 
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass) {
+                    const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
+                    const ClaimSpec* claims) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
-    const size_t lds = 2 * sizeof(uint32_t) * nbins;
+    const bool claim = claims != nullptr;
+    const size_t lds = (claim ? 3 : 2) * sizeof(uint32_t) * nbins;
     const bool learned = bs.kind == kBinLearned;
-    auto k = learned ? (last_pass ? k_scatter<true, true> : k_scatter<true, false>)
-                     : (last_pass ? k_scatter<false, true> : k_scatter<false, false>);
+    Claims cl{};
+    if (claim) {
+        cl.starts = claims->starts;
+        cl.cursors = claims->cursors;
+        cl.overflow = claims->overflow;
+        cl.cb0 = claims->cb0;
+        cl.cmul = claims->cmul;
+        cl.cstride = claims->cstride;
+        cl.xmul = claims->xmul;
+    }
+    decltype(&k_scatter<false, false, false>) k;
+    if (claim)
+        k = learned ? (last_pass ? k_scatter<true, true, true> : k_scatter<true, false, true>)
+                    : (last_pass ? k_scatter<false, true, true> : k_scatter<false, false, true>);
+    else
+        k = learned ? (last_pass ? k_scatter<true, true, false> : k_scatter<true, false, false>)
+                    : (last_pass ? k_scatter<false, true, false> : k_scatter<false, false, false>);
     hipLaunchKernelGGL(k, dim3(ntiles), dim3(kPartBlock), lds, st, in, out, tiles, ntiles_dev,
-                       tile_base, tile_base + ntiles, offs, bs, nbins);
+                       tile_base, tile_base + ntiles, offs, cl, bs, nbins);
 }
 
 void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,
@@ -932,19 +1131,60 @@ void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t
                        hist, n, count_dev, mult);
 }
 
-void launch_region_caps(hipStream_t st, const uint32_t* fine, const SegDesc* segs2,
-                        uint32_t nseg2, uint32_t nbins2, uint32_t* caps) {
-    hipLaunchKernelGGL(k_region_caps, dim3(nseg2), dim3(256), 0, st, fine, segs2, nbins2, caps,
-                       nseg2 * nbins2);
+void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
+                        uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
+                        const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
+                        uint32_t* cnt1, uint32_t* fine) {
+    if (nrowR + nrowS == 0) return;
+    hipLaunchKernelGGL(k_sample_rows, dim3((nrowR + nrowS) * kChunksPerRow), dim3(kSampleBlock), 0,
+                       st, R, nR, S, nS, nrowR, nrowS, bsr, bss, bs2, F1, F2, cnt1, fine);
 }
 
-void launch_scatter_est(hipStream_t st, uint32_t grid, const uint2* in, uint2* out,
-                        const TileDesc* tiles, const uint32_t* ntiles_dev, const uint32_t* starts,
-                        uint32_t* cursors, uint32_t* overflow, const BinSpec& bs,
-                        uint32_t nbins) {
-    if (!grid) return;
-    hipLaunchKernelGGL(k_scatter_est, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, out,
-                       tiles, ntiles_dev, starts, cursors, overflow, bs, nbins);
+void launch_row_caps(hipStream_t st, const uint32_t* cnt, uint32_t n, uint32_t* caps) {
+    hipLaunchKernelGGL(k_row_caps, dim3((n + 1 + 255) / 256), dim3(256), 0, st, cnt, n, caps);
+}
+
+void launch_part_sizes(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                       uint32_t F1, uint32_t nrowR, uint32_t nrowS, uint32_t* npart,
+                       uint32_t* total) {
+    hipLaunchKernelGGL(k_part_sizes, dim3((2 * F1 + 255) / 256), dim3(256), 0, st, starts,
+                       cursors, F1, nrowR, nrowS, npart, total);
+}
+
+void launch_est_part_max(hipStream_t st, const uint32_t* cnt1, uint32_t F1, uint32_t nrowR,
+                         uint32_t* stats) {
+    hipLaunchKernelGGL(k_est_part_max, dim3((F1 + 255) / 256), dim3(256), 0, st, cnt1, F1, nrowR,
+                       stats);
+}
+
+void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* npart,
+                        uint32_t nseg2, uint32_t nbins2, const uint32_t* first,
+                        const uint32_t* ntiles2, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
+                        uint32_t* caps) {
+    hipLaunchKernelGGL(k_region_caps, dim3(nseg2), dim3(256), 0, st, fine, npart, nbins2, first,
+                       ntiles2, F1, nrowR, nrowS, caps, nseg2 * nbins2);
+}
+
+void launch_region_tiles(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                         uint32_t n, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
+                         uint32_t* ntiles) {
+    hipLaunchKernelGGL(k_region_tiles, dim3((n + 1 + 255) / 256), dim3(256), 0, st, starts,
+                       cursors, n, F1, nrowR, nrowS, ntiles);
+}
+
+void launch_region_tile_descs(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                              const uint32_t* first, uint32_t n, uint32_t F1, uint32_t nrowR,
+                              uint32_t nrowS, TileDesc* tiles) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_region_tile_descs, dim3((n + 255) / 256), dim3(256), 0, st, starts,
+                       cursors, first, n, F1, nrowR, nrowS, tiles);
+}
+
+void launch_bounds_x8(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
+                      uint32_t F1, uint32_t F2, uint2* bounds) {
+    const uint32_t n = 16 * F1 * F2;
+    hipLaunchKernelGGL(k_bounds_x8, dim3((n + 255) / 256), dim3(256), 0, st, starts, cursors, F1,
+                       F2, bounds);
 }
 
 void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,

@@ -6,11 +6,18 @@
 // become per-tile histograms, and the task queues become a device task list
 // consumed through an atomic counter.
 //
-// Both passes are exact and deterministic (histogram, scan, scatter).  With
-// tbj_options.sampled_regions the second pass instead sizes every
-// sub-partition from a 1/16 sample taken during the first histogram and
-// claims space with atomics, skipping its histogram read; should a
-// sub-partition outgrow its region, the call is redone exactly.
+// Two ways to place the partitions:
+//  * exact: per pass a histogram read of the input, offsets from a scan,
+//    then the scatter (deterministic layout);
+//  * sample-sized (two-pass plans of large joins): one kernel reads 1/16 of
+//    the input's 16-tuple groups and sizes, from those counts, every output
+//    region of both passes; the scatters claim space in them with one atomic
+//    per (tile, bin), so neither pass reads its input twice.  First-pass
+//    regions are per (bin, tile row): a row is 224 consecutive tiles of one
+//    XCD's eighth, so that a region is written by one XCD.  Should any region
+//    turn out too small, the call is redone exactly.  The first-pass regions
+//    are laid out row by row (a tile's claims hit consecutive words); the
+//    second pass visits them partition by partition.
 #include <string.h>
 
 #include <algorithm>
@@ -29,9 +36,13 @@ struct Small {  // device scalars, one allocation
     uint32_t stats[2];
     uint32_t minmax[2];
     uint32_t total1, total2, overflow, region_total;
-    uint32_t ngroups2, pad[3];
+    uint32_t ngroups2, est_max, pad[2];
     unsigned long long counters[4];
 };
+
+// Joins at least this large (|R| + |S|) use sample-sized regions when
+// tbj_options.sampled_regions is 0 (auto).
+constexpr uint64_t kSampledMin = 1ull << 24;
 
 uint32_t ceil_log2(uint64_t v) {
     uint32_t b = 0;
@@ -102,15 +113,300 @@ struct Job {
     int mode;
 };
 
+// The per-call state shared by the partition paths.
+struct Call {
+    Context& C;
+    Workspace& W;
+    const Job& J;
+    const tbj_options& opt;
+    tbj_result* res;
+    hipStream_t st;
+    PhaseTimer& pt;
+    Small* sm;
+    PassPlan plan;
+    uint32_t nparts;
+    uint2* bounds;
+    uint32_t* task_off;
+    uint32_t* tpartial;
+    bool learned;
+    const uint2* Rb;  // the join's inputs: the partitioned relations
+    const uint2* Sb;
+    uint32_t npieces;  // ranges per partition side in bounds (1, or 8 per-XCD pieces)
+};
+
+// The learned model (fitted on a strided key sample of R and R's exact key
+// range) and the partition functions of both passes.
+int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2) {
+    const PassPlan& plan = k.plan;
+    if (!k.learned) {
+        *s1r = *s1s = radix_spec(plan, 1);
+        *s2 = radix_spec(plan, 2);
+        return TBJ_OK;
+    }
+    Context& C = k.C;
+    const uint32_t nR = k.J.nR;
+    CdfModel* model = nullptr;
+    TBJ_TRY(k.W.get("model", sizeof(CdfModel), (void**)&model));
+    uint32_t m = 2;
+    while (m * 2 <= (uint32_t)std::min<uint64_t>(nR, kFitMax)) m *= 2;
+    if (k.opt.learned_samples > 0)
+        while (m > (uint32_t)k.opt.learned_samples && m > 2) m >>= 1;
+    uint32_t* sample = nullptr;
+    TBJ_TRY(k.W.get("sample", sizeof(uint32_t) * m, (void**)&sample));
+    // 64 samples per segment: knot noise of about 1/8 of a segment
+    const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kLdsKnots, m / 64));
+    launch_fill_u32(k.st, k.sm->minmax, 1, 0xFFFFFFFFu);
+    // the sample and its sort (one workgroup) run on the side stream while
+    // the exact key range reads all of R
+    TBJ_HIP(hipEventRecord(C.fork_ev, k.st));
+    TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
+    launch_sample(C.side_stream, k.J.R, nR, m, sample);
+    launch_fit_sort(C.side_stream, sample, m);
+    TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
+    launch_minmax(k.st, k.J.R, nR, k.sm->minmax);
+    TBJ_HIP(hipStreamWaitEvent(k.st, C.join_ev, 0));
+    launch_fit(k.st, sample, m, nseg, plan.B, k.sm->minmax, model);
+    *s1r = learned_spec(plan, 1, model, 0);
+    *s1s = learned_spec(plan, 1, model, 1);
+    *s2 = learned_spec(plan, 2, model, 1);
+    return TBJ_OK;
+}
+
+// The judge of the learned partitioner: the largest first-pass R partition
+// (exact, or estimated from the sample) against the per-partition bound.
+// Returns true when the call must fall back to radix partitioning.
+int judge_learned(Call& k, bool* fall_back) {
+    Small h;
+    TBJ_HIP(hipMemcpyAsync(&h, k.sm, sizeof(Small), hipMemcpyDeviceToHost, k.st));
+    TBJ_HIP(hipStreamSynchronize(k.st));
+    const uint64_t per = k.opt.learned_bound ? k.opt.learned_bound : (uint64_t)kJoinCap * 85 / 100;
+    *fall_back = h.stats[0] > per * k.plan.F2;
+    return TBJ_OK;
+}
+
+// Exact partitioning: per pass a histogram, offsets, scatter.
+int partition_exact(Call& k) {
+    Workspace& W = k.W;
+    const PassPlan& plan = k.plan;
+    hipStream_t st = k.st;
+    Small* sm = k.sm;
+    const uint32_t nR = k.J.nR, nS = k.J.nS;
+    const uint64_t ntot = (uint64_t)nR + nS;
+    uint2* C1 = nullptr;
+    TBJ_TRY(W.get("c1", sizeof(uint2) * ntot, (void**)&C1));
+    const uint32_t tR = (nR + kTile - 1) / kTile, tS = (nS + kTile - 1) / kTile;
+    const uint32_t T1 = tR + tS;
+    TileDesc* tiles1 = nullptr;
+    SegDesc* segs1 = nullptr;
+    uint16_t* hist1 = nullptr;
+    uint32_t *offs1 = nullptr, *gs1 = nullptr, *partial = nullptr;
+    // per-tile counts and offsets (T1 rows) and the group sums of R's and
+    // S's tiles
+    const uint64_t L1 = (uint64_t)plan.F1 * (seg_groups(0, tR) + seg_groups(tR, tS)) + 1;
+    TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
+    TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
+    TBJ_TRY(W.get("hist1", sizeof(uint16_t) * plan.F1 * T1, (void**)&hist1));
+    TBJ_TRY(W.get("offs1", sizeof(uint32_t) * plan.F1 * T1, (void**)&offs1));
+    TBJ_TRY(W.get("gs1", sizeof(uint32_t) * L1, (void**)&gs1));
+    const uint32_t T2max = plan.passes == 2 ? (uint32_t)(ntot / kTile) + 2 * plan.F1 + 2 : 0;
+    // groups of the second pass: at most one more than tiles / kGroupTiles
+    // per segment, plus one straddling each segment start
+    const uint64_t G2max = T2max / kGroupTiles + 2ull * (2 * plan.F1) + 1;
+    const uint64_t L2max = plan.passes == 2 ? (uint64_t)plan.F2 * G2max + 1 : 0;
+    TBJ_TRY(W.get("partial",
+                  sizeof(uint32_t) * scan_partials(std::max({L1, L2max, (uint64_t)k.nparts + 1})),
+                  (void**)&partial));
+    BinSpec s1r, s1s, s2;
+    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2));
+
+    // ---- pass 1: histogram, scan, scatter
+    launch_make_tiles1(st, tiles1, segs1, nR, nS, plan.F1, &sm->ntiles1);
+    launch_hist(st, tR, k.J.R, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1);
+    launch_hist(st, tS, k.J.S, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1);
+    launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
+    TBJ_TRY(k.pt.mark(kPhHist1));
+    launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1,
+                        partial, &sm->total1);
+    if (k.learned) {
+        // Check the model on the first-pass R partitions; fall back to radix
+        // when one would overflow the per-partition bound.
+        launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, k.bounds);
+        launch_count_tasks(st, k.bounds, plan.F1, kJoinCap, 1u << 30, k.task_off, k.tpartial,
+                           &sm->ntasks, sm->stats);
+        bool fb = false;
+        TBJ_TRY(judge_learned(k, &fb));
+        if (fb) {
+            k.learned = false;
+            k.res->fallback = 1;
+            s1r = s1s = radix_spec(plan, 1);
+            s2 = radix_spec(plan, 2);
+            launch_hist(st, tR, k.J.R, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1);
+            launch_hist(st, tS, k.J.S, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1);
+            launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
+            launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1,
+                                L1, partial, &sm->total1);
+        }
+        launch_fill_u32(st, sm->stats, 2, 0);
+    }
+    TBJ_TRY(k.pt.mark(kPhScan1));
+    launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, plan.F1,
+                   plan.passes == 1);
+    launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, plan.F1,
+                   plan.passes == 1);
+    TBJ_TRY(k.pt.mark(kPhScatter1));
+
+    if (plan.passes == 1) {
+        launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, k.bounds);
+        k.Rb = k.Sb = C1;
+        return TBJ_OK;
+    }
+    // ---- pass 2: every first-pass partition is a segment
+    TileDesc* tiles2 = nullptr;
+    SegDesc* segs2 = nullptr;
+    uint2* C2 = nullptr;
+    uint16_t* hist2 = nullptr;
+    uint32_t *offs2 = nullptr, *gs2 = nullptr;
+    TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2max, (void**)&tiles2));
+    TBJ_TRY(W.get("segs2", sizeof(SegDesc) * 2 * plan.F1, (void**)&segs2));
+    TBJ_TRY(W.get("c2", sizeof(uint2) * ntot, (void**)&C2));
+    TBJ_TRY(W.get("hist2", sizeof(uint16_t) * plan.F2 * T2max, (void**)&hist2));
+    TBJ_TRY(W.get("offs2", sizeof(uint32_t) * plan.F2 * T2max, (void**)&offs2));
+    TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2max, (void**)&gs2));
+    launch_make_segments(st, gs1, segs1, 2, plan.F1, plan.F2, segs2, tiles2, &sm->ntiles2, T2max,
+                         &sm->ngroups2);
+    launch_hist(st, T2max, C1, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
+    // group sums past the segments' own are zero, so the scan's total is the
+    // tuple count
+    launch_zero_tail(st, gs2, L2max, &sm->ngroups2, plan.F2, L2max);
+    launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, plan.F2, gs2, L2max,
+                        partial, &sm->total2);
+    TBJ_TRY(k.pt.mark(kPhSize2));
+    launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, plan.F2, true);
+    launch_part_bounds(st, gs2, segs2, plan.F1, plan.F2, 2, nR, nS, k.bounds);
+    TBJ_TRY(k.pt.mark(kPhScatter2));
+    k.Rb = k.Sb = C2;
+    return TBJ_OK;
+}
+
+bool sampled_fits(const PassPlan& plan) {
+    return plan.passes == 2 && plan.F1 * plan.F2 <= (1u << 16);
+}
+
+// Sample-sized two-pass partitioning (no histogram read).
+int partition_sampled(Call& k) {
+    Workspace& W = k.W;
+    const PassPlan& plan = k.plan;
+    hipStream_t st = k.st;
+    Small* sm = k.sm;
+    const uint32_t nR = k.J.nR, nS = k.J.nS;
+    const uint64_t ntot = (uint64_t)nR + nS;
+    const uint32_t F1 = plan.F1, F2 = plan.F2;
+    const uint32_t tR = (nR + kTile - 1) / kTile, tS = (nS + kTile - 1) / kTile;
+    const uint32_t T1 = tR + tS;
+    const uint32_t nrowR = 8 * rows_per_eighth(tR), nrowS = 8 * rows_per_eighth(tS);
+    const uint32_t np1 = F1 * (nrowR + nrowS);  // first-pass regions
+    const uint32_t nsub = 2 * k.nparts;           // second-pass regions
+    const uint32_t nsub8 = 8 * nsub;              // their per-XCD sub-regions
+    // Region sizes are only known on the device; the buffers take the largest
+    // sum the capacity rule allows (sum of 16 * counts <= |R| + |S| plus the
+    // sampled blocks' rounding; sum of sqrt <= sqrt(n * sum), Cauchy-Schwarz;
+    // second-pass estimates sum to at most twice each partition).
+    const double e1 = (double)ntot + 256.0 * (nrowR + nrowS);
+    const double cap1 = 1.1 * e1 + 48.0 * sqrt((double)np1 * e1) + 640.0 * np1 + 1;
+    // second pass: at most F2 * (2 F1 + 7) sub-regions are non-empty (a
+    // partition's tiles span one eighth more per eighth boundary inside them)
+    const double e2 = 2.0 * (double)ntot;
+    const double nsx = (double)F2 * (2.0 * F1 + 7.0);
+    const double cap2 = 1.1 * e2 + 48.0 * sqrt(nsx * e2) + 640.0 * nsx + 1;
+    if (cap1 >= 4294967295.0 || cap2 >= 4294967295.0) return partition_exact(k);
+
+    uint2 *C1 = nullptr, *C2 = nullptr;
+    TileDesc *tiles1 = nullptr, *tiles2 = nullptr;
+    SegDesc* segs1 = nullptr;
+    uint32_t *cnt1 = nullptr, *starts1 = nullptr, *cursors1 = nullptr, *fine = nullptr,
+             *npart = nullptr, *starts2 = nullptr, *cursors2 = nullptr, *first2 = nullptr,
+             *partial = nullptr;
+    const uint32_t T2max = (uint32_t)(ntot / kTile) + np1 + 1;
+    TBJ_TRY(W.get("c1", sizeof(uint2) * (uint64_t)cap1, (void**)&C1));
+    TBJ_TRY(W.get("c2", sizeof(uint2) * (uint64_t)cap2, (void**)&C2));
+    TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
+    TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
+    TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2max, (void**)&tiles2));
+    TBJ_TRY(W.get("cnt1", sizeof(uint32_t) * np1, (void**)&cnt1));
+    TBJ_TRY(W.get("starts1", sizeof(uint32_t) * (np1 + 1), (void**)&starts1));
+    TBJ_TRY(W.get("cursors1", sizeof(uint32_t) * np1, (void**)&cursors1));
+    TBJ_TRY(W.get("fine", sizeof(uint32_t) * nsub, (void**)&fine));
+    TBJ_TRY(W.get("npart", sizeof(uint32_t) * 2 * F1, (void**)&npart));
+    TBJ_TRY(W.get("starts2", sizeof(uint32_t) * (nsub8 + 1), (void**)&starts2));
+    TBJ_TRY(W.get("cursors2", sizeof(uint32_t) * nsub8, (void**)&cursors2));
+    TBJ_TRY(W.get("first2", sizeof(uint32_t) * (np1 + 1), (void**)&first2));
+    TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(std::max<uint64_t>(np1, nsub8) + 1),
+                  (void**)&partial));
+    BinSpec s1r, s1s, s2;
+    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2));
+
+    // ---- sample: first-pass row counts and (b1, b2) counts
+    launch_make_tiles1(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
+    launch_fill_u32(st, fine, nsub, 0);
+    launch_fill_u32(st, cnt1, np1, 0);
+    launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1, fine);
+    if (k.learned) {
+        launch_est_part_max(st, cnt1, F1, nrowR, sm->stats);
+        bool fb = false;
+        TBJ_TRY(judge_learned(k, &fb));
+        launch_fill_u32(st, sm->stats, 2, 0);
+        if (fb) {
+            k.learned = false;
+            k.res->fallback = 1;
+            s1r = s1s = radix_spec(plan, 1);
+            s2 = radix_spec(plan, 2);
+            launch_fill_u32(st, fine, nsub, 0);
+            launch_fill_u32(st, cnt1, np1, 0);
+            launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2,
+                               cnt1, fine);
+        }
+    }
+    TBJ_TRY(k.pt.mark(kPhHist1));
+    launch_row_caps(st, cnt1, np1, starts1);
+    launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
+    launch_fill_u32(st, cursors1, np1, 0);
+    launch_fill_u32(st, cursors2, nsub8, 0);
+    TBJ_TRY(k.pt.mark(kPhScan1));
+
+    // ---- pass 1: claims per (tile, bin) in the tile row's regions
+    ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
+    ClaimSpec c1s{starts1, cursors1, &sm->overflow, F1 * nrowR, F1, 1, 0};
+    launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
+    launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false, &c1s);
+    TBJ_TRY(k.pt.mark(kPhScatter1));
+
+    // ---- pass 2: every filled first-pass region is a segment of tiles
+    launch_part_sizes(st, starts1, cursors1, F1, nrowR, nrowS, npart, &sm->total1);
+    launch_region_tiles(st, starts1, cursors1, np1, F1, nrowR, nrowS, first2);
+    launch_scan(st, first2, (uint64_t)np1 + 1, partial, &sm->ntiles2);
+    launch_region_caps(st, fine, npart, 2 * F1, F2, first2, &sm->ntiles2, F1, nrowR, nrowS,
+                       starts2);
+    launch_scan(st, starts2, (uint64_t)nsub8 + 1, partial, &sm->total2);
+    launch_region_tile_descs(st, starts1, cursors1, first2, np1, F1, nrowR, nrowS, tiles2);
+    TBJ_TRY(k.pt.mark(kPhSize2));
+    ClaimSpec c2{starts2, cursors2, &sm->overflow, 0, 8 * F2, 1, F2};
+    launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, nullptr, s2, F2, true, &c2);
+    // partition p's side rel is the 8 pieces bounds[(rel * nparts + p) * 8 + x]
+    launch_bounds_x8(st, starts2, cursors2, F1, F2, k.bounds);
+    k.npieces = 8;
+    TBJ_TRY(k.pt.mark(kPhScatter2));
+    k.Rb = k.Sb = C2;
+    return TBJ_OK;
+}
+
 // One full run; *overflow_out reports a too-small sample-sized region.  The
 // host waits for the device once, at the end (and once more for the learned
 // partitioner's judge).
-int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_result* res,
-             hipStream_t st, bool* overflow_out) {
+int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sampled,
+             tbj_result* res, hipStream_t st, bool* overflow_out) {
     Workspace& W = C.ws;
     PhaseTimer pt(C, st, opt.profile != 0);
     *overflow_out = false;
-    bool learned = opt.partitioner == TBJ_PART_LEARNED;
     const uint32_t nR = J.nR, nS = J.nS;
 
     Small* sm = nullptr;
@@ -119,188 +415,37 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_
     TBJ_TRY(pt.begin());
     TBJ_HIP(hipMemsetAsync(sm, 0, sizeof(Small), st));
 
-    const PassPlan plan = plan_passes((uint32_t)choose_bits(nR, opt), opt.passes);
-    if (plan.passes == 0) learned = false;
-    const uint32_t nparts = plan.passes ? plan.F1 * plan.F2 : 1;
+    Call k{C, W, J, opt, res, st, pt, sm, {}, 0, nullptr, nullptr, nullptr, false, J.R, J.S, 1};
+    k.plan = plan_passes((uint32_t)choose_bits(nR, opt), opt.passes);
+    const PassPlan& plan = k.plan;
+    k.learned = opt.partitioner == TBJ_PART_LEARNED && plan.passes > 0;
+    k.nparts = plan.passes ? plan.F1 * plan.F2 : 1;
     res->radix_bits = (int)plan.B;
     res->passes = (int)plan.passes;
-    res->num_partitions = nparts;
+    res->num_partitions = k.nparts;
+    const uint32_t nb = std::max(k.nparts, plan.F1);
+    TBJ_TRY(W.get("bounds", sizeof(uint2) * 2 * 8 * nb, (void**)&k.bounds));
+    TBJ_TRY(W.get("task_off", sizeof(uint32_t) * (nb + 1), (void**)&k.task_off));
+    TBJ_TRY(W.get("tpartial", sizeof(uint32_t) * scan_partials((uint64_t)nb + 1),
+                  (void**)&k.tpartial));
 
-    const uint2* Rb = J.R;
-    const uint2* Sb = J.S;
-    uint2* bounds = nullptr;
-    TBJ_TRY(W.get("bounds", sizeof(uint2) * 2 * std::max(nparts, plan.F1), (void**)&bounds));
-    uint32_t* task_off = nullptr;
-    TBJ_TRY(W.get("task_off", sizeof(uint32_t) * (std::max(nparts, plan.F1) + 1),
-                  (void**)&task_off));
-    uint32_t* tpartial = nullptr;
-    TBJ_TRY(W.get("tpartial", sizeof(uint32_t) * scan_partials((uint64_t)std::max(nparts, plan.F1) + 1),
-                  (void**)&tpartial));
-
+    const uint64_t ntot = (uint64_t)nR + nS;
+    const bool sampled = allow_sampled && sampled_fits(plan) &&
+                         (opt.sampled_regions > 0 || (opt.sampled_regions == 0 && ntot >= kSampledMin));
     if (plan.passes == 0) {
-        launch_part_bounds(st, nullptr, nullptr, 1, 1, 0, nR, nS, bounds);
+        launch_part_bounds(st, nullptr, nullptr, 1, 1, 0, nR, nS, k.bounds);
+    } else if (sampled) {
+        TBJ_TRY(partition_sampled(k));
     } else {
-        const uint64_t ntot = (uint64_t)nR + nS;
-        uint2* C1 = nullptr;
-        TBJ_TRY(W.get("c1", sizeof(uint2) * ntot, (void**)&C1));
-        const uint32_t tR = (nR + kTile - 1) / kTile, tS = (nS + kTile - 1) / kTile;
-        const uint32_t T1 = tR + tS;
-        TileDesc* tiles1 = nullptr;
-        SegDesc* segs1 = nullptr;
-        uint16_t* hist1 = nullptr;
-        uint32_t *offs1 = nullptr, *gs1 = nullptr, *partial = nullptr, *fine = nullptr;
-        // per-tile counts and offsets (T1 rows) and the group sums of R's and
-        // S's tiles
-        const uint64_t L1 = (uint64_t)plan.F1 * (seg_groups(0, tR) + seg_groups(tR, tS)) + 1;
-        TBJ_TRY(W.get("tiles1", sizeof(TileDesc) * T1, (void**)&tiles1));
-        TBJ_TRY(W.get("segs1", sizeof(SegDesc) * 2, (void**)&segs1));
-        TBJ_TRY(W.get("hist1", sizeof(uint16_t) * plan.F1 * T1, (void**)&hist1));
-        TBJ_TRY(W.get("offs1", sizeof(uint32_t) * plan.F1 * T1, (void**)&offs1));
-        TBJ_TRY(W.get("gs1", sizeof(uint32_t) * L1, (void**)&gs1));
-        const uint32_t T2max = plan.passes == 2 ? (uint32_t)(ntot / kTile) + 2 * plan.F1 + 2 : 0;
-        // groups of the second pass: at most one more than tiles / kGroupTiles
-        // per segment, plus one straddling each segment start
-        const uint64_t G2max = T2max / kGroupTiles + 2ull * (2 * plan.F1) + 1;
-        const uint64_t L2max = exact2 ? (uint64_t)plan.F2 * G2max + 1 : 0;
-        const uint64_t nsub = 2ull * plan.F1 * plan.F2;
-        TBJ_TRY(W.get("partial",
-                      sizeof(uint32_t) * scan_partials(std::max({L1, L2max, nsub + 1, (uint64_t)nparts + 1})),
-                      (void**)&partial));
-        const bool sample_fine = plan.passes == 2 && !exact2;
-        if (sample_fine) {
-            TBJ_TRY(W.get("fine", sizeof(uint32_t) * nsub, (void**)&fine));
-            launch_fill_u32(st, fine, nsub, 0);
-        }
-
-        CdfModel* model = nullptr;
-        BinSpec s1r, s1s, s2;
-        if (learned) {
-            TBJ_TRY(W.get("model", sizeof(CdfModel), (void**)&model));
-            uint32_t m = 2;
-            while (m * 2 <= (uint32_t)std::min<uint64_t>(nR, kFitMax)) m *= 2;
-            if (opt.learned_samples > 0)
-                while (m > (uint32_t)opt.learned_samples && m > 2) m >>= 1;
-            uint32_t* sample = nullptr;
-            TBJ_TRY(W.get("sample", sizeof(uint32_t) * m, (void**)&sample));
-            // 64 samples per segment: knot noise of about 1/8 of a segment
-            const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kLdsKnots, m / 64));
-            launch_fill_u32(st, sm->minmax, 1, 0xFFFFFFFFu);
-            // the sample and its sort (one workgroup) run on the side stream
-            // while the exact key range reads all of R
-            TBJ_HIP(hipEventRecord(C.fork_ev, st));
-            TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
-            launch_sample(C.side_stream, Rb, nR, m, sample);
-            launch_fit_sort(C.side_stream, sample, m);
-            TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
-            launch_minmax(st, Rb, nR, sm->minmax);
-            TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
-            launch_fit(st, sample, m, nseg, plan.B, sm->minmax, model);
-            s1r = learned_spec(plan, 1, model, 0);
-            s1s = learned_spec(plan, 1, model, 1);
-            s2 = learned_spec(plan, 2, model, 1);
-        } else {
-            s1r = s1s = radix_spec(plan, 1);
-            s2 = radix_spec(plan, 2);
-        }
-
-        // ---- pass 1 (exact): histogram, scan, scatter
-        launch_make_tiles1(st, tiles1, segs1, nR, nS, plan.F1, &sm->ntiles1);
-        launch_hist(st, tR, Rb, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine, &s2, plan.F2);
-        launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2, plan.F2);
-        launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
-        TBJ_TRY(pt.mark(kPhHist1));
-        launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1, gs1, L1,
-                            partial, &sm->total1);
-
-        if (learned) {
-            // Check the model on the first-pass R partitions; fall back to radix
-            // when one would overflow the per-partition bound.
-            launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, bounds);
-            launch_count_tasks(st, bounds, plan.F1, kJoinCap, 1u << 30, task_off, tpartial,
-                               &sm->ntasks, sm->stats);
-            Small h;
-            TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small), hipMemcpyDeviceToHost, st));
-            TBJ_HIP(hipStreamSynchronize(st));
-            const uint64_t per = opt.learned_bound ? opt.learned_bound : (uint64_t)kJoinCap * 85 / 100;
-            if (h.stats[0] > per * plan.F2) {
-                learned = false;
-                res->fallback = 1;
-                s1r = s1s = radix_spec(plan, 1);
-                s2 = radix_spec(plan, 2);
-                if (sample_fine) launch_fill_u32(st, fine, nsub, 0);
-                launch_hist(st, tR, Rb, tiles1, &sm->ntiles1, 0, hist1, s1r, plan.F1, fine, &s2,
-                            plan.F2);
-                launch_hist(st, tS, Sb, tiles1, &sm->ntiles1, tR, hist1, s1s, plan.F1, fine, &s2,
-                            plan.F2);
-                launch_fill_u32(st, gs1 + (L1 - 1), 1, 0);
-                launch_tile_offsets(st, T1, hist1, offs1, tiles1, segs1, &sm->ntiles1, plan.F1,
-                                    gs1, L1, partial, &sm->total1);
-            } else {
-                res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
-            }
-            launch_fill_u32(st, sm->stats, 2, 0);
-        }
-        TBJ_TRY(pt.mark(kPhScan1));
-        launch_scatter(st, tR, Rb, C1, tiles1, &sm->ntiles1, 0, offs1, s1r, plan.F1,
-                       plan.passes == 1);
-        launch_scatter(st, tS, Sb, C1, tiles1, &sm->ntiles1, tR, offs1, s1s, plan.F1,
-                       plan.passes == 1);
-        TBJ_TRY(pt.mark(kPhScatter1));
-
-        if (plan.passes == 1) {
-            launch_part_bounds(st, gs1, segs1, 1, plan.F1, 1, nR, nS, bounds);
-            Rb = Sb = C1;
-        } else {
-            // ---- pass 2: every first-pass partition is a segment
-            TileDesc* tiles2 = nullptr;
-            SegDesc* segs2 = nullptr;
-            TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2max, (void**)&tiles2));
-            TBJ_TRY(W.get("segs2", sizeof(SegDesc) * 2 * plan.F1, (void**)&segs2));
-            launch_make_segments(st, gs1, segs1, 2, plan.F1, plan.F2, segs2, tiles2,
-                                 &sm->ntiles2, T2max, &sm->ngroups2);
-            uint2* C2 = nullptr;
-            if (exact2) {
-                uint16_t* hist2 = nullptr;
-                uint32_t *offs2 = nullptr, *gs2 = nullptr;
-                TBJ_TRY(W.get("c2", sizeof(uint2) * ntot, (void**)&C2));
-                TBJ_TRY(W.get("hist2", sizeof(uint16_t) * plan.F2 * T2max, (void**)&hist2));
-                TBJ_TRY(W.get("offs2", sizeof(uint32_t) * plan.F2 * T2max, (void**)&offs2));
-                TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2max, (void**)&gs2));
-                launch_hist(st, T2max, C1, tiles2, &sm->ntiles2, 0, hist2, s2, plan.F2);
-                // group sums past the segments' own are zero, so the scan's
-                // total is the tuple count
-                launch_zero_tail(st, gs2, L2max, &sm->ngroups2, plan.F2, L2max);
-                launch_tile_offsets(st, T2max, hist2, offs2, tiles2, segs2, &sm->ntiles2, plan.F2,
-                                    gs2, L2max, partial, &sm->total2);
-                TBJ_TRY(pt.mark(kPhSize2));
-                launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, plan.F2,
-                               true);
-                launch_part_bounds(st, gs2, segs2, plan.F1, plan.F2, 2, nR, nS, bounds);
-            } else {
-                // regions sized from the 1/16 sample (ratio estimate inside each
-                // first-pass partition); room for 1.1x + 6 sigma + 64 tuples
-                uint32_t *starts = nullptr, *cursors = nullptr;
-                TBJ_TRY(W.get("starts", sizeof(uint32_t) * (nsub + 1), (void**)&starts));
-                TBJ_TRY(W.get("cursors", sizeof(uint32_t) * nsub, (void**)&cursors));
-                // caps: 1.1 e + 6 sqrt(16 e) + 64 with sum(e) <= 2 ntot (raw and
-                // shrunk estimates each sum to the segment size)
-                const double cap_tuples = 2.2 * (double)ntot +
-                                          24.0 * sqrt(2.0 * (double)ntot * (double)nsub) +
-                                          64.0 * nsub + 1;
-                if (cap_tuples >= 4294967295.0) return TBJ_ERR_TOO_LARGE;
-                TBJ_TRY(W.get("c2", sizeof(uint2) * (uint64_t)cap_tuples, (void**)&C2));
-                launch_region_caps(st, fine, segs2, 2 * plan.F1, plan.F2, starts);
-                launch_scan(st, starts, nsub + 1, partial, &sm->region_total);
-                launch_fill_u32(st, cursors, nsub, 0);
-                TBJ_TRY(pt.mark(kPhSize2));
-                launch_scatter_est(st, T2max, C1, C2, tiles2, &sm->ntiles2, starts, cursors,
-                                   &sm->overflow, s2, plan.F2);
-                launch_bounds_est(st, starts, cursors, (uint32_t)nsub, bounds);
-            }
-            TBJ_TRY(pt.mark(kPhScatter2));
-            Rb = Sb = C2;
-        }
+        TBJ_TRY(partition_exact(k));
     }
+    const bool learned = k.learned;
+    const uint2* Rb = k.Rb;
+    const uint2* Sb = k.Sb;
+    const uint32_t nparts = k.nparts;
+    uint2* bounds = k.bounds;
+    uint32_t* task_off = k.task_off;
+    uint32_t* tpartial = k.tpartial;
     res->partitioner_used = learned ? TBJ_PART_LEARNED : TBJ_PART_RADIX;
     TBJ_HIP(hipEventRecord(C.ev[1], st));
 
@@ -309,7 +454,7 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_
     const uint32_t s_chunk = (opt.s_chunk > 0 && (uint32_t)opt.s_chunk < max_chunk)
                                  ? (uint32_t)opt.s_chunk : max_chunk;
     launch_count_tasks(st, bounds, nparts, kJoinCap, s_chunk, task_off, tpartial, &sm->ntasks,
-                       sm->stats);
+                       sm->stats, k.npieces);
     // the task list is sized without the count: every partition's tasks are
     // listed unless one spans several builds and several S chunks at once
     // (then the join decodes the rest from task_off)
@@ -320,7 +465,8 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_
     ts.nparts = nparts;
     ts.cap = kJoinCap;
     ts.chunk = s_chunk;
-    const uint64_t tcap = (uint64_t)nparts + (nR + kJoinCap - 1) / kJoinCap +
+    ts.npieces = k.npieces;
+    const uint64_t tcap = (uint64_t)nparts * k.npieces + (nR + kJoinCap - 1) / kJoinCap +
                           (nS + s_chunk - 1) / s_chunk + 64;
     ts.task_cap = (uint32_t)std::min<uint64_t>(tcap, 1ull << 31);
     TBJ_TRY(W.get("tasks", sizeof(JoinTask) * ts.task_cap, (void**)&ts.tasks));
@@ -339,6 +485,7 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool exact2, tbj_
     TBJ_HIP(hipStreamSynchronize(st));
     TBJ_TRY(pt.finish(res));
     res->max_r_partition = h.stats[0];
+    if (learned) res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
     *overflow_out = h.overflow != 0;
     res->matches = h.counters[0];
     res->checksum = h.counters[1];
@@ -377,12 +524,12 @@ int run_radix_join(Context& C, int algo, const tbj_tuple* R, uint64_t nR64, cons
         return TBJ_OK;
     }
     bool overflow = false;
-    TBJ_TRY(run_once(C, J, opt, opt.sampled_regions == 0, res, st, &overflow));
+    TBJ_TRY(run_once(C, J, opt, opt.sampled_regions >= 0, res, st, &overflow));
     if (overflow) {
         // a sample-sized region was too small: redo with exact histograms
         const float first_ms = res->ms_total;
         memset(res, 0, sizeof(*res));
-        TBJ_TRY(run_once(C, J, opt, true, res, st, &overflow));
+        TBJ_TRY(run_once(C, J, opt, false, res, st, &overflow));
         res->redone_exact = 1;
         res->ms_total += first_ms;
     }

@@ -431,7 +431,7 @@ __global__ void k_make_tiles64(TileDesc* __restrict__ tiles, SegDesc* __restrict
         d.in_off = k * kSm64Span;
         d.len = min((uint32_t)kSm64Span, n - k * kSm64Span);
         d.seg = r ? 0 : 1;
-        d.pad = 0;
+        d.aux = 0;
         tiles[i] = d;
     }
     if (i == 0) {
@@ -480,7 +480,7 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments64(
             d.ntiles = nt;
             d.first_tile = ft;
             d.ngroups = ng;
-            d.pad = 0;
+            d.aux = 0;
             d.hist_base = (uint64_t)(gcarry_s + gex) * nbins2;
             segs2[s2] = d;
         }
@@ -514,7 +514,7 @@ __global__ void k_make_tiles64_2(const SegDesc* __restrict__ segs2, uint32_t nse
     td.in_off = sd.in_off + k * kSm64Span;
     td.len = min((uint32_t)kSm64Span, sd.n - k * kSm64Span);
     td.seg = lo;
-    td.pad = 0;
+    td.aux = 0;
     tiles2[t] = td;
 }
 

@@ -119,7 +119,8 @@ def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
     o.materialize = 1 if materialize else 0
     o.workspace_cap, o.learned_bound = workspace_cap, learned_bound
     o.learned_samples, o.s_chunk = learned_samples, s_chunk
-    o.sampled_regions = 1 if sampled_regions else 0
+    # True = always, False = auto; -1 = never
+    o.sampled_regions = int(sampled_regions)
     o.profile = 1 if profile else 0
     o.output = int(output)
     return o

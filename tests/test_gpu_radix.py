@@ -175,25 +175,61 @@ def test_reference_config1_count(tbj):
     assert res.matches == 2097152 and res.checksum == 0
 
 
-@pytest.mark.parametrize("algo", ["RHO", "RHT"])
-def test_sampled_second_pass(tbj, oracle, algo):
+def sampled_positions(n):
+    """Tuples the sample-sized passes sample: exactly one 16-tuple group in
+    every block of 256 tuples, at a hashed position."""
+    i = np.arange(n, dtype=np.uint64)
+    block = i >> np.uint64(8)
+    pick = ((block * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)) >> np.uint64(28)
+    return ((i >> np.uint64(4)) & np.uint64(15)) == pick
+
+
+@pytest.mark.parametrize("algo", ["RHO", "RHT", "RSM", "PSM"])
+@pytest.mark.parametrize("z", [0.0, 1.2])
+def test_sampled_passes(tbj, oracle, algo, z):
     nR, nS = 1 << 18, 1 << 20
     R = tbj.gen_pk(11111, nR)
-    S = tbj.gen_zipf(22222, nS, nR, 1.2)
+    S = tbj.gen_zipf(22222, nS, nR, z) if z else tbj.gen_fk(22222, nS, nR)
     res = assert_join(tbj, oracle, algo, R, S, sampled_regions=True, radix_bits=14, passes=2)
     assert res.passes == 2 and res.redone_exact == 0
 
 
+@pytest.mark.parametrize("nR,nS", [(4097, 1), (100003, 300007), (1 << 20, 123457), (1 << 16, 1 << 22)])
+def test_sampled_passes_ragged_rows(tbj, oracle, nR, nS):
+    # relations whose tiles do not fill the XCD eighths or the rows evenly
+    R = tbj.gen_pk(5, nR)
+    S = tbj.gen_zipf(6, nS, nR, 0.7)
+    res = assert_join(tbj, oracle, "RHO", R, S, sampled_regions=True, passes=2,
+                      radix_bits=max(9, min(16, int(np.ceil(np.log2(nR / 512))))))
+    assert res.passes == 2 and res.redone_exact == 0
+
+
+def test_sampled_learned_drops_outside_range(tbj, oracle):
+    nR, nS = 1 << 18, 1 << 20
+    R = tbj.gen_pk(11111, nR)
+    S = tbj.remap_odd(tbj.gen_zipf(22222, nS, nR, 1.0), nR)
+    res = assert_join(tbj, oracle, "RHO", R, S, partitioner="learned", sampled_regions=True,
+                      radix_bits=14, passes=2)
+    assert res.partitioner_used == tbj.PART_LEARNED and res.s_dropped == nS // 2
+    assert res.redone_exact == 0
+
+
 def test_sample_sized_region_overflow_redoes_exact(tbj, oracle):
-    # tuples the 1/16 sample sees all carry key 1; the rest carry key 2, whose
-    # sub-partition the sample believes empty
+    # tuples the sample sees all carry key 1; the rest carry key 2, whose
+    # regions the sample believes empty
     nR, nS = 1 << 14, 1 << 16
     R = tbj.gen_pk(1, nR)
-    idx = np.arange(nS, dtype=np.uint64)
-    sampled = ((idx * 2654435761) & 0xFFFFFFFF) >> 28 == 0
-    S = rel(np.where(sampled, 1, 2))
+    S = rel(np.where(sampled_positions(nS), 1, 2))
     res = assert_join(tbj, oracle, "RHO", R, S, radix_bits=12, passes=2, sampled_regions=True)
     assert res.redone_exact == 1 and res.matches == nS
+
+
+def test_exact_passes_when_asked(tbj, oracle):
+    nR, nS = 1 << 18, 1 << 20
+    R = tbj.gen_pk(3, nR)
+    S = tbj.gen_fk(4, nS, nR)
+    res = assert_join(tbj, oracle, "RHO", R, S, sampled_regions=-1, radix_bits=14, passes=2)
+    assert res.redone_exact == 0
 
 
 def test_profile_phases(tbj, oracle):

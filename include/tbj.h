@@ -165,6 +165,29 @@ int tbj_join3(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
               const tbj_tuple* T, uint64_t nT, tbj_tuple* out, uint64_t out_cap,
               const tbj_options* opt, tbj_result* res, void* stream);
 
+/*
+ * tbj_join without any device allocation (no counterpart in the reference:
+ * for callers that cannot allocate device memory, such as code under HIP
+ * graph capture, tight serving loops and harnesses that forbid hipMalloc).
+ * The same kernels as tbj_join; their workspace comes from one fixed device
+ * array of TBJ_NOALLOC_SCRATCH_MB MiB that the library holds on each GPU it
+ * has been initialised on, until the process ends.  tbj_init must have run
+ * first.  Limits: radix-family algorithms (RHO, RHT, RJ, RHO_atomic, RSM,
+ * PSM, MWAY; others TBJ_ERR_UNSUPPORTED), nR + nS <= TBJ_NOALLOC_MAX_TUPLES
+ * (TBJ_ERR_TOO_LARGE above; also when a forced radix plan needs more
+ * workspace than the array holds).  opt->sampled_regions is ignored.  Every
+ * call on one GPU shares that GPU's array, which keeps the last call's
+ * contents (each call clears what must start at zero).  Two calls that
+ * overlapped on one GPU would race on it; the calls are synchronous and the
+ * library holds one lock per GPU around each, so calls from several host
+ * threads run one after another.
+ */
+#define TBJ_NOALLOC_MAX_TUPLES (1u << 22)
+#define TBJ_NOALLOC_SCRATCH_MB 80
+int tbj_join_noalloc(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+                     tbj_pair* pairs, uint64_t pairs_cap, const tbj_options* opt,
+                     tbj_result* res, void* stream);
+
 /* Same with host pointers: copies R and S to HBM, joins, copies pairs back. */
 int tbj_join_host(int algo,
                   const tbj_tuple* R, uint64_t nR,

@@ -16,7 +16,7 @@
 namespace tbj {
 
 static Device g_dev;
-static std::mutex g_mu;  // g_dev and the session table
+static std::mutex g_mu;  // g_dev, the session table, the scratch addresses
 static std::unordered_map<void*, std::unique_ptr<Context>> g_sessions;
 static thread_local char g_last_error[512];
 
@@ -29,8 +29,44 @@ void set_last_error(const char* what, hipError_t e, const char* file, int line) 
 
 void set_last_message(const char* msg) { snprintf(g_last_error, sizeof(g_last_error), "%s", msg); }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch of the no-allocation entry point: one copy per GPU,
+// held until the process ends.  Host code never passes `scratch` itself to a
+// kernel (in host code the name is a host-side stand-in, not a device
+// address): each device's address comes from hipGetSymbolAddress at
+// tbj_init, with that device current, and only that address is used.
+constexpr size_t kNoAllocBytes = (size_t)TBJ_NOALLOC_SCRATCH_MB << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::unique_ptr<Context> g_noalloc[kMaxDevices];
+
 int Workspace::get(const char* name, size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
+    if (arena_) {
+        for (int i = 0; i < arena_n_; i++) {
+            if (!strcmp(arena_name_[i], name)) {
+                if (arena_buf_[i].bytes >= bytes) {
+                    *out = arena_buf_[i].ptr;
+                    return TBJ_OK;
+                }
+                break;  // larger than before: carve a new block
+            }
+        }
+        const size_t want = (bytes + 255) & ~(size_t)255;
+        if (arena_used_ + want > arena_bytes_ || arena_n_ >= kArenaNames) {
+            set_last_message("no-allocation scratch exhausted (workspace above "
+                             "TBJ_NOALLOC_SCRATCH_MB)");
+            return TBJ_ERR_TOO_LARGE;
+        }
+        arena_name_[arena_n_] = name;
+        arena_buf_[arena_n_].ptr = arena_ + arena_used_;
+        arena_buf_[arena_n_].bytes = want;
+        arena_n_++;
+        *out = arena_ + arena_used_;
+        arena_used_ += want;
+        return TBJ_OK;
+    }
     Buf& b = bufs_[name];
     if (b.bytes >= bytes) {
         *out = b.ptr;
@@ -62,6 +98,10 @@ int Workspace::get(const char* name, size_t bytes, void** out) {
 }
 
 void Workspace::release_all() {
+    if (arena_) {
+        reset();
+        return;
+    }
     for (auto& kv : bufs_)
         if (kv.second.ptr) (void)hipFree(kv.second.ptr);
     bufs_.clear();
@@ -132,7 +172,7 @@ static int init_locked(int device) {
         set_last_message("no HIP device visible");
         return TBJ_ERR_NO_DEVICE;
     }
-    if (device < 0 || device >= n) return TBJ_ERR_INVALID;
+    if (device < 0 || device >= n || device >= kMaxDevices) return TBJ_ERR_INVALID;
     // another device than before: the old device's sessions go
     if (g_dev.ready) release_sessions_locked();
     TBJ_HIP(hipSetDevice(device));
@@ -141,6 +181,22 @@ static int init_locked(int device) {
     g_dev.device = device;
     g_dev.num_cus = prop.multiProcessorCount;
     if (!g_dev.own_stream) TBJ_HIP(hipStreamCreateWithFlags(&g_dev.own_stream, hipStreamNonBlocking));
+    // this device's copy of the no-allocation scratch (device current)
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        TBJ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) {
+            set_last_message("hipGetSymbolAddress returned no address for the scratch");
+            return TBJ_ERR_HIP;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    if (!g_noalloc[device]) {
+        std::unique_ptr<Context> c(new Context());
+        TBJ_TRY(c->setup(g_dev.num_cus));
+        c->ws.set_arena(g_scratch_ptr[device], kNoAllocBytes);
+        g_noalloc[device] = std::move(c);
+    }
     g_dev.ready = true;
     return TBJ_OK;
 }
@@ -278,6 +334,35 @@ int tbj_join(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint
     Context* C = nullptr;
     TBJ_TRY(session(stream, &C));
     std::lock_guard<std::mutex> g(C->mu);
+    return dispatch(*C, algo, R, nR, S, nS, pairs, pairs_cap, o, res, (hipStream_t)stream);
+}
+
+int tbj_join_noalloc(int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
+                     tbj_pair* pairs, uint64_t pairs_cap, const tbj_options* opt,
+                     tbj_result* res, void* stream) {
+    tbj_options o;
+    if (opt) o = *opt;
+    else tbj_default_options(&o);
+    TBJ_TRY(check_args(algo, R, nR, S, nS, pairs, pairs_cap, o, res));
+    if (!radix_family(algo)) return TBJ_ERR_UNSUPPORTED;
+    if (nR + nS > (uint64_t)TBJ_NOALLOC_MAX_TUPLES) {
+        set_last_message("tbj_join_noalloc: nR + nS above TBJ_NOALLOC_MAX_TUPLES");
+        return TBJ_ERR_TOO_LARGE;
+    }
+    // sample-sized regions need more room than exact partitions
+    o.sampled_regions = -1;
+    o.workspace_cap = 0;
+    Context* C = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_mu);
+        if (!g_dev.ready) {
+            set_last_message("tbj_join_noalloc: call tbj_init first");
+            return TBJ_ERR_NO_DEVICE;
+        }
+        C = g_noalloc[g_dev.device].get();
+    }
+    std::lock_guard<std::mutex> g(C->mu);
+    C->ws.reset();
     return dispatch(*C, algo, R, nR, S, nS, pairs, pairs_cap, o, res, (hipStream_t)stream);
 }
 

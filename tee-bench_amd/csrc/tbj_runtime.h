@@ -17,9 +17,12 @@ struct Buf {
     size_t bytes = 0;
 };
 
-// Device workspace of one session: buffers are grown on demand with
-// hipMalloc and kept between calls, keyed by name; `cap` (bytes, 0 = none)
-// bounds the total and turns an over-cap request into TBJ_ERR_OOM.
+// Device workspace of one session.  Normally buffers are grown on demand
+// with hipMalloc and kept between calls, keyed by name; `cap` (bytes, 0 =
+// none) bounds the total and turns an over-cap request into TBJ_ERR_OOM.
+// In arena mode (set_arena) every request is carved from one fixed region
+// instead, front to back, and nothing is allocated: reset() starts a call
+// over, and a request past the region's end returns TBJ_ERR_TOO_LARGE.
 class Workspace {
 public:
     int get(const char* name, size_t bytes, void** out);
@@ -27,9 +30,29 @@ public:
     size_t total() const { return total_; }
     size_t cap = 0;
 
+    void set_arena(void* base, size_t bytes) {
+        arena_ = (uint8_t*)base;
+        arena_bytes_ = bytes;
+        arena_used_ = 0;
+        arena_n_ = 0;
+    }
+    bool arena() const { return arena_ != nullptr; }
+    void reset() {
+        arena_used_ = 0;
+        arena_n_ = 0;
+    }
+
 private:
     std::unordered_map<std::string, Buf> bufs_;
     size_t total_ = 0;
+    // arena mode: fixed table of the current call's named buffers (no host
+    // allocation either)
+    static constexpr int kArenaNames = 64;
+    uint8_t* arena_ = nullptr;
+    size_t arena_bytes_ = 0, arena_used_ = 0;
+    int arena_n_ = 0;
+    const char* arena_name_[kArenaNames] = {};
+    Buf arena_buf_[kArenaNames];
 };
 
 // Process-wide state of the device the library runs on.

@@ -94,6 +94,8 @@ def lib():
         L.tbj_join3.argtypes = [ctypes.c_int, vp, u64, vp, u64, vp, u64, vp, u64,
                                 ctypes.POINTER(Options), ctypes.POINTER(Result), vp]
         L.tbj_join3.restype = ctypes.c_int
+        L.tbj_join_noalloc.argtypes = L.tbj_join.argtypes
+        L.tbj_join_noalloc.restype = ctypes.c_int
         L.tbj_release_stream.argtypes = [vp]
         L.tbj_release_stream.restype = ctypes.c_int
         _lib = L

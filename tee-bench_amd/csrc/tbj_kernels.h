@@ -70,10 +70,12 @@ struct ClaimSpec {
 // fill most of a wave: lanes sharing a bin then take their ranks from one LDS
 // atomic (in a first pass of two, whose tiles mix the whole key range, the
 // plain per-tuple atomics are cheaper).
+// range_out (optional; {UINT_MAX, 0} beforehand): the exact key range of
+// the launch's tuples.
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims = nullptr);
+                    const ClaimSpec* claims = nullptr, uint32_t* range_out = nullptr);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
@@ -90,7 +92,8 @@ void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
 // Sorts the m sampled keys in place (one workgroup).
 void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m);
-// The model from the sorted sample and R's exact key range.
+// The model from the sorted sample and R's exact key range (minmax), or the
+// sample's range when minmax is NULL.
 void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nseg,
                 uint32_t bits, const uint32_t* minmax, CdfModel* model);
 void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax);
@@ -99,10 +102,15 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 // tile row the sampled counts of each first-pass bin (cnt1[pbase(rel) + row *
 // F1 + b1], pbase(S) = F1 * nrowR) and per relation the sampled (b1, b2)
 // counts (fine, 2 * F1 * F2 words, zeroed; F1 * F2 <= 65536).
+// which: 0 = R's rows, 1 = S's rows, 2 = both.  The second-pass bin comes
+// from the same evaluation of the partition function as the first-pass bin
+// (bs2 contributes its pass shift and mask only).
 void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
                         uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
                         const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
-                        uint32_t* cnt1, uint32_t* fine);
+                        uint32_t* cnt1, uint32_t* fine, int which = 2);
+// a[i] += *base for i < n.
+void launch_add_base(hipStream_t st, uint32_t* a, uint32_t n, const uint32_t* base);
 // caps[i] from 16 * cnt[i] for i < n, caps[n] = 0 (scan to region starts).
 void launch_row_caps(hipStream_t st, const uint32_t* cnt, uint32_t n, uint32_t* caps);
 // Exact first-pass partition sizes (2 * F1 words) from the filled regions;

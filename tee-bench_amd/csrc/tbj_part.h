@@ -130,8 +130,11 @@ struct BinSpec {
     uint32_t pass_shift;
     uint32_t pass_mask;
     const CdfModel* model;
-    uint32_t drop;        // 1: drop keys outside [model->kmin, model->kmax]
+    uint32_t drop;        // 1: drop keys outside R's key range
     uint32_t pad;
+    // R's exact key range {min, max} in device memory for the drop test
+    // (NULL: the model's [kmin, kmax])
+    const uint32_t* range;
 };
 
 }  // namespace tbj

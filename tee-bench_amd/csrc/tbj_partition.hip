@@ -35,27 +35,74 @@ struct BinFn {
     BinSpec bs;
     const LdsModel* lm;
     uint32_t kmin, kmax, cshift, ncells, last;
+    uint32_t dlo, dhi;  // keys outside [dlo, dhi] are dropped (bs.drop)
+
+    // The key's partition among all 2^bits (both passes), or TBJ_DROP.
+    __device__ __forceinline__ uint32_t full(uint32_t key) const {
+        if (bs.kind == kBinRadix) return bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
+        if (bs.drop && (key < dlo || key > dhi)) return TBJ_DROP;
+        if (key < kmin) return 0;
+        const uint32_t d = key - kmin, c = d >> cshift;
+        if (c >= ncells) return last;
+        const uint32_t y0 = lm->cells[c], y1 = lm->cells[c + 1];
+        const uint32_t off = d & ((1u << cshift) - 1);
+        return (y0 + (uint32_t)(((uint64_t)off * (y1 - y0)) >> cshift)) >> 8;
+    }
 
     __device__ __forceinline__ uint32_t operator()(uint32_t key) const {
-        uint32_t full;
+        const uint32_t f = full(key);
+        return f == TBJ_DROP ? TBJ_DROP : (f >> bs.pass_shift) & bs.pass_mask;
+    }
+
+    // full() of N keys at once (TBJ_DROP for dropped keys and for items with
+    // valid[q] false).  The learned model's cell reads of all N keys are
+    // issued before any is used (global loads when GMODEL), so their round
+    // trips overlap; branch-free, the same values as full().
+    template <bool GMODEL, int N>
+    __device__ __forceinline__ void fulls(const uint32_t (&key)[N], const bool (&valid)[N],
+                                          uint32_t (&out)[N]) const {
         if (bs.kind == kBinRadix) {
-            full = bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
-        } else {
-            if (bs.drop && (key < kmin || key > kmax)) return TBJ_DROP;
-            if (key < kmin) {
-                full = 0;
+#pragma unroll
+            for (int q = 0; q < N; q++) out[q] = valid[q] ? full(key[q]) : TBJ_DROP;
+            return;
+        }
+        typedef __attribute__((address_space(1))) const uint32_t gu32;
+        const uint32_t cmask = (1u << cshift) - 1;
+        uint32_t c[N], y0[N], y1[N];
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const uint32_t d = key[q] - kmin;
+            c[q] = key[q] < kmin ? 0 : min(d >> cshift, ncells - 1);
+        }
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            if (GMODEL) {
+                gu32* g = (gu32*)lm->cells;
+                y0[q] = g[c[q]];
+                y1[q] = g[c[q] + 1];
             } else {
-                const uint32_t d = key - kmin, c = d >> cshift;
-                if (c >= ncells) {
-                    full = last;
-                } else {
-                    const uint32_t y0 = lm->cells[c], y1 = lm->cells[c + 1];
-                    const uint32_t off = d & ((1u << cshift) - 1);
-                    full = (y0 + (uint32_t)(((uint64_t)off * (y1 - y0)) >> cshift)) >> 8;
-                }
+                y0[q] = lm->cells[c[q]];
+                y1[q] = lm->cells[c[q] + 1];
             }
         }
-        return (full >> bs.pass_shift) & bs.pass_mask;
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            const uint32_t k = key[q], d = k - kmin;
+            uint32_t f = (y0[q] + (uint32_t)(((uint64_t)(d & cmask) * (y1[q] - y0[q])) >> cshift)) >> 8;
+            f = k < kmin ? 0 : (d >> cshift) >= ncells ? last : f;
+            const bool drop = !valid[q] || (bs.drop && (k < dlo || k > dhi));
+            out[q] = drop ? TBJ_DROP : f;
+        }
+    }
+
+    // The pass's bins of N keys (fulls, then this pass's bits).
+    template <bool GMODEL, int N>
+    __device__ __forceinline__ void bins(const uint32_t (&key)[N], const bool (&valid)[N],
+                                         uint32_t (&out)[N]) const {
+        fulls<GMODEL>(key, valid, out);
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            out[q] = out[q] == TBJ_DROP ? TBJ_DROP : (out[q] >> bs.pass_shift) & bs.pass_mask;
     }
 };
 
@@ -70,7 +117,7 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     f.bs = bs;
     f.lm = GMODEL && bs.kind == kBinLearned ? reinterpret_cast<const LdsModel*>(bs.model->cells)
                                             : lm;
-    f.kmin = f.kmax = f.cshift = f.ncells = f.last = 0;
+    f.kmin = f.kmax = f.cshift = f.ncells = f.last = f.dlo = f.dhi = 0;
     if (bs.kind == kBinLearned) {
         if (!GMODEL) load_model(lm, bs.model);
         f.kmin = bs.model->kmin;
@@ -78,6 +125,8 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
         f.cshift = bs.model->cshift;
         f.ncells = bs.model->ncells;
         f.last = bs.bits >= 32 ? 0xFFFFFFFFu : (1u << bs.bits) - 1;
+        f.dlo = bs.range ? bs.range[0] : f.kmin;
+        f.dhi = bs.range ? bs.range[1] : f.kmax;
     }
     return f;
 }
@@ -285,17 +334,23 @@ struct Claims {
 // round trip sits between the tile's load and its writes.
 constexpr int kScatterBinsPerThread = kMaxBins / kPartBlock;
 
+// range_out (optional, {UINT_MAX, 0} beforehand) receives the exact key
+// range of the launch's tuples (one filtered atomic pair per workgroup).
+// Three workgroups per CU (six waves per SIMD) is what the LDS stage allows;
+// the register budget is held to it.
 template <bool GMODEL, bool AGG, bool CLAIM>
-__global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict__ in,
+__global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles_dev,
                                                         uint32_t tile_base, uint32_t tile_end,
                                                         const uint32_t* __restrict__ offs,
-                                                        Claims cl, BinSpec bs, uint32_t nbins) {
+                                                        Claims cl, BinSpec bs, uint32_t nbins,
+                                                        uint32_t* __restrict__ range_out) {
     __shared__ uint2 stage[kTile];
     __shared__ uint16_t sbin[kTile];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t wrange[2][kPartBlock / TBJ_WAVE];  // range_out only
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
     uint32_t* loff = reinterpret_cast<uint32_t*>(
@@ -339,16 +394,53 @@ __global__ __launch_bounds__(kPartBlock) void k_scatter(const uint2* __restrict_
             }
         }
         __syncthreads();
+        {
+            uint32_t keys[kPartItems];
+            bool ok[kPartItems];
+#pragma unroll
+            for (uint32_t k = 0; k < kPartItems; k++) {
+                keys[k] = v[k].x;
+                ok[k] = k * kPartBlock + threadIdx.x < td.len;
+            }
+            bin.template bins<GMODEL>(keys, ok, b);
+        }
 #pragma unroll
         for (uint32_t k = 0; k < kPartItems; k++) {
-            const uint32_t j = k * kPartBlock + threadIdx.x;
-            if (j < td.len) b[k] = bin(v[k].x);
             if (AGG)
                 rank[k] = agg_inc(loff, b[k]);
             else if (b[k] != TBJ_DROP)
                 rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
+        if (range_out) {
+            uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kPartItems; k++) {
+                if (k * kPartBlock + threadIdx.x < td.len) {
+                    mn = min(mn, v[k].x);
+                    mx = max(mx, v[k].x);
+                }
+            }
+            for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+                mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
+                mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
+            }
+            if (lane_id() == 0) {
+                wrange[0][wave_id()] = mn;
+                wrange[1][wave_id()] = mx;
+            }
+        }
         __syncthreads();
+        if (range_out && threadIdx.x == 0) {
+            uint32_t mn = 0xFFFFFFFFu, mx = 0;
+            for (int w = 0; w < kPartBlock / TBJ_WAVE; w++) {
+                mn = min(mn, wrange[0][w]);
+                mx = max(mx, wrange[1][w]);
+            }
+            // the stored range only narrows toward the result: a stale read
+            // can only let an unneeded atomic through
+            if (mn < range_out[0]) atomicMin(&range_out[0], mn);
+            if (mx > range_out[1]) atomicMax(&range_out[1], mx);
+        }
         // claims: one atomic per non-empty bin, issued now, awaited after the
         // scan and the staging
         uint32_t cnt[kScatterBinsPerThread], base[kScatterBinsPerThread];
@@ -408,19 +500,23 @@ __device__ __forceinline__ uint32_t sampled_group(uint32_t block) {
 // row's sampled counts per first-pass bin at cnt1[pbase[rel] + row * F1 +
 // b1] (the first-pass regions are laid out row by row, so that a tile's
 // claims touch consecutive words); fine (zeroed, 2 * F1 * F2 words)
-// accumulates (rel, b1, b2) counts.  A chunk's (b1, b2) counts live in LDS as
-// 16-bit halves of 32-bit words: a chunk holds at most kSampleTiles * kTile /
-// 16 < 2^16 sampled tuples.
+// accumulates (rel, b1, b2) counts from every kFineEvery-th chunk of a row (they
+// only apportion each exact first-pass partition size among its second-pass
+// bins, so a quarter of the sample serves, and the other chunks skip the
+// 128 KB LDS histogram).  A chunk's (b1, b2) counts live in LDS as 16-bit
+// halves of 32-bit words: a chunk holds at most kSampleTiles * kTile / 16 <
+// 2^16 sampled tuples.
+constexpr uint32_t kFineEvery = 4;
 constexpr uint32_t kFineLds = 1u << 16;  // F1 * F2 at most
 constexpr int kSampleBlock = 512;
 
 __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint2* __restrict__ R, uint32_t nR, const uint2* __restrict__ S, uint32_t nS,
-    uint32_t nrowR, uint32_t nrowS, BinSpec bsr, BinSpec bss, BinSpec bs2, uint32_t F1,
-    uint32_t F2, uint32_t* __restrict__ cnt1, uint32_t* __restrict__ fine) {
+    uint32_t nrowR, uint32_t nrowS, uint32_t row0, BinSpec bsr, BinSpec bss, BinSpec bs2,
+    uint32_t F1, uint32_t F2, uint32_t* __restrict__ cnt1, uint32_t* __restrict__ fine) {
     __shared__ uint32_t h[kFineLds / 2];
     __shared__ uint32_t c1[kMaxBins];
-    const uint32_t rowg = blockIdx.x / kChunksPerRow, chunk = blockIdx.x % kChunksPerRow;
+    const uint32_t rowg = row0 + blockIdx.x / kChunksPerRow, chunk = blockIdx.x % kChunksPerRow;
     const uint32_t rel = rowg < nrowR ? 0 : 1;
     const uint32_t row = rel ? rowg - nrowR : rowg;
     const uint2* in = rel ? S : R;
@@ -433,27 +529,39 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint32_t t0 = min(r0 + chunk * kSampleTiles, r1), t1 = min(t0 + kSampleTiles, r1);
     if (t0 == t1) return;  // whole workgroup
     const uint32_t nf = F1 * F2;
-    for (uint32_t i = threadIdx.x; i < nf / 2 + 1 && i < kFineLds / 2; i += kSampleBlock) h[i] = 0;
+    const bool do_fine = chunk % kFineEvery == 0;  // chunk 0 of every row
+    if (do_fine)
+        for (uint32_t i = threadIdx.x; i < nf / 2 + 1 && i < kFineLds / 2; i += kSampleBlock) h[i] = 0;
     for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock) c1[i] = 0;
     __syncthreads();
+    // one evaluation of the partition function per key gives both passes'
+    // bins (and the drop decision of the first pass)
     BinFn b1f = make_binfn<true>(rel ? bss : bsr, nullptr);
-    BinFn b2f = make_binfn<true>(bs2, nullptr);
+    const uint32_t s1 = b1f.bs.pass_shift, m1 = b1f.bs.pass_mask;
+    const uint32_t s2 = bs2.pass_shift, m2 = bs2.pass_mask;
     const uint32_t lo = t0 * kTile, hi = min(t1 * kTile, n);
     // blocks of 16 groups = 256 tuples; one thread per block
     const uint32_t blo = lo / 256, bhi = (hi + 255) / 256;
     for (uint32_t blk = blo + threadIdx.x; blk < bhi; blk += kSampleBlock) {
         const uint32_t first = blk * 256 + sampled_group(blk) * 16;
-#pragma unroll 4
+        uint32_t key[16], full[16];
+        bool ok[16];
+#pragma unroll
         for (uint32_t k = 0; k < 16; k++) {
             const uint32_t i = first + k;
-            if (i >= lo && i < hi) {
-                const uint32_t key = in[i].x;
-                const uint32_t b1 = b1f(key), b2 = b2f(key);
-                if (b1 != TBJ_DROP && b2 != TBJ_DROP) {
-                    const uint32_t f = b1 * F2 + b2;
+            ok[k] = i >= lo && i < hi;
+            key[k] = in[ok[k] ? i : lo].x;
+        }
+        b1f.template fulls<true>(key, ok, full);
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            if (full[k] != TBJ_DROP) {
+                const uint32_t b1 = (full[k] >> s1) & m1;
+                if (do_fine) {
+                    const uint32_t f = b1 * F2 + ((full[k] >> s2) & m2);
                     atomicAdd(&h[f >> 1], 1u << ((f & 1) * 16));
-                    atomicAdd(&c1[b1], 1u);
                 }
+                atomicAdd(&c1[b1], 1u);
             }
         }
     }
@@ -461,6 +569,7 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint32_t pbase = rel ? F1 * nrowR : 0;
     for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock)
         if (c1[i]) atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
+    if (!do_fine) return;
     uint32_t* fr = fine + (uint64_t)rel * nf;
     for (uint32_t f = threadIdx.x; f < nf; f += kSampleBlock) {
         const uint32_t v = (h[f >> 1] >> ((f & 1) * 16)) & 0xFFFFu;
@@ -785,9 +894,10 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
     for (uint32_t j = threadIdx.x; j < nseg_req; j += kFitBlock)
         quant[j] = sorted[(uint64_t)j * m / nseg_req];
     __syncthreads();
-    const uint32_t kmin = minmax[0], kmax = minmax[1];
+    // the model's key range: R's exact one, or without it the sample's
+    const uint32_t kmin = minmax ? minmax[0] : sorted[0], kmax = minmax ? minmax[1] : sorted[m - 1];
     if (threadIdx.x == 0) {
-        // first knot at R's exact minimum, then distinct sample quantiles
+        // first knot at the range's minimum, then distinct sample quantiles
         uint32_t nk = 0;
         knots[nk++] = kmin;
         for (uint32_t j = 1; j < nseg_req; j++) {
@@ -1012,7 +1122,7 @@ void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDes
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims) {
+                    const ClaimSpec* claims, uint32_t* range_out) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
@@ -1037,7 +1147,7 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
         k = learned ? (last_pass ? k_scatter<true, true, false> : k_scatter<true, false, false>)
                     : (last_pass ? k_scatter<false, true, false> : k_scatter<false, false, false>);
     hipLaunchKernelGGL(k, dim3(ntiles), dim3(kPartBlock), lds, st, in, out, tiles, ntiles_dev,
-                       tile_base, tile_base + ntiles, offs, cl, bs, nbins);
+                       tile_base, tile_base + ntiles, offs, cl, bs, nbins, range_out);
 }
 
 void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,
@@ -1134,10 +1244,21 @@ void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t
 void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
                         uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
                         const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
-                        uint32_t* cnt1, uint32_t* fine) {
-    if (nrowR + nrowS == 0) return;
-    hipLaunchKernelGGL(k_sample_rows, dim3((nrowR + nrowS) * kChunksPerRow), dim3(kSampleBlock), 0,
-                       st, R, nR, S, nS, nrowR, nrowS, bsr, bss, bs2, F1, F2, cnt1, fine);
+                        uint32_t* cnt1, uint32_t* fine, int which) {
+    const uint32_t row0 = which == 1 ? nrowR : 0;
+    const uint32_t rows = which == 0 ? nrowR : which == 1 ? nrowS : nrowR + nrowS;
+    if (rows == 0) return;
+    hipLaunchKernelGGL(k_sample_rows, dim3(rows * kChunksPerRow), dim3(kSampleBlock), 0, st, R, nR,
+                       S, nS, nrowR, nrowS, row0, bsr, bss, bs2, F1, F2, cnt1, fine);
+}
+
+__global__ void k_add_base(uint32_t* __restrict__ a, uint32_t n, const uint32_t* __restrict__ base) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += *base;
+}
+
+void launch_add_base(hipStream_t st, uint32_t* a, uint32_t n, const uint32_t* base) {
+    if (n) hipLaunchKernelGGL(k_add_base, dim3((n + 255) / 256), dim3(256), 0, st, a, n, base);
 }
 
 void launch_row_caps(hipStream_t st, const uint32_t* cnt, uint32_t n, uint32_t* caps) {

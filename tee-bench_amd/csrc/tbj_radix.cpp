@@ -134,9 +134,13 @@ struct Call {
     uint32_t npieces;  // ranges per partition side in bounds (1, or 8 per-XCD pieces)
 };
 
-// The learned model (fitted on a strided key sample of R and R's exact key
-// range) and the partition functions of both passes.
-int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2) {
+// The learned model, fitted on a strided key sample of R, and the partition
+// functions of both passes.  exact_range: the model spans R's exact key range
+// (a pass over R, beside the sample's sort); otherwise it spans the sample's
+// range, and R's exact range -- which decides what S drops -- is measured by
+// R's first scatter into sm->minmax.  Either way the drop test reads
+// sm->minmax.
+int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_range) {
     const PassPlan& plan = k.plan;
     if (!k.learned) {
         *s1r = *s1s = radix_spec(plan, 1);
@@ -155,20 +159,29 @@ int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2) {
     TBJ_TRY(k.W.get("sample", sizeof(uint32_t) * m, (void**)&sample));
     // 64 samples per segment: knot noise of about 1/8 of a segment
     const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kLdsKnots, m / 64));
+    // {UINT_MAX, 0}: the start of both ways of measuring R's range
     launch_fill_u32(k.st, k.sm->minmax, 1, 0xFFFFFFFFu);
-    // the sample and its sort (one workgroup) run on the side stream while
-    // the exact key range reads all of R
-    TBJ_HIP(hipEventRecord(C.fork_ev, k.st));
-    TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
-    launch_sample(C.side_stream, k.J.R, nR, m, sample);
-    launch_fit_sort(C.side_stream, sample, m);
-    TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
-    launch_minmax(k.st, k.J.R, nR, k.sm->minmax);
-    TBJ_HIP(hipStreamWaitEvent(k.st, C.join_ev, 0));
-    launch_fit(k.st, sample, m, nseg, plan.B, k.sm->minmax, model);
+    launch_fill_u32(k.st, k.sm->minmax + 1, 1, 0);
+    if (exact_range) {
+        // the sample and its sort (one workgroup) run on the side stream while
+        // the exact key range reads all of R
+        TBJ_HIP(hipEventRecord(C.fork_ev, k.st));
+        TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
+        launch_sample(C.side_stream, k.J.R, nR, m, sample);
+        launch_fit_sort(C.side_stream, sample, m);
+        TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
+        launch_minmax(k.st, k.J.R, nR, k.sm->minmax);
+        TBJ_HIP(hipStreamWaitEvent(k.st, C.join_ev, 0));
+        launch_fit(k.st, sample, m, nseg, plan.B, k.sm->minmax, model);
+    } else {
+        launch_sample(k.st, k.J.R, nR, m, sample);
+        launch_fit_sort(k.st, sample, m);
+        launch_fit(k.st, sample, m, nseg, plan.B, nullptr, model);
+    }
     *s1r = learned_spec(plan, 1, model, 0);
     *s1s = learned_spec(plan, 1, model, 1);
     *s2 = learned_spec(plan, 2, model, 1);
+    s1s->range = s2->range = k.sm->minmax;
     return TBJ_OK;
 }
 
@@ -217,7 +230,7 @@ int partition_exact(Call& k) {
                   sizeof(uint32_t) * scan_partials(std::max({L1, L2max, (uint64_t)k.nparts + 1})),
                   (void**)&partial));
     BinSpec s1r, s1s, s2;
-    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2));
+    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, true));
 
     // ---- pass 1: histogram, scan, scatter
     launch_make_tiles1(st, tiles1, segs1, nR, nS, plan.F1, &sm->ntiles1);
@@ -343,14 +356,17 @@ int partition_sampled(Call& k) {
     TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(std::max<uint64_t>(np1, nsub8) + 1),
                   (void**)&partial));
     BinSpec s1r, s1s, s2;
-    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2));
-
-    // ---- sample: first-pass row counts and (b1, b2) counts
+    TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, false));
+    const uint32_t nR1 = F1 * nrowR;  // R's first-pass regions (S's follow)
     launch_make_tiles1(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
     launch_fill_u32(st, fine, nsub, 0);
     launch_fill_u32(st, cnt1, np1, 0);
-    launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1, fine);
+    launch_fill_u32(st, cursors1, np1, 0);
+    launch_fill_u32(st, cursors2, nsub8, 0);
     if (k.learned) {
+        // R first: its sample, then the judge on R's estimated partitions
+        launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
+                           fine, 0);
         launch_est_part_max(st, cnt1, F1, nrowR, sm->stats);
         bool fb = false;
         TBJ_TRY(judge_learned(k, &fb));
@@ -362,22 +378,41 @@ int partition_sampled(Call& k) {
             s2 = radix_spec(plan, 2);
             launch_fill_u32(st, fine, nsub, 0);
             launch_fill_u32(st, cnt1, np1, 0);
-            launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2,
-                               cnt1, fine);
         }
     }
-    TBJ_TRY(k.pt.mark(kPhHist1));
-    launch_row_caps(st, cnt1, np1, starts1);
-    launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
-    launch_fill_u32(st, cursors1, np1, 0);
-    launch_fill_u32(st, cursors2, nsub8, 0);
-    TBJ_TRY(k.pt.mark(kPhScan1));
-
-    // ---- pass 1: claims per (tile, bin) in the tile row's regions
-    ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
-    ClaimSpec c1s{starts1, cursors1, &sm->overflow, F1 * nrowR, F1, 1, 0};
-    launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
-    launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false, &c1s);
+    if (!k.learned) {
+        // ---- sample both relations, size the first pass's regions
+        launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
+                           fine, 2);
+        TBJ_TRY(k.pt.mark(kPhHist1));
+        launch_row_caps(st, cnt1, np1, starts1);
+        launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
+        TBJ_TRY(k.pt.mark(kPhScan1));
+        // ---- pass 1: claims per (tile, bin) in the tile row's regions
+        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
+        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0};
+        launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
+        launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
+                       &c1s);
+    } else {
+        // R's regions and scatter, which measures R's exact key range; then
+        // S's sample, regions and scatter, which drop S keys outside it
+        TBJ_TRY(k.pt.mark(kPhHist1));
+        launch_row_caps(st, cnt1, nR1, starts1);
+        launch_scan(st, starts1, (uint64_t)nR1 + 1, partial, &sm->region_total);
+        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
+        launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r,
+                       sm->minmax);
+        launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
+                           fine, 1);
+        launch_row_caps(st, cnt1 + nR1, np1 - nR1, starts1 + nR1);
+        launch_scan(st, starts1 + nR1, (uint64_t)(np1 - nR1) + 1, partial, nullptr);
+        launch_add_base(st, starts1 + nR1, np1 - nR1 + 1, &sm->region_total);
+        TBJ_TRY(k.pt.mark(kPhScan1));
+        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0};
+        launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
+                       &c1s);
+    }
     TBJ_TRY(k.pt.mark(kPhScatter1));
 
     // ---- pass 2: every filled first-pass region is a segment of tiles

@@ -216,23 +216,34 @@ int tbj_join64(int algo,
  * per process, App/native.cpp:121-149).  All relations live in one device
  * arena; item i names R_i = arena[r_off, r_off + nR) and S_i = arena[s_off,
  * s_off + nS).  counts (device, n entries) receives each join's match count.
- * Count-only RHO semantics (algo TBJ_RHO; others: TBJ_ERR_UNSUPPORTED).
- * The batch is processed in waves of at most 2^31 partitioned tuples; each
- * item's R and S must lie within one window of 2^32 - 2^20 tuples.
+ * Count-only joins of two kinds: TBJ_RHO (radix partitioning, then LDS hash
+ * tables per partition: radix_join.c:1557-1561) and TBJ_PHT (one shared
+ * hash table per join in HBM: no_partitioning_join.h:543-623); both count
+ * the exact equi-join.  tbj_batch_create's algo is TBJ_RHO or TBJ_PHT for
+ * every item, or TBJ_BATCH_PER_ITEM to take each item's own algo field
+ * (other values: TBJ_ERR_UNSUPPORTED).
+ * The batch is processed in waves of at most 2^31 tuples; each item's R and
+ * S must lie within one window of 2^32 - 2^20 tuples.
  *
  * A plan fixes the launch sequence for the batch's sizes; tbj_batch_run
  * launches it (no host synchronisation inside) and, after
  * tbj_batch_capture, replays it as one HIP graph.  The arena contents may
  * change between runs; the sizes and offsets may not.
  */
-typedef struct tbj_batch_item { uint64_t r_off, nR, s_off, nS; } tbj_batch_item;
+typedef struct tbj_batch_item {
+    uint64_t r_off, nR, s_off, nS;
+    int32_t algo;               /* TBJ_RHO or TBJ_PHT, read under TBJ_BATCH_PER_ITEM */
+    uint32_t reserved;
+} tbj_batch_item;
+#define TBJ_BATCH_PER_ITEM (-1)
 typedef struct tbj_batch_plan tbj_batch_plan;
 
 typedef struct tbj_batch_tuning {
     uint32_t target_r;          /* average R tuples per partition; 0 = 2048 */
     uint32_t one_pass_bits;     /* most partition bits done in one pass; 0 = 8 */
     uint32_t s_chunk;           /* S tuples per join task (probed 8192 at a time); 0 = 32768 */
-    uint32_t wave_tuples;       /* partitioned tuples per wave; 0 = 2^31 */
+    uint32_t wave_tuples;       /* tuples per wave; 0 = 2^31 */
+    uint32_t tile;              /* tuples per partition tile: 2048, 4096 or 8192; 0 = 4096 */
 } tbj_batch_tuning;
 
 typedef struct tbj_batch_result {
@@ -250,8 +261,11 @@ int tbj_batch_run(tbj_batch_plan* plan, void* stream, tbj_batch_result* res);
  * the library's own when stream is NULL); later runs replay the graph. */
 int tbj_batch_capture(tbj_batch_plan* plan, void* stream);
 void tbj_batch_destroy(tbj_batch_plan* plan);
-/* Times candidate tunings (target_r, one_pass_bits, s_chunk) on the batch
- * itself (one warm and `reps` timed runs each) and returns the fastest. */
+/* Times candidate tunings on the batch itself (one warm and `reps` timed
+ * runs each) and returns the fastest: first the tile shape (2048, 4096 or
+ * 8192 tuples) at the default radix plan, then, at that tile, target_r
+ * (which sets each join's radix bits) x one_pass_bits (which sets the pass
+ * count) x s_chunk (the join's task shape).  best->wave_tuples is kept. */
 int tbj_batch_autotune(int algo, const tbj_tuple* arena, const tbj_batch_item* items, uint32_t n,
                        uint64_t* counts, int reps, tbj_batch_tuning* best, float* best_ms);
 

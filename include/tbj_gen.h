@@ -75,7 +75,8 @@ int tbj_gen_zipf_fk64(uint64_t seed, uint64_t n, const tbj_tuple64* R, uint64_t 
  * s_i = splitmix64(base_seed ^ i) and from its splitmix64 stream draws
  *   u ~ U[0,1): |R_i| = floor(2^(10 + 12u));
  *   k ~ U{1..16}: |S_i| = k * |R_i|;
- *   z ~ 0.1 * U{0..15} (so z = 0, a uniform foreign key, has weight 1/16).
+ *   z ~ U[0, 1.5) (1.5 times a 53-bit uniform);
+ *   a ~ U{0, 1}: join i is TBJ_RHO (a = 0) or TBJ_PHT (a = 1).
  * R_i: keys 1..|R_i| in a pseudo-random order (a Feistel permutation of the
  * row index, seeded by s_i).  S_i: keys in 1..|R_i|, uniform for z = 0 and
  * Zipf(z) ranks otherwise (rejection-inversion sampling, Hormann and
@@ -88,6 +89,8 @@ typedef struct tbj_batch_spec {
     uint64_t nR, nS;
     double z;
     uint64_t r_off, s_off;  /* tuple offsets in the arena */
+    int32_t algo;           /* TBJ_RHO or TBJ_PHT */
+    uint32_t reserved;
 } tbj_batch_spec;
 
 /* Fills specs[0..n) for joins of base_seed (offsets: R_0, S_0, R_1, S_1, ...

@@ -9,6 +9,7 @@
 //   ref_gen_from_file(seed, path, ...)                          -> keys only
 //   ref_join_count(name, ...)                                  -> matches
 //   ref_join_pairs(...)   (JOIN_MATERIALIZE build only)         -> pairs
+//   ref_join_mat_timed(...) (JOIN_MATERIALIZE build only)       -> matches, join time
 //   ref_rand_stream(seed, n, out)                              -> glibc rand()
 #include <stdint.h>
 #include <stdlib.h>
@@ -247,6 +248,44 @@ int64_t ref_join_pairs(const char* name,
     free(res->resultlist);
     free(res);
     return rc < 0 ? rc : (int64_t)k;
+}
+
+// The materialising join's own time: RHO / RHT with JOIN_MATERIALIZE (one
+// output-list node per match, built inside the join), timed around the join
+// call alone; the lists are then walked and freed, untimed.  Returns the
+// matches the lists hold, or -1.
+int64_t ref_join_mat_timed(const char* name,
+                           const uint32_t* rkeys, const uint32_t* rpay, uint64_t nr,
+                           const uint32_t* skeys, const uint32_t* spay, uint64_t ns,
+                           int nthreads, int quiet, double* elapsed_us_out) {
+    StdoutSilencer s(quiet);
+    clock_gettime(CLOCK_MONOTONIC, &ts_start);
+    table_t R = make_rel(rkeys, rpay, nr);
+    table_t S = make_rel(skeys, spay, ns);
+    result_t* res = nullptr;
+    struct timespec t0, t1;
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    if (!strcmp(name, "RHO")) res = RHO(&R, &S, nthreads);
+    else if (!strcmp(name, "RHT")) res = RHT(&R, &S, nthreads);
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (elapsed_us_out)
+        *elapsed_us_out = (t1.tv_sec - t0.tv_sec) * 1e6 + (t1.tv_nsec - t0.tv_nsec) * 1e-3;
+    free(R.tuples);
+    free(S.tuples);
+    if (!res) return -1;
+    int64_t k = 0;
+    for (int t = 0; t < res->nthreads; t++) {
+        output_list_t* node = res->resultlist[t].results;
+        for (int64_t i = 0; i < res->resultlist[t].nresults; i++) {
+            output_list_t* nxt = node->next;
+            free(node);
+            node = nxt;
+            k++;
+        }
+    }
+    free(res->resultlist);
+    free(res);
+    return k;
 }
 #endif
 

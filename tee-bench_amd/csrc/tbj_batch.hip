@@ -1,14 +1,17 @@
 // Batches of joins: partition bounds over many joins at once, a persistent
 // count-only LDS hash join whose tasks are decoded from a per-partition
-// prefix (no task list, no host round trip), and the device generator of
-// the batch workload.
+// prefix (no task list, no host round trip), the PHT joins of a wave built
+// and probed in one launch each, and the device generator of the batch
+// workload.
 //
 // The join semantics are RHO's (radix partition, then a per-partition
-// bucket-chained hash join: Joins/radix_join.c:373-440, 1557-1561); the
-// batch itself has no reference counterpart.
+// bucket-chained hash join: Joins/radix_join.c:373-440, 1557-1561) and PHT's
+// (one shared hash table per join: Joins/no_partitioning_join.h:543-623);
+// the batch itself has no reference counterpart.
 #include "tbj_batch.h"
 #include "tbj_device.h"
 #include "tbj_kernels.h"
+#include "tbj_pht.h"
 
 namespace tbj {
 
@@ -196,6 +199,86 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
     }
 }
 
+// ---------------------------------------------------------------------------
+// PHT joins of a wave.  Chunk c of a launch belongs to the last job whose
+// first chunk is <= c (jobs without chunks share their first chunk with the
+// next job, so the search never stops on one).
+
+__device__ __forceinline__ uint32_t pht_job_of(const PhtJob* __restrict__ jobs, uint32_t njobs,
+                                               uint32_t c, bool probe) {
+    uint32_t lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((probe ? jobs[mid].sc0 : jobs[mid].rc0) <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr int kPhtBlock = 256;
+constexpr int kPhtBuildItems = kPhtBuildChunk / kPhtBlock;  // inserts in flight per thread
+constexpr int kPhtProbeItems = 8;                           // bucket loads in flight per thread
+
+__global__ __launch_bounds__(kPhtBlock) void k_bpht_build(
+    const uint2* __restrict__ arena, const PhtJob* __restrict__ jobs, uint32_t njobs,
+    uint32_t nchunks, PhtBucket* __restrict__ tab, OvNode* __restrict__ pool,
+    uint32_t* __restrict__ pool_ctr) {
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const PhtJob jb = jobs[pht_job_of(jobs, njobs, c, false)];
+        const uint32_t base = (c - jb.rc0) * kPhtBuildChunk + threadIdx.x;
+        PhtBucket* t0 = tab + jb.tab;
+        uint2 t[kPhtBuildItems];
+        uint32_t pos[kPhtBuildItems];
+#pragma unroll
+        for (int k = 0; k < kPhtBuildItems; k++) {
+            const uint32_t i = base + k * kPhtBlock;
+            t[k] = i < jb.nR ? arena[jb.r_off + i] : make_uint2(0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kPhtBuildItems; k++)
+            pos[k] = base + k * kPhtBlock < jb.nR ? atomicAdd(&t0[t[k].x & jb.mask].count, 1u)
+                                                  : ~0u;
+#pragma unroll
+        for (int k = 0; k < kPhtBuildItems; k++)
+            if (pos[k] != ~0u) pht_place(t0 + (t[k].x & jb.mask), t[k], pos[k], pool, pool_ctr);
+    }
+}
+
+__global__ __launch_bounds__(kPhtBlock) void k_bpht_probe(
+    const uint2* __restrict__ arena, const PhtJob* __restrict__ jobs, uint32_t njobs,
+    uint32_t nchunks, const PhtBucket* __restrict__ tab, const OvNode* __restrict__ pool,
+    unsigned long long* __restrict__ counts) {
+    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const PhtJob jb = jobs[pht_job_of(jobs, njobs, c, true)];
+        const PhtBucket* t0 = tab + jb.tab;
+        const uint32_t base = (c - jb.sc0) * kPhtProbeChunk + threadIdx.x;
+        uint32_t m = 0;
+        for (uint32_t r = 0; r < kPhtProbeChunk; r += kPhtBlock * kPhtProbeItems) {
+            uint2 sv[kPhtProbeItems];
+            PhtBucket bk[kPhtProbeItems];
+#pragma unroll
+            for (int k = 0; k < kPhtProbeItems; k++) {
+                const uint32_t i = base + r + k * kPhtBlock;
+                sv[k] = i < jb.nS ? arena[jb.s_off + i] : make_uint2(0, 0);
+            }
+            // every bucket load issued before any is used
+#pragma unroll
+            for (int k = 0; k < kPhtProbeItems; k++) {
+                if (base + r + k * kPhtBlock < jb.nS) {
+                    bk[k] = t0[sv[k].x & jb.mask];
+                } else {
+                    bk[k].count = 0;
+                    bk[k].ovf = 0;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < kPhtProbeItems; k++) m += pht_matches(bk[k], sv[k].x, pool);
+        }
+        const uint64_t wm = wave_reduce_u64(m);
+        if (lane_id() == 0 && wm) atomicAdd(&counts[jb.id], (unsigned long long)wm);
+    }
+}
+
 __global__ void k_fill_u64b(unsigned long long* __restrict__ p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -333,6 +416,24 @@ void batch_join(hipStream_t st, uint32_t grid, const uint2* Rb, const uint2* Sb,
                 const uint32_t* join_ids, unsigned long long* counts) {
     hipLaunchKernelGGL(k_bjoin, dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, bounds, NP, P,
                        task_off, ntasks_dev, task_ctr, s_chunk, hshift, join_ids, counts);
+}
+
+void batch_pht(hipStream_t st, uint32_t grid, const uint2* arena, const PhtJob* jobs,
+               uint32_t njobs, uint32_t nrc, uint32_t nsc, void* tab, uint64_t nbuckets,
+               void* pool, uint32_t* pool_ctr, unsigned long long* counts) {
+    if (!njobs) return;
+    // a bucket is four 8-byte words; kernels, not memsets, so that the
+    // launches can be captured into a graph
+    batch_zero_u64(st, reinterpret_cast<unsigned long long*>(tab), 4 * nbuckets);
+    launch_fill_u32(st, pool_ctr, 1, 0);
+    if (nrc)
+        hipLaunchKernelGGL(k_bpht_build, dim3(min(grid, nrc)), dim3(kPhtBlock), 0, st, arena, jobs,
+                           njobs, nrc, reinterpret_cast<PhtBucket*>(tab),
+                           reinterpret_cast<OvNode*>(pool), pool_ctr);
+    if (nsc)
+        hipLaunchKernelGGL(k_bpht_probe, dim3(min(grid, nsc)), dim3(kPhtBlock), 0, st, arena, jobs,
+                           njobs, nsc, reinterpret_cast<const PhtBucket*>(tab),
+                           reinterpret_cast<const OvNode*>(pool), counts);
 }
 
 void batch_zero_u64(hipStream_t st, unsigned long long* p, uint64_t n) {

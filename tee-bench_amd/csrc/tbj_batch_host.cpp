@@ -1,13 +1,15 @@
 // Batches of joins: the planner (waves of at most tuning.wave_tuples
-// partitioned tuples; inside a wave, one group per radix plan), the launch
-// sequence (no host synchronisation, so it can be captured as one HIP
-// graph), the autotuner and the batch generator's host side.
+// tuples; inside a wave, one group per radix plan of its RHO joins and one
+// group of all its PHT joins), the launch sequence (no host
+// synchronisation, so it can be captured as one HIP graph), the autotuner
+// and the batch generator's host side.
 //
-// A group partitions all of its joins in the same passes: the first pass
-// sees one segment per relation of each join (R_0, S_0, R_1, S_1, ...), the
-// second one segment per (relation, first-pass bin), exactly as one join's
-// second pass does.  Per-join results come from the join kernel, which maps
-// a partition back to its join.
+// A radix group partitions all of its joins in the same passes: the first
+// pass sees one segment per relation of each join (R_0, S_0, R_1, S_1, ...),
+// the second one segment per (relation, first-pass bin), exactly as one
+// join's second pass does.  Per-join results come from the join kernel,
+// which maps a partition back to its join.  A PHT group builds the tables
+// of all of its joins in one launch and probes them in another.
 #include <math.h>
 #include <string.h>
 
@@ -25,7 +27,10 @@ namespace {
 
 constexpr uint64_t kWindow = (1ull << 32) - (1ull << 20);  // u32 tuple offsets inside a wave
 
+enum GroupKind : uint32_t { kGroupRadix = 0, kGroupPht = 1 };
+
 struct Group {
+    uint32_t kind;
     uint32_t B, passes, b1, b2, F1, F2;
     uint32_t nj;
     uint64_t tuples;
@@ -36,6 +41,11 @@ struct Group {
     size_t segs_off;       // byte offsets into the plan's constant block
     size_t ids_off;
     uint32_t small_idx;    // 8 words each: ntiles1, ntiles2, ntasks, task_ctr, stats[2], total1, total2
+                           // (PHT groups: word 0 = the overflow pool counter)
+    // PHT groups
+    size_t jobs_off;
+    uint32_t njobs, nrc, nsc;
+    uint64_t nbuckets, npool;
 };
 
 struct DevBuf {
@@ -88,11 +98,11 @@ struct tbj_batch_plan {
     uint32_t waves = 0;
     uint64_t tuples = 0;
     DevBuf c1, c2, tiles1, hist1, offs1, gs1, tiles2, hist2, offs2, gs2, segs2, bounds, task_off,
-        partial, konst, small, total;
+        partial, konst, small, total, ptab, ppool;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    uint32_t grid = 0;
+    uint32_t grid = 0, pht_grid = 0;
 
     ~tbj_batch_plan() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -106,6 +116,13 @@ struct tbj_batch_plan {
         using namespace tbj;
         batch_zero_u64(st, counts, n);
         for (const Group& g : groups) {
+            if (g.kind == kGroupPht) {
+                const PhtJob* jobs = reinterpret_cast<const PhtJob*>((char*)konst.p + g.jobs_off);
+                batch_pht(st, pht_grid, reinterpret_cast<const uint2*>(arena), jobs, g.njobs, g.nrc,
+                          g.nsc, ptab.p, g.nbuckets, ppool.p, (uint32_t*)small.p + 8 * g.small_idx,
+                          counts);
+                continue;
+            }
             const uint2* base = reinterpret_cast<const uint2*>(arena) + g.base_off;
             const SegDesc* segs1 = reinterpret_cast<const SegDesc*>((char*)konst.p + g.segs_off);
             const uint32_t* ids = reinterpret_cast<const uint32_t*>((char*)konst.p + g.ids_off);
@@ -121,13 +138,13 @@ struct tbj_batch_plan {
                 uint32_t* o1 = (uint32_t*)offs1.p;
                 uint32_t* q1 = (uint32_t*)gs1.p;
                 const BinSpec s1 = radix_pass(g.B, 0, g.F1);
-                launch_tiles_from_segs(st, segs1, 2 * g.nj, sm, t1, g.T1);
-                launch_hist(st, g.T1, base, t1, sm, 0, h1, s1, g.F1);
+                launch_tiles_from_segs(st, segs1, 2 * g.nj, sm, t1, g.T1, tune.tile);
+                launch_hist(st, g.T1, base, t1, sm, 0, h1, s1, g.F1, tune.tile);
                 launch_fill_u32(st, q1 + (g.L1 - 1), 1, 0);
                 launch_tile_offsets(st, g.T1, h1, o1, t1, segs1, sm, g.F1, q1, g.L1,
                                     (uint32_t*)partial.p, sm + 6);
                 launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, sm, 0, o1, s1, g.F1,
-                               g.passes == 1);
+                               g.passes == 1, nullptr, nullptr, tune.tile);
                 if (g.passes == 1) {
                     batch_bounds(st, q1, segs1, g.nj, g.F1, 1, 1, bnd);
                     Rb = (const uint2*)c1.p;
@@ -139,14 +156,15 @@ struct tbj_batch_plan {
                     SegDesc* s2d = (SegDesc*)segs2.p;
                     const BinSpec s2 = radix_pass(g.B, g.b1, g.F2);
                     launch_make_segments(st, q1, segs1, 2 * g.nj, g.F1, g.F2, s2d, t2, sm + 1,
-                                         g.T2max);
-                    launch_hist(st, g.T2max, (const uint2*)c1.p, t2, sm + 1, 0, h2, s2, g.F2);
+                                         g.T2max, nullptr, tune.tile);
+                    launch_hist(st, g.T2max, (const uint2*)c1.p, t2, sm + 1, 0, h2, s2, g.F2,
+                                tune.tile);
                     // group sums past the segments' own (up to g.L2) are not
                     // read: every bound lies before them
                     launch_tile_offsets(st, g.T2max, h2, o2, t2, s2d, sm + 1, g.F2, q2, g.L2,
                                         (uint32_t*)partial.p, sm + 7);
                     launch_scatter(st, g.T2max, (const uint2*)c1.p, (uint2*)c2.p, t2, sm + 1, 0,
-                                   o2, s2, g.F2, true);
+                                   o2, s2, g.F2, true, nullptr, nullptr, tune.tile);
                     batch_bounds(st, q2, s2d, g.nj, g.F1, g.F2, 2, bnd);
                     Rb = (const uint2*)c2.p;
                 }
@@ -173,8 +191,14 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     const uint32_t max_span = (uint32_t)kJoinSChunk * kJoinSTaskChunks;
     if (!T.s_chunk || T.s_chunk > max_span) T.s_chunk = max_span;
     if (!T.wave_tuples) T.wave_tuples = 1u << 31;
+    if (!T.tile) T.tile = kTile;
+    if (!valid_tile(T.tile)) return TBJ_ERR_INVALID;
+    const uint32_t tile = T.tile;
     T.one_pass_bits = std::min<uint32_t>(T.one_pass_bits, kMaxPassBits);
     const uint32_t n = P->n;
+    auto algo_of = [&](uint32_t i) {
+        return P->algo == TBJ_BATCH_PER_ITEM ? items[i].algo : P->algo;
+    };
     std::vector<uint32_t> order(n);
     std::iota(order.begin(), order.end(), 0);
     auto lo_of = [&](uint32_t i) { return std::min(items[i].r_off, items[i].s_off); };
@@ -184,6 +208,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     for (uint32_t i = 0; i < n; i++) {
         if (items[i].nR >= kWindow || items[i].nS >= kWindow || hi_of(i) - lo_of(i) > kWindow)
             return TBJ_ERR_TOO_LARGE;
+        if (algo_of(i) != TBJ_RHO && algo_of(i) != TBJ_PHT) return TBJ_ERR_UNSUPPORTED;
         P->tuples += items[i].nR + items[i].nS;
     }
     std::stable_sort(order.begin(), order.end(),
@@ -191,7 +216,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
 
     std::vector<char> konst;
     size_t max_tup = 0, max_T1 = 0, max_T2 = 0, max_L1 = 0, max_L2 = 0, max_s2 = 0, max_NP = 0;
-    size_t max_H1 = 0, max_H2 = 0;
+    size_t max_H1 = 0, max_H2 = 0, max_bk = 0, max_pool = 0;
     auto align = [&](size_t a) { konst.resize((konst.size() + a - 1) / a * a); };
     size_t w0 = 0;
     while (w0 < n) {
@@ -209,10 +234,16 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             w1++;
         }
         P->waves++;
-        // ---- groups: joins of the wave with the same partition bits
+        // ---- groups: the wave's PHT joins, and its RHO joins with the same
+        // partition bits
         std::vector<std::vector<uint32_t>> byB(2 * kMaxPassBits + 1);
+        std::vector<uint32_t> pht;
         for (size_t k = w0; k < w1; k++) {
             const uint32_t i = order[k];
+            if (algo_of(i) == TBJ_PHT) {
+                pht.push_back(i);
+                continue;
+            }
             const uint64_t nR = items[i].nR;
             uint32_t B = nR <= T.target_r ? 0 : ceil_log2((nR + T.target_r - 1) / T.target_r);
             B = std::min<uint32_t>(B, 2 * kMaxPassBits);
@@ -222,6 +253,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             const auto& js = byB[B];
             if (js.empty()) continue;
             Group g{};
+            g.kind = kGroupRadix;
             g.B = B;
             g.passes = B == 0 ? 0 : B <= T.one_pass_bits ? 1 : 2;
             g.b1 = g.passes == 2 ? (B + 1) / 2 : B;
@@ -240,7 +272,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
                     SegDesc& d = segs[2 * j + rel];
                     d.in_off = (uint32_t)((rel ? it.s_off : it.r_off) - wlo);
                     d.n = (uint32_t)(rel ? it.nS : it.nR);
-                    d.ntiles = (d.n + kTile - 1) / kTile;
+                    d.ntiles = (d.n + tile - 1) / tile;
                     d.first_tile = tiles;
                     d.ngroups = seg_groups(tiles, d.ntiles);
                     d.aux = 0;
@@ -252,7 +284,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             }
             g.T1 = tiles;
             g.L1 = (uint64_t)g.F1 * groups + 1;
-            g.T2max = g.passes == 2 ? (uint32_t)(g.tuples / kTile) + 2 * g.nj * g.F1 + 1 : 0;
+            g.T2max = g.passes == 2 ? (uint32_t)(g.tuples / tile) + 2 * g.nj * g.F1 + 1 : 0;
             // second-pass groups: tiles / kGroupTiles plus two per segment
             g.L2 = g.passes == 2
                        ? (uint64_t)g.F2 * (g.T2max / kGroupTiles + 2ull * (2 * g.nj * g.F1) + 1) + 1
@@ -277,6 +309,42 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             max_NP = std::max<size_t>(max_NP, g.NP);
             P->groups.push_back(g);
         }
+        if (!pht.empty()) {
+            // one table of nextpow2(|R| / 2) buckets per join
+            // (no_partitioning_join.h:555-556), all in one bucket array
+            Group g{};
+            g.kind = kGroupPht;
+            g.nj = g.njobs = (uint32_t)pht.size();
+            std::vector<PhtJob> jobs(pht.size());
+            for (uint32_t j = 0; j < g.njobs; j++) {
+                const tbj_batch_item& it = items[pht[j]];
+                PhtJob& jb = jobs[j];
+                jb.r_off = it.r_off;
+                jb.s_off = it.s_off;
+                jb.nR = (uint32_t)it.nR;
+                jb.nS = (uint32_t)it.nS;
+                const uint64_t nb = 1ull << ceil_log2(std::max<uint64_t>(1, it.nR / 2));
+                jb.mask = (uint32_t)(nb - 1);
+                jb.tab = g.nbuckets;
+                jb.id = pht[j];
+                jb.rc0 = g.nrc;
+                jb.sc0 = g.nsc;
+                g.nbuckets += nb;
+                g.nrc += (uint32_t)((it.nR + kPhtBuildChunk - 1) / kPhtBuildChunk);
+                // no R tuple, no match: S is not probed
+                if (it.nR) g.nsc += (uint32_t)((it.nS + kPhtProbeChunk - 1) / kPhtProbeChunk);
+                g.npool += it.nR;
+                g.tuples += it.nR + it.nS;
+            }
+            align(64);
+            g.jobs_off = konst.size();
+            konst.resize(konst.size() + jobs.size() * sizeof(PhtJob));
+            memcpy(konst.data() + g.jobs_off, jobs.data(), jobs.size() * sizeof(PhtJob));
+            g.small_idx = (uint32_t)P->groups.size();
+            max_bk = std::max<size_t>(max_bk, g.nbuckets);
+            max_pool = std::max<size_t>(max_pool, g.npool);
+            P->groups.push_back(g);
+        }
         w0 = w1;
     }
     // ---- device buffers (owned by the plan so that a captured graph stays valid)
@@ -298,14 +366,18 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     TBJ_TRY(P->konst.get(konst.size()));
     TBJ_TRY(P->small.get(sizeof(uint32_t) * 8 * std::max<size_t>(1, P->groups.size())));
     TBJ_TRY(P->total.get(sizeof(unsigned long long)));
+    TBJ_TRY(P->ptab.get(32 * max_bk));
+    TBJ_TRY(P->ppool.get(16 * max_pool));
     if (!konst.empty())
         TBJ_HIP(hipMemcpy(P->konst.p, konst.data(), konst.size(), hipMemcpyHostToDevice));
     std::vector<uint32_t> sm(8 * std::max<size_t>(1, P->groups.size()), 0);
-    for (const Group& g : P->groups) sm[8 * g.small_idx] = g.T1;
+    for (const Group& g : P->groups)
+        if (g.kind == kGroupRadix) sm[8 * g.small_idx] = g.T1;
     TBJ_HIP(hipMemcpy(P->small.p, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     TBJ_HIP(hipEventCreate(&P->e0));
     TBJ_HIP(hipEventCreate(&P->e1));
     P->grid = (uint32_t)tbj::dev().num_cus * 4;
+    P->pht_grid = (uint32_t)tbj::dev().num_cus * 8;
     return TBJ_OK;
 }
 
@@ -317,7 +389,7 @@ int tbj_batch_create(int algo, const tbj_tuple* arena, const tbj_batch_item* ite
                      uint64_t* counts, const tbj_batch_tuning* tuning, tbj_batch_plan** out) {
     if (!out) return TBJ_ERR_INVALID;
     *out = nullptr;
-    if (algo != TBJ_RHO) return TBJ_ERR_UNSUPPORTED;
+    if (algo != TBJ_RHO && algo != TBJ_PHT && algo != TBJ_BATCH_PER_ITEM) return TBJ_ERR_UNSUPPORTED;
     if (n == 0 || !items || !arena || !counts) return TBJ_ERR_INVALID;
     int s = tbj_init(tbj::dev().ready ? tbj::dev().device : 0);
     if (s != TBJ_OK) return s;
@@ -388,15 +460,63 @@ int tbj_batch_run(tbj_batch_plan* P, void* stream, tbj_batch_result* res) {
 
 void tbj_batch_destroy(tbj_batch_plan* P) { delete P; }
 
+namespace {
+
+// Median device time of `reps` runs (after one warm run) of the batch
+// planned with tuning t.
+int time_tuning(int algo, const tbj_tuple* arena, const tbj_batch_item* items, uint32_t n,
+                uint64_t* counts, int reps, const tbj_batch_tuning& t, float* ms_out) {
+    tbj_batch_plan* P = nullptr;
+    TBJ_TRY(tbj_batch_create(algo, arena, items, n, counts, &t, &P));
+    std::vector<float> ms;
+    int s = TBJ_OK;
+    for (int r = 0; r <= reps && s == TBJ_OK; r++) {
+        tbj_batch_result res;
+        s = tbj_batch_run(P, nullptr, &res);
+        if (r) ms.push_back(res.ms);
+    }
+    tbj_batch_destroy(P);
+    TBJ_TRY(s);
+    std::sort(ms.begin(), ms.end());
+    *ms_out = ms[ms.size() / 2];
+    return TBJ_OK;
+}
+
+}  // namespace
+
 int tbj_batch_autotune(int algo, const tbj_tuple* arena, const tbj_batch_item* items, uint32_t n,
                        uint64_t* counts, int reps, tbj_batch_tuning* best, float* best_ms) {
     if (!best) return TBJ_ERR_INVALID;
     if (reps < 1) reps = 1;
+    float bms = 1e30f;
+    tbj_batch_tuning bt{};
+    bt.wave_tuples = best->wave_tuples;
+    auto consider = [&](const tbj_batch_tuning& t) -> int {
+        float ms = 0;
+        TBJ_TRY(time_tuning(algo, arena, items, n, counts, reps, t, &ms));
+        if (ms < bms) {
+            bms = ms;
+            bt = t;
+        }
+        return TBJ_OK;
+    };
+    // stage 1: the partition tile shape at the default plan
+    const uint32_t tiles[] = {tbj::kTile / 2, tbj::kTile, tbj::kMaxTile};
+    for (uint32_t tile : tiles) {
+        tbj_batch_tuning t{};
+        t.target_r = tbj::kJoinCap / 2;
+        t.one_pass_bits = 8;
+        t.s_chunk = (uint32_t)tbj::kJoinSChunk * tbj::kJoinSTaskChunks;
+        t.wave_tuples = best->wave_tuples;
+        t.tile = tile;
+        TBJ_TRY(consider(t));
+    }
+    // stage 2: radix bits (target_r), pass count (one_pass_bits) and the
+    // join's task shape (s_chunk) at that tile
+    const uint32_t tile = bt.tile;
     const uint32_t targets[] = {1024, 2048};
     const uint32_t onepass[] = {6, 8, 10};
     const uint32_t chunks[] = {8192, 32768};
-    float bms = 1e30f;
-    tbj_batch_tuning bt{};
     for (uint32_t tr : targets)
         for (uint32_t op : onepass)
             for (uint32_t ch : chunks) {
@@ -405,23 +525,8 @@ int tbj_batch_autotune(int algo, const tbj_tuple* arena, const tbj_batch_item* i
                 t.one_pass_bits = op;
                 t.s_chunk = ch;
                 t.wave_tuples = best->wave_tuples;
-                tbj_batch_plan* P = nullptr;
-                TBJ_TRY(tbj_batch_create(algo, arena, items, n, counts, &t, &P));
-                std::vector<float> ms;
-                int s = TBJ_OK;
-                for (int r = 0; r <= reps && s == TBJ_OK; r++) {
-                    tbj_batch_result res;
-                    s = tbj_batch_run(P, nullptr, &res);
-                    if (r) ms.push_back(res.ms);
-                }
-                tbj_batch_destroy(P);
-                TBJ_TRY(s);
-                std::sort(ms.begin(), ms.end());
-                const float med = ms[ms.size() / 2];
-                if (med < bms) {
-                    bms = med;
-                    bt = t;
-                }
+                t.tile = tile;
+                TBJ_TRY(consider(t));
             }
     *best = bt;
     if (best_ms) *best_ms = bms;
@@ -440,12 +545,15 @@ uint64_t tbj_gen_batch_specs(uint64_t base_seed, uint32_t n, tbj_batch_spec* spe
         const double u = (double)(splitmix(y) >> 11) * (1.0 / 9007199254740992.0);
         const uint64_t nR = (uint64_t)floor(pow(2.0, 10.0 + 12.0 * u));
         const uint64_t k = 1 + splitmix(y) % 16;
-        const double z = 0.1 * (double)(splitmix(y) % 16);
+        const double z = 1.5 * (double)(splitmix(y) >> 11) * (1.0 / 9007199254740992.0);
+        const int algo = (splitmix(y) >> 63) ? TBJ_PHT : TBJ_RHO;
         tbj_batch_spec& sp = specs[i];
         sp.seed = s;
         sp.nR = nR;
         sp.nS = k * nR;
         sp.z = z;
+        sp.algo = algo;
+        sp.reserved = 0;
         sp.r_off = off;
         sp.s_off = off + nR;
         off += nR + sp.nS;

@@ -12,6 +12,10 @@ namespace tbj {
 constexpr int kPartBlock = 512;
 constexpr int kPartItems = 8;
 constexpr int kTile = kPartBlock * kPartItems;  // 4096 tuples = 32 KiB
+// Other tile shapes (tuples per tile) the batch's partition passes may use:
+// kTile / 2, kTile and kMaxTile
+constexpr int kMaxTile = 2 * kTile;
+inline bool valid_tile(uint32_t t) { return t == kTile / 2 || t == kTile || t == kMaxTile; }
 constexpr int kMaxBins = 2048;                  // bins of one pass
 constexpr int kMaxPassBits = 11;
 // Device-wide scan
@@ -38,10 +42,10 @@ enum JoinMode : int { kModeChain = 0, kModeHist = 1, kModeSort = 2 };
 
 // Tiles [tile_base, tile_base + ntiles) (and below *ntiles_dev); tile t's
 // bin counts go to hist[t * nbins, (t + 1) * nbins) (u16: a tile holds at
-// most kTile tuples).
+// most kMaxTile tuples).  tile: the tiles' shape (valid_tile).
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
-                 const BinSpec& bs, uint32_t nbins);
+                 const BinSpec& bs, uint32_t nbins, uint32_t tile = kTile);
 // Per-tile counts (u16 rows of nbins) -> per-tile output offsets (u32 rows
 // in offs), for the tiles below *ntiles_dev (at most max_tiles).  gsum (glen
 // words, the last one zero and glen - 1 >= nbins * the groups of all
@@ -75,7 +79,8 @@ struct ClaimSpec {
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims = nullptr, uint32_t* range_out = nullptr);
+                    const ClaimSpec* claims = nullptr, uint32_t* range_out = nullptr,
+                    uint32_t tile = kTile);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
@@ -85,10 +90,11 @@ void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
 void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1,
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
                           TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2,
-                          uint32_t* ngroups2_dev = nullptr);
-// Tiles of kTile tuples over a segment list (segment first_tile / ntiles set).
+                          uint32_t* ngroups2_dev = nullptr, uint32_t tile = kTile);
+// Tiles of `tile` tuples over a segment list (segment first_tile / ntiles set).
 void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
-                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles);
+                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles,
+                            uint32_t tile = kTile);
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
 // Sorts the m sampled keys in place (one workgroup).
 void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m);

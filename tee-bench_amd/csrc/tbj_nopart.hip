@@ -1,12 +1,8 @@
 // No-partitioning joins: PHT / NPO_st (shared hash table) and CHT (concise
 // hash table), each built in HBM with global atomics and probed by streaming S.
 //
-// PHT restates no_partitioning_join.h:119-356,219-259: nextpow2(|R|/2)
-// buckets, identity hash h = key & (buckets-1), two tuples per bucket, more
-// in overflow chains.  A bucket here is one 32-byte record {count, overflow
-// head, 2 tuples}; the reference's per-bucket spin latch becomes an
-// atomicAdd on count, and the per-thread overflow arenas become one pool
-// claimed with atomicAdd.
+// PHT restates no_partitioning_join.h:119-356,219-259 with nextpow2(|R|/2)
+// buckets (the table of tbj_pht.h).
 //
 // CHT restates CHT.hpp:26-254 / CHTJoin.hpp:46-322: a bitmap of
 // nextpow2(|R|) bits in 32-bit words with a running popcount per word, a
@@ -17,21 +13,9 @@
 // the tuple in an overflow hash table, so the result is the exact equi-join.
 #include "tbj_device.h"
 #include "tbj_kernels.h"
+#include "tbj_pht.h"
 
 namespace tbj {
-
-struct __align__(32) PhtBucket {
-    uint32_t count;
-    uint32_t ovf;  // 1-based index into the overflow pool, 0 = none
-    uint2 slot[2];
-    uint32_t pad[2];
-};
-
-struct __align__(16) OvNode {
-    uint2 t;
-    uint32_t next;  // 1-based
-    uint32_t pad;
-};
 
 __global__ void k_fill_u4(uint4* __restrict__ p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -61,14 +45,7 @@ __global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, 
 #pragma unroll
     for (int k = 0; k < kBuildItems; k++) {
         if (pos[k] == ~0u) continue;
-        PhtBucket* b = tab + (t[k].x & mask);
-        if (pos[k] < 2) {
-            b->slot[pos[k]] = t[k];
-        } else {
-            const uint32_t n = atomicAdd(pool_ctr, 1u);
-            pool[n].t = t[k];
-            pool[n].next = atomicExch(&b->ovf, n + 1);
-        }
+        pht_place(tab + (t[k].x & mask), t[k], pos[k], pool, pool_ctr);
     }
 }
 

@@ -138,7 +138,9 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
 // Persistent: each workgroup walks tiles tile_base + blockIdx.x, + gridDim.x,
 // ... below tile_end and *ntiles_dev, loading the partition function (the
 // learned model) into LDS once.  Tile t writes its counts to row t of hist
-// (nbins u16: a tile holds at most kTile tuples).
+// (nbins u16: a tile holds at most kMaxTile tuples).  IT: tuples per thread
+// (the tile holds kPartBlock * IT).
+template <int IT>
 __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ in,
                                                      const TileDesc* __restrict__ tiles,
                                                      const uint32_t* __restrict__ ntiles_dev,
@@ -154,14 +156,14 @@ __global__ __launch_bounds__(kPartBlock) void k_hist(const uint2* __restrict__ i
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) cnt[i] = 0;
         __syncthreads();
         const TileDesc td = tiles[t];
-        uint32_t key[kPartItems];
+        uint32_t key[IT];
 #pragma unroll
-        for (uint32_t k = 0; k < kPartItems; k++) {
+        for (uint32_t k = 0; k < IT; k++) {
             const uint32_t j = k * kPartBlock + threadIdx.x;
             key[k] = j < td.len ? in[td.in_off + j].x : 0;
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kPartItems; k++) {
+        for (uint32_t k = 0; k < IT; k++) {
             const uint32_t j = k * kPartBlock + threadIdx.x;
             if (j < td.len) {
                 const uint32_t b = bin(key[k]);
@@ -336,10 +338,12 @@ constexpr int kScatterBinsPerThread = kMaxBins / kPartBlock;
 
 // range_out (optional, {UINT_MAX, 0} beforehand) receives the exact key
 // range of the launch's tuples (one filtered atomic pair per workgroup).
-// Three workgroups per CU (six waves per SIMD) is what the LDS stage allows;
-// the register budget is held to it.
-template <bool GMODEL, bool AGG, bool CLAIM>
-__global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_scatter(const uint2* __restrict__ in,
+// IT: tuples per thread (the tile holds kPartBlock * IT).  With 4096-tuple
+// tiles three workgroups per CU (six waves per SIMD) is what the LDS stage
+// allows, and the register budget is held to it; 8192-tuple tiles fit one
+// workgroup per CU.
+template <bool GMODEL, bool AGG, bool CLAIM, int IT = kPartItems>
+__global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT > kPartItems ? 2 : 6))) void k_scatter(const uint2* __restrict__ in,
                                                         uint2* __restrict__ out,
                                                         const TileDesc* __restrict__ tiles,
                                                         const uint32_t* __restrict__ ntiles_dev,
@@ -347,8 +351,8 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6)))
                                                         const uint32_t* __restrict__ offs,
                                                         Claims cl, BinSpec bs, uint32_t nbins,
                                                         uint32_t* __restrict__ range_out) {
-    __shared__ uint2 stage[kTile];
-    __shared__ uint16_t sbin[kTile];
+    __shared__ uint2 stage[kPartBlock * IT];
+    __shared__ uint16_t sbin[kPartBlock * IT];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
     __shared__ uint32_t wrange[2][kPartBlock / TBJ_WAVE];  // range_out only
     extern __shared__ uint8_t dyn_lds[];
@@ -365,11 +369,11 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6)))
         const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
         for (uint32_t i = threadIdx.x; i < nbins; i += kPartBlock) loff[i] = 0;
         const TileDesc td = tiles[t];
-        uint2 v[kPartItems];
-        uint32_t b[kPartItems];
-        uint32_t rank[kPartItems];
+        uint2 v[IT];
+        uint32_t b[IT];
+        uint32_t rank[IT];
 #pragma unroll
-        for (uint32_t k = 0; k < kPartItems; k++) {
+        for (uint32_t k = 0; k < IT; k++) {
             const uint32_t j = k * kPartBlock + threadIdx.x;
             b[k] = TBJ_DROP;
             if (j < td.len) v[k] = in[td.in_off + j];
@@ -395,17 +399,17 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6)))
         }
         __syncthreads();
         {
-            uint32_t keys[kPartItems];
-            bool ok[kPartItems];
+            uint32_t keys[IT];
+            bool ok[IT];
 #pragma unroll
-            for (uint32_t k = 0; k < kPartItems; k++) {
+            for (uint32_t k = 0; k < IT; k++) {
                 keys[k] = v[k].x;
                 ok[k] = k * kPartBlock + threadIdx.x < td.len;
             }
             bin.template bins<GMODEL>(keys, ok, b);
         }
 #pragma unroll
-        for (uint32_t k = 0; k < kPartItems; k++) {
+        for (uint32_t k = 0; k < IT; k++) {
             if (AGG)
                 rank[k] = agg_inc(loff, b[k]);
             else if (b[k] != TBJ_DROP)
@@ -414,7 +418,7 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6)))
         if (range_out) {
             uint32_t mn = 0xFFFFFFFFu, mx = 0;
 #pragma unroll
-            for (uint32_t k = 0; k < kPartItems; k++) {
+            for (uint32_t k = 0; k < IT; k++) {
                 if (k * kPartBlock + threadIdx.x < td.len) {
                     mn = min(mn, v[k].x);
                     mx = max(mx, v[k].x);
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(6)))
         }
         const uint32_t nvalid = block_scan_lds<kPartBlock>(loff, nbins, scratch);
 #pragma unroll
-        for (uint32_t k = 0; k < kPartItems; k++) {
+        for (uint32_t k = 0; k < IT; k++) {
             if (b[k] != TBJ_DROP) {
                 const uint32_t slot = loff[b[k]] + rank[k];
                 stage[slot] = v[k];
@@ -764,7 +768,7 @@ __global__ void k_bounds_est(const uint32_t* __restrict__ starts,
 __global__ __launch_bounds__(kScanBlock) void k_make_segments(
     const uint32_t* __restrict__ scanned1, const SegDesc* __restrict__ segs1, uint32_t nseg1,
     uint32_t nbins1, uint32_t nbins2, SegDesc* __restrict__ segs2,
-    uint32_t* __restrict__ ntiles2_dev, uint32_t* __restrict__ ngroups2_dev) {
+    uint32_t* __restrict__ ntiles2_dev, uint32_t* __restrict__ ngroups2_dev, uint32_t tile) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
     __shared__ uint32_t carry_s, gcarry_s;
     const uint32_t nseg2 = nseg1 * nbins1;
@@ -779,7 +783,7 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments(
             st = scanned1[sd.hist_base + (uint64_t)b * sd.ngroups];
             const uint32_t en = scanned1[sd.hist_base + (uint64_t)(b + 1) * sd.ngroups];
             n = en - st;
-            nt = (n + kTile - 1) / kTile;
+            nt = (n + tile - 1) / tile;
         }
         uint32_t total, gtotal;
         const uint32_t ft = carry_s + block_exclusive_scan<kScanBlock>(nt, scratch, &total);
@@ -814,7 +818,7 @@ __global__ __launch_bounds__(kScanBlock) void k_make_segments(
 // over first_tile, then describe the tile.
 __global__ void k_make_tiles2(const SegDesc* __restrict__ segs2, uint32_t nseg2,
                               const uint32_t* __restrict__ ntiles2_dev,
-                              TileDesc* __restrict__ tiles2) {
+                              TileDesc* __restrict__ tiles2, uint32_t tile) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *ntiles2_dev) return;
     uint32_t lo = 0, hi = nseg2;  // last segment with first_tile <= t and ntiles > 0
@@ -830,8 +834,8 @@ __global__ void k_make_tiles2(const SegDesc* __restrict__ segs2, uint32_t nseg2,
     const SegDesc sd = segs2[lo];
     const uint32_t k = t - sd.first_tile;
     TileDesc td;
-    td.in_off = sd.in_off + k * kTile;
-    td.len = min((uint32_t)kTile, sd.n - k * kTile);
+    td.in_off = sd.in_off + k * tile;
+    td.len = min(tile, sd.n - k * tile);
     td.seg = lo;
     td.aux = 0;
     tiles2[t] = td;
@@ -1110,19 +1114,21 @@ static uint32_t persistent_grid(uint32_t ntiles) {
 
 void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDesc* tiles,
                  const uint32_t* ntiles_dev, uint32_t tile_base, uint16_t* hist,
-                 const BinSpec& bs, uint32_t nbins) {
+                 const BinSpec& bs, uint32_t nbins, uint32_t tile) {
     if (!ntiles) return;
     // the learned model is loaded into LDS once per workgroup: persistent
     // grid; radix: one tile per workgroup
     const uint32_t grid = bs.kind == kBinLearned ? persistent_grid(ntiles) : ntiles;
-    hipLaunchKernelGGL(k_hist, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles,
+    const auto k = tile == kTile / 2 ? k_hist<kPartItems / 2>
+                   : tile == kMaxTile ? k_hist<2 * kPartItems> : k_hist<kPartItems>;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kPartBlock), model_lds(bs), st, in, tiles,
                        ntiles_dev, tile_base, tile_base + ntiles, hist, bs, nbins);
 }
 
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims, uint32_t* range_out) {
+                    const ClaimSpec* claims, uint32_t* range_out, uint32_t tile) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
@@ -1143,6 +1149,12 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
     if (claim)
         k = learned ? (last_pass ? k_scatter<true, true, true> : k_scatter<true, false, true>)
                     : (last_pass ? k_scatter<false, true, true> : k_scatter<false, false, true>);
+    else if (tile == kTile / 2 && !learned)  // other tile shapes: radix passes without claims
+        k = last_pass ? k_scatter<false, true, false, kPartItems / 2>
+                      : k_scatter<false, false, false, kPartItems / 2>;
+    else if (tile == kMaxTile && !learned)
+        k = last_pass ? k_scatter<false, true, false, 2 * kPartItems>
+                      : k_scatter<false, false, false, 2 * kPartItems>;
     else
         k = learned ? (last_pass ? k_scatter<true, true, false> : k_scatter<true, false, false>)
                     : (last_pass ? k_scatter<false, true, false> : k_scatter<false, false, false>);
@@ -1178,19 +1190,20 @@ void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
 void launch_make_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1,
                           uint32_t nseg1, uint32_t nbins1, uint32_t nbins2, SegDesc* segs2,
                           TileDesc* tiles2, uint32_t* ntiles2_dev, uint32_t max_tiles2,
-                          uint32_t* ngroups2_dev) {
+                          uint32_t* ngroups2_dev, uint32_t tile) {
     hipLaunchKernelGGL(k_make_segments, dim3(1), dim3(kScanBlock), 0, st, scanned1, segs1, nseg1,
-                       nbins1, nbins2, segs2, ntiles2_dev, ngroups2_dev);
+                       nbins1, nbins2, segs2, ntiles2_dev, ngroups2_dev, tile);
     if (max_tiles2)
         hipLaunchKernelGGL(k_make_tiles2, dim3((max_tiles2 + 255) / 256), dim3(256), 0, st, segs2,
-                           nseg1 * nbins1, ntiles2_dev, tiles2);
+                           nseg1 * nbins1, ntiles2_dev, tiles2, tile);
 }
 
 void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
-                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles) {
+                            const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles,
+                            uint32_t tile) {
     if (max_tiles)
         hipLaunchKernelGGL(k_make_tiles2, dim3((max_tiles + 255) / 256), dim3(256), 0, st, segs,
-                           nseg, ntiles_dev, tiles);
+                           nseg, ntiles_dev, tiles, tile);
 }
 
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample) {

@@ -318,14 +318,19 @@ def gen_zipf_fk64(seed, n, R, z, nthreads=0):
 # Batches of joins (include/tbj.h: tbj_batch_*; include/tbj_gen.h:
 # tbj_gen_batch_*).  All relations of a batch live in one device arena.
 
+BATCH_PER_ITEM = -1  # tbj_batch_create: each item's own algo
+
+
 class BatchItem(ctypes.Structure):
     _fields_ = [("r_off", ctypes.c_uint64), ("nR", ctypes.c_uint64),
-                ("s_off", ctypes.c_uint64), ("nS", ctypes.c_uint64)]
+                ("s_off", ctypes.c_uint64), ("nS", ctypes.c_uint64),
+                ("algo", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
 
 
 class BatchTuning(ctypes.Structure):
     _fields_ = [("target_r", ctypes.c_uint32), ("one_pass_bits", ctypes.c_uint32),
-                ("s_chunk", ctypes.c_uint32), ("wave_tuples", ctypes.c_uint32)]
+                ("s_chunk", ctypes.c_uint32), ("wave_tuples", ctypes.c_uint32),
+                ("tile", ctypes.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -338,7 +343,8 @@ class BatchResult(ctypes.Structure):
 
 class BatchSpec(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("nR", ctypes.c_uint64), ("nS", ctypes.c_uint64),
-                ("z", ctypes.c_double), ("r_off", ctypes.c_uint64), ("s_off", ctypes.c_uint64)]
+                ("z", ctypes.c_double), ("r_off", ctypes.c_uint64), ("s_off", ctypes.c_uint64),
+                ("algo", ctypes.c_int32), ("reserved", ctypes.c_uint32)]
 
 
 _batch_ready = False
@@ -369,29 +375,36 @@ def _batch_lib():
 
 
 def batch_items(specs_or_tuples):
-    """BatchItem array from BatchSpecs or (r_off, nR, s_off, nS) tuples."""
+    """BatchItem array from BatchSpecs or (r_off, nR, s_off, nS[, algo])
+    tuples (algo: a name or enum value, default RHO)."""
     arr = (BatchItem * len(specs_or_tuples))()
     for i, x in enumerate(specs_or_tuples):
         if isinstance(x, BatchSpec):
-            arr[i] = BatchItem(x.r_off, x.nR, x.s_off, x.nS)
+            arr[i] = BatchItem(x.r_off, x.nR, x.s_off, x.nS, x.algo, 0)
         else:
-            arr[i] = BatchItem(*x)
+            algo = _algo(x[4]) if len(x) > 4 else ALGOS["RHO"]
+            arr[i] = BatchItem(*x[:4], algo, 0)
     return arr
 
 
+def _batch_algo(algo):
+    return BATCH_PER_ITEM if algo in (None, "per_item", BATCH_PER_ITEM) else _algo(algo)
+
+
 class Batch:
-    """A planned batch of count-only RHO joins over one device arena.
+    """A planned batch of count-only RHO/PHT joins over one device arena.
 
     arena: device tensor holding the tuples (8 bytes each); items: BatchItem
-    array; counts: device int64 tensor with one entry per join.
+    array; counts: device int64 tensor with one entry per join; algo: "RHO"
+    or "PHT" for every item, or "per_item" (each item's algo field).
     """
 
-    def __init__(self, arena, items, counts, tuning=None, algo="RHO"):
+    def __init__(self, arena, items, counts, tuning=None, algo="per_item"):
         L = _batch_lib()
         self.arena, self.items, self.counts = arena, items, counts
         self._plan = ctypes.c_void_p()
         t = ctypes.byref(tuning) if tuning is not None else None
-        check(L.tbj_batch_create(_algo(algo), arena.data_ptr(), items, len(items),
+        check(L.tbj_batch_create(_batch_algo(algo), arena.data_ptr(), items, len(items),
                                  counts.data_ptr(), t, ctypes.byref(self._plan)))
 
     def _stream(self):
@@ -418,11 +431,11 @@ class Batch:
             pass
 
 
-def batch_autotune(arena, items, counts, reps=2, wave_tuples=0, algo="RHO"):
+def batch_autotune(arena, items, counts, reps=2, wave_tuples=0, algo="per_item"):
     """Times candidate tunings on the batch; returns (BatchTuning, ms)."""
-    best = BatchTuning(0, 0, 0, wave_tuples)
+    best = BatchTuning(0, 0, 0, wave_tuples, 0)
     ms = ctypes.c_float()
-    check(_batch_lib().tbj_batch_autotune(_algo(algo), arena.data_ptr(), items, len(items),
+    check(_batch_lib().tbj_batch_autotune(_batch_algo(algo), arena.data_ptr(), items, len(items),
                                           counts.data_ptr(), reps, ctypes.byref(best),
                                           ctypes.byref(ms)))
     return best, ms.value

@@ -4,7 +4,9 @@ Every join of a batch is checked against the oracle's count on the same
 tuples (copied back from the arena) or, for generated batches, against the
 closed form: every S key is drawn from R's key set, so join i has exactly
 |S_i| matches.  The batch path has no reference counterpart; its per-join
-semantics are RHO's (true equi-join counts).
+semantics are RHO's and PHT's (true equi-join counts, checked against the
+oracle's count and, for PHT items, the oracle's restatement of the
+reference's PHT table).
 """
 import sys
 
@@ -14,27 +16,28 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def arena_of(rels):
+def arena_of(rels, algos=None):
     """Packs (R, S) pairs of (n, 2) uint32 arrays into one arena; returns
-    (device arena tensor, BatchItem array)."""
+    (device arena tensor, BatchItem array).  algos: per-join algo names."""
     import torch
     import tbjoin as tb
     parts, items, off = [], [], 0
-    for R, S in rels:
+    for i, (R, S) in enumerate(rels):
         parts += [R, S]
-        items.append((off, len(R), off + len(R), len(S)))
+        items.append((off, len(R), off + len(R), len(S), algos[i] if algos else "RHO"))
         off += len(R) + len(S)
     host = np.ascontiguousarray(np.concatenate(parts) if parts else np.zeros((1, 2), np.uint32))
     arena = torch.from_numpy(host.view(np.int64).reshape(-1)).to("cuda:0")
     return arena, tb.batch_items(items)
 
 
-def run_batch(tb, arena, items, capture=False, **tuning):
+def run_batch(tb, arena, items, capture=False, algo="per_item", **tuning):
     import torch
     counts = torch.zeros(len(items), dtype=torch.int64, device="cuda:0")
     t = tb.BatchTuning(**tuning) if tuning else None
-    print(f"[case] batch n={len(items)} capture={capture} {tuning}", file=sys.stderr, flush=True)
-    b = tb.Batch(arena, items, counts, t)
+    print(f"[case] batch n={len(items)} capture={capture} algo={algo} {tuning}",
+          file=sys.stderr, flush=True)
+    b = tb.Batch(arena, items, counts, t, algo=algo)
     if capture:
         b.capture()
     res = b.run()
@@ -78,12 +81,41 @@ def test_mixed_batch_matches_oracle(tbj, oracle):
     assert res.groups >= 4
 
 
+def test_pht_batch_matches_oracle(tbj, oracle):
+    # every join PHT (the batch's algo argument): one table per join, built
+    # and probed for all of them in one launch each; the counts are the
+    # oracle's restatement of the reference PHT (no_partitioning_join.h)
+    rels = mixed_rels(tbj, np.random.default_rng(5))
+    arena, items = arena_of(rels)
+    _, res, counts = run_batch(tbj, arena, items, algo="PHT")
+    want = [oracle.pht(R[:, 0], S[:, 0]) if len(S) else 0 for R, S in rels]
+    assert want == [oracle.count(R[:, 0], S[:, 0]) if len(S) else 0 for R, S in rels]
+    assert counts.tolist() == want and res.matches == sum(want)
+    assert res.groups == 1  # one wave, one PHT group
+
+
+@pytest.mark.parametrize("seed", [6, 7])
+def test_mixed_rho_pht_batch_matches_oracle(tbj, oracle, seed):
+    # per-item algos: RHO and PHT joins interleaved in one arena
+    rng = np.random.default_rng(seed)
+    rels = mixed_rels(tbj, rng)
+    algos = [("RHO", "PHT")[int(b)] for b in rng.integers(0, 2, len(rels))]
+    algos[0], algos[-1] = "PHT", "RHO"
+    arena, items = arena_of(rels, algos)
+    _, res, counts = run_batch(tbj, arena, items)
+    want = [oracle.count(R[:, 0], S[:, 0]) if len(S) else 0 for R, S in rels]
+    assert counts.tolist() == want and res.matches == sum(want)
+
+
 @pytest.mark.parametrize("tuning", [dict(wave_tuples=1 << 20), dict(one_pass_bits=4),
                                     dict(target_r=1024, s_chunk=4096),
-                                    dict(one_pass_bits=11, target_r=4096)])
+                                    dict(one_pass_bits=11, target_r=4096),
+                                    dict(tile=2048), dict(tile=8192),
+                                    dict(tile=8192, one_pass_bits=4, wave_tuples=1 << 20)])
 def test_tunings_and_waves_agree(tbj, oracle, tuning):
     rels = mixed_rels(tbj, np.random.default_rng(2))
-    arena, items = arena_of(rels)
+    # half of the joins PHT, so that waves hold both kinds of group
+    arena, items = arena_of(rels, ["RHO", "PHT"] * (len(rels) // 2) + ["RHO"] * (len(rels) % 2))
     _, res, counts = run_batch(tbj, arena, items, **tuning)
     want = [oracle.count(R[:, 0], S[:, 0]) if len(S) else 0 for R, S in rels]
     assert counts.tolist() == want
@@ -135,7 +167,8 @@ def test_generated_batch_closed_form_and_samples(tbj, oracle):
     arena = torch.empty(total, dtype=torch.int64, device="cuda:0")
     print("[case] gen_batch_device 300 joins", file=sys.stderr, flush=True)
     tbj.gen_batch_device(arena, specs)
-    items = tbj.batch_items(list(specs))
+    items = tbj.batch_items(list(specs))  # the specs' own RHO / PHT choice
+    assert {it.algo for it in items} == {tbj.ALGOS["RHO"], tbj.ALGOS["PHT"]}
     _, res, counts = run_batch(tbj, arena, items)
     assert counts.tolist() == [s.nS for s in specs]
     assert res.matches == sum(s.nS for s in specs)
@@ -180,6 +213,7 @@ def test_autotune_returns_a_working_tuning(tbj, oracle):
     print("[case] batch autotune", file=sys.stderr, flush=True)
     best, ms = tbj.batch_autotune(arena, items, counts, reps=1)
     assert ms > 0 and best.target_r in (1024, 2048) and best.one_pass_bits in (6, 8, 10)
+    assert best.tile in (2048, 4096, 8192)
     _, _, got = run_batch(tbj, arena, items, **best.as_dict())
     assert got.tolist() == [oracle.count(R[:, 0], S[:, 0]) if len(S) else 0 for R, S in rels]
 
@@ -188,8 +222,19 @@ def test_rejects_bad_arguments(tbj):
     import torch
     arena, items = arena_of([(rel([1, 2]), rel([2]))])
     counts = torch.zeros(1, dtype=torch.int64, device="cuda:0")
-    with pytest.raises(tbj.TbjError):
-        tbj.Batch(arena, items, counts, algo="PHT")
+    # joins other than RHO and PHT, for the batch or one item
+    for algo in ("CHT", "RSM"):
+        with pytest.raises(tbj.TbjError) as e:
+            tbj.Batch(arena, items, counts, algo=algo)
+        assert e.value.status == tbj.ERR_UNSUPPORTED
+    _, bad = arena_of([(rel([1, 2]), rel([2]))], ["CHT"])
+    with pytest.raises(tbj.TbjError) as e:
+        tbj.Batch(arena, bad, counts)
+    assert e.value.status == tbj.ERR_UNSUPPORTED
+    # tile shapes other than 2048, 4096, 8192
+    with pytest.raises(tbj.TbjError) as e:
+        tbj.Batch(arena, items, counts, tbj.BatchTuning(tile=1000))
+    assert e.value.status == tbj.ERR_INVALID
     big = tbj.batch_items([(0, 1 << 33, 0, 1)])
     with pytest.raises(tbj.TbjError):
         tbj.Batch(arena, big, counts)

@@ -135,8 +135,9 @@ def test_c5_psm_and_rsm_same_pairs_full_size(tbj):
 
 def test_c4_full_batch_closed_form(tbj, oracle):
     # The benchmark batch: 4096 generated joins (|R_i| log-uniform in
-    # [2^10, 2^22], |S_i| = k |R_i|, Zipf z over R_i's keys 1..|R_i|, R_i
-    # unique), so count_i = |S_i|; eager run, then the captured graph twice
+    # [2^10, 2^22], |S_i| = k |R_i|, Zipf z ~ U[0, 1.5) over R_i's keys
+    # 1..|R_i|, R_i unique, each join RHO or PHT), so count_i = |S_i|; eager
+    # run, then the captured graph twice
     import torch
     torch.cuda.empty_cache()  # the arena alone is 170 GB
     specs, total = tbj.gen_batch_specs(0xC4C4000000000000, 4096)

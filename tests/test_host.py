@@ -188,7 +188,14 @@ def test_batch_specs_follow_the_batch_recipe(tb):
     assert nR.min() >= 1024 and nR.max() < (1 << 22)
     k = nS // nR
     assert np.all(nS == k * nR) and k.min() == 1 and k.max() == 16
-    assert set(np.round(z * 10).astype(int)) == set(range(16))
+    # z ~ U[0, 1.5): continuous, each tenth of the range about equally often
+    assert z.min() >= 0 and z.max() < 1.5 and len(set(z.tolist())) == len(z)
+    zh = np.histogram(z, bins=10, range=(0, 1.5))[0]
+    assert zh.min() > 330 and zh.max() < 490
+    # algo ~ U{RHO, PHT}
+    algo = np.array([s.algo for s in specs])
+    assert set(algo.tolist()) == {tb.ALGOS["RHO"], tb.ALGOS["PHT"]}
+    assert 1900 < np.count_nonzero(algo == tb.ALGOS["PHT"]) < 2200
     # log-uniform sizes: log2 |R| roughly uniform on [10, 22)
     hist = np.histogram(np.log2(nR), bins=12, range=(10, 22))[0]
     assert hist.min() > 250 and hist.max() < 450

@@ -253,6 +253,7 @@ def side(tb, results, elapsed, n, part=None):
         d["fallback"] = r.fallback
         d["s_dropped"] = r.s_dropped
         d["max_r_partition"] = r.max_r_partition
+        d["learned_model"] = ["none", "knots+cells", "linear"][r.learned_model]
     return d
 
 

@@ -114,6 +114,9 @@ typedef struct tbj_result {
     float ms_total;             /* device time of the whole call */
     int32_t redone_exact;       /* 1 if a sample-sized region overflowed and the
                                    call was redone with exact histograms */
+    int32_t learned_model;      /* the learned partitioner's model: 1 = knots and a
+                                   cell table (piecewise-linear CDF), 2 = one linear
+                                   map of the key range (near-linear sample); 0 = none */
     /* with opt->profile: device ms of hist1, scan1, scatter1, pass2 sizing,
        scatter2, tasks, join, other */
     float phase_ms[8];

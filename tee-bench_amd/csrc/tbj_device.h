@@ -147,6 +147,54 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
     return x * q + min(x, r) + b / kNumXcd;
 }
 
+// Near-linear test of a key sample, for one whole workgroup (every thread
+// calls it; it ends with a barrier).  Splits the sample's range [kmin, kmax]
+// into kLinBuckets equal-width buckets (the last one may be narrower) and
+// answers true when none holds more than mean + mean / 4 + 4 sqrt(mean) of
+// the m keys: at that resolution the sample's CDF stays within a quarter
+// bucket of the straight line from kmin to kmax, so a linear map of the key
+// range spreads R over the partitions like the fitted model would.
+// lds: kLinBuckets + 5 words of 8 bytes.
+constexpr uint32_t kLinBuckets = 64;
+template <typename K>
+__device__ bool sample_is_linear(const K* __restrict__ a, uint32_t m, unsigned long long* lds,
+                                 K* kmin_out, K* kmax_out) {
+    unsigned long long* hist = lds;
+    unsigned long long* mn = lds + kLinBuckets;
+    unsigned long long* mx = mn + 1;
+    unsigned long long* flag = mn + 2;
+    if (threadIdx.x < kLinBuckets) hist[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        *mn = ~0ull;
+        *mx = 0;
+    }
+    __syncthreads();
+    K lo = (K)~(K)0, hi = 0;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        lo = min(lo, a[i]);
+        hi = max(hi, a[i]);
+    }
+    atomicMin(mn, (unsigned long long)lo);
+    atomicMax(mx, (unsigned long long)hi);
+    __syncthreads();
+    const K kmin = (K)*mn, kmax = (K)*mx;
+    const K w = (K)((kmax - kmin) / kLinBuckets + 1);
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x)
+        atomicAdd(&hist[(uint32_t)((a[i] - kmin) / w)], 1ull);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double mean = (double)m / kLinBuckets;
+        const double lim = mean + 0.25 * mean + 4.0 * sqrt(mean);
+        unsigned long long top = 0;
+        for (uint32_t b = 0; b < kLinBuckets; b++) top = max(top, hist[b]);
+        *flag = (double)top <= lim;
+    }
+    __syncthreads();
+    *kmin_out = kmin;
+    *kmax_out = kmax;
+    return *flag != 0;
+}
+
 __device__ __forceinline__ uint32_t next_pow2_u32(uint32_t v) {
     if (v <= 1) return 1;
     return 1u << (32 - __clz(v - 1));

@@ -96,8 +96,14 @@ void launch_tiles_from_segs(hipStream_t st, const SegDesc* segs, uint32_t nseg,
                             const uint32_t* ntiles_dev, TileDesc* tiles, uint32_t max_tiles,
                             uint32_t tile = kTile);
 void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint32_t* sample);
-// Sorts the m sampled keys in place (one workgroup).
-void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m);
+// Sorts the m sampled keys in place (one workgroup); nothing when model
+// (optional) is already the linear model.
+void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m,
+                     const CdfModel* model = nullptr);
+// The linear model from the (unsorted) sample when it is near-linear; sets
+// model->linear either way (launch_fit then leaves a linear model alone).
+void launch_fit_linear(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t bits,
+                       CdfModel* model, uint32_t* lin_out);
 // The model from the sorted sample and R's exact key range (minmax), or the
 // sample's range when minmax is NULL.
 void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nseg,

@@ -105,6 +105,13 @@ struct TaskSrc {
 // key interpolates linearly (integer arithmetic) between the values at its
 // cell's two ends, so the map is monotone: partitions are key ranges in key
 // order.  Keys outside [kmin, kmax] cannot match R and are dropped from S.
+//
+// A sample whose CDF is near-linear (sample_is_linear) gives a linear model
+// instead: no knots and no cells, the key range [kmin, kmax] (the sample's,
+// widened by 8 expected gaps per side) mapped straight onto the partitions,
+// partition = ((key - kmin) * lmul) >> 32 with lmul = floor(2^(32 + bits) /
+// (kmax - kmin + 1)): still key ranges in key order, at the cost of one
+// multiply.
 constexpr int kMaxKnots = 4096;
 constexpr int kCells = 4096;
 struct CdfModel {
@@ -114,7 +121,8 @@ struct CdfModel {
     uint32_t drop_outside;
     uint32_t cshift;      // cell of a key = (key - kmin) >> cshift
     uint32_t ncells;      // cells over [kmin, kmax]; cells[ncells] = 256 << bits
-    uint32_t pad;
+    uint32_t linear;      // 1: the linear model (lmul), no knots or cells
+    uint64_t lmul;
     uint64_t x[kMaxKnots + 1];  // knot keys (x[nseg] may be 2^32)
     uint32_t y[kMaxKnots + 1];  // cumulative partition position
     uint32_t cells[kCells + 1];

@@ -36,11 +36,19 @@ struct BinFn {
     const LdsModel* lm;
     uint32_t kmin, kmax, cshift, ncells, last;
     uint32_t dlo, dhi;  // keys outside [dlo, dhi] are dropped (bs.drop)
+    uint32_t lin;       // the linear model: ((key - kmin) * lmul) >> 32
+    uint64_t lmul;
+
+    __device__ __forceinline__ uint32_t linear(uint32_t key) const {
+        return key < kmin ? 0 : key > kmax ? last
+                                           : (uint32_t)(((uint64_t)(key - kmin) * lmul) >> 32);
+    }
 
     // The key's partition among all 2^bits (both passes), or TBJ_DROP.
     __device__ __forceinline__ uint32_t full(uint32_t key) const {
         if (bs.kind == kBinRadix) return bs.bits >= 32 ? key : (key & ((1u << bs.bits) - 1));
         if (bs.drop && (key < dlo || key > dhi)) return TBJ_DROP;
+        if (lin) return linear(key);
         if (key < kmin) return 0;
         const uint32_t d = key - kmin, c = d >> cshift;
         if (c >= ncells) return last;
@@ -61,7 +69,7 @@ struct BinFn {
     template <bool GMODEL, int N>
     __device__ __forceinline__ void fulls(const uint32_t (&key)[N], const bool (&valid)[N],
                                           uint32_t (&out)[N]) const {
-        if (bs.kind == kBinRadix) {
+        if (bs.kind == kBinRadix || lin) {
 #pragma unroll
             for (int q = 0; q < N; q++) out[q] = valid[q] ? full(key[q]) : TBJ_DROP;
             return;
@@ -117,9 +125,12 @@ __device__ __forceinline__ BinFn make_binfn(const BinSpec& bs, LdsModel* lm) {
     f.bs = bs;
     f.lm = GMODEL && bs.kind == kBinLearned ? reinterpret_cast<const LdsModel*>(bs.model->cells)
                                             : lm;
-    f.kmin = f.kmax = f.cshift = f.ncells = f.last = f.dlo = f.dhi = 0;
+    f.kmin = f.kmax = f.cshift = f.ncells = f.last = f.dlo = f.dhi = f.lin = 0;
+    f.lmul = 0;
     if (bs.kind == kBinLearned) {
-        if (!GMODEL) load_model(lm, bs.model);
+        f.lin = bs.model->linear;
+        f.lmul = bs.model->lmul;
+        if (!GMODEL && !f.lin) load_model(lm, bs.model);
         f.kmin = bs.model->kmin;
         f.kmax = bs.model->kmax;
         f.cshift = bs.model->cshift;
@@ -856,11 +867,44 @@ __global__ void k_sample_keys(const uint2* __restrict__ in, uint64_t n, uint32_t
     sample[i] = in[pos].x;
 }
 
+// The linear model when the sample (m keys, any order) is near-linear;
+// model->linear says which.  Its key range is the sample's, widened on each
+// side by 8 expected gaps between sampled keys (span / m), where the R keys
+// beyond the sample's extremes lie; it needs span + 1 >= 2^bits so that
+// lmul fits the product's 64 bits.
+__global__ __launch_bounds__(kFitBlock) void k_fit_linear(const uint32_t* __restrict__ sample,
+                                                          uint32_t m, uint32_t bits,
+                                                          CdfModel* __restrict__ model,
+                                                          uint32_t* __restrict__ lin_out) {
+    __shared__ unsigned long long lds[kLinBuckets + 5];
+    uint32_t kmin, kmax;
+    const bool lin = sample_is_linear(sample, m, lds, &kmin, &kmax);
+    if (threadIdx.x != 0) return;
+    const uint64_t gap = 8ull * ((uint64_t)(kmax - kmin) / m);
+    const uint64_t lo = kmin > gap ? kmin - gap : 0;
+    const uint64_t hi = min((uint64_t)kmax + gap, (uint64_t)0xFFFFFFFFu);
+    const uint64_t span1 = hi - lo + 1;
+    const bool ok = lin && span1 >= (1ull << bits);
+    model->linear = ok;
+    *lin_out = ok;
+    if (!ok) return;
+    model->lmul = (1ull << (32 + bits)) / span1;  // bits <= 22: 2^(32 + bits) fits
+    model->kmin = (uint32_t)lo;
+    model->kmax = (uint32_t)hi;
+    model->drop_outside = 1;
+    model->nseg = 0;
+    model->frac = 32 - bits;
+    model->cshift = 0;
+    model->ncells = 0;
+}
+
 // Bitonic sort of m (power of two, <= kFitMax) sampled keys in LDS, in
 // place.  It needs only the sample, so it runs beside the exact key range.
 __global__ __launch_bounds__(kFitBlock) void k_fit_sort(uint32_t* __restrict__ sample,
-                                                        uint32_t m) {
+                                                        uint32_t m,
+                                                        const CdfModel* __restrict__ model) {
     __shared__ uint32_t a[kFitMax];
+    if (model && model->linear) return;  // no knots to place
     for (uint32_t i = threadIdx.x; i < m; i += kFitBlock) a[i] = sample[i];
     __syncthreads();
     // one compare-exchange per pair index q < m / 2 (every lane busy)
@@ -890,6 +934,7 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
                                                        uint32_t bits,
                                                        const uint32_t* __restrict__ minmax,
                                                        CdfModel* __restrict__ model) {
+    if (model->linear) return;  // the linear model is complete
     __shared__ uint32_t quant[kLdsKnots];
     __shared__ uint32_t knots[kLdsKnots + 1];
     __shared__ uint64_t kx[kLdsKnots + 1];
@@ -1210,8 +1255,14 @@ void launch_sample(hipStream_t st, const uint2* in, uint64_t n, uint32_t m, uint
     hipLaunchKernelGGL(k_sample_keys, dim3((m + 255) / 256), dim3(256), 0, st, in, n, m, sample);
 }
 
-void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m) {
-    hipLaunchKernelGGL(k_fit_sort, dim3(1), dim3(kFitBlock), 0, st, sample, m);
+void launch_fit_sort(hipStream_t st, uint32_t* sample, uint32_t m, const CdfModel* model) {
+    hipLaunchKernelGGL(k_fit_sort, dim3(1), dim3(kFitBlock), 0, st, sample, m, model);
+}
+
+void launch_fit_linear(hipStream_t st, const uint32_t* sample, uint32_t m, uint32_t bits,
+                       CdfModel* model, uint32_t* lin_out) {
+    hipLaunchKernelGGL(k_fit_linear, dim3(1), dim3(kFitBlock), 0, st, sample, m, bits, model,
+                       lin_out);
 }
 
 void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nseg, uint32_t bits,

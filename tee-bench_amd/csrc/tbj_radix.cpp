@@ -36,7 +36,9 @@ struct Small {  // device scalars, one allocation
     uint32_t stats[2];
     uint32_t minmax[2];
     uint32_t total1, total2, overflow, region_total;
-    uint32_t ngroups2, est_max, pad[2];
+    uint32_t ngroups2, est_max;
+    uint32_t lin;  // the learned fit gave the linear model
+    uint32_t pad;
     unsigned long long counters[4];
 };
 
@@ -168,14 +170,16 @@ int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_rang
         TBJ_HIP(hipEventRecord(C.fork_ev, k.st));
         TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
         launch_sample(C.side_stream, k.J.R, nR, m, sample);
-        launch_fit_sort(C.side_stream, sample, m);
+        launch_fit_linear(C.side_stream, sample, m, plan.B, model, &k.sm->lin);
+        launch_fit_sort(C.side_stream, sample, m, model);
         TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
         launch_minmax(k.st, k.J.R, nR, k.sm->minmax);
         TBJ_HIP(hipStreamWaitEvent(k.st, C.join_ev, 0));
         launch_fit(k.st, sample, m, nseg, plan.B, k.sm->minmax, model);
     } else {
         launch_sample(k.st, k.J.R, nR, m, sample);
-        launch_fit_sort(k.st, sample, m);
+        launch_fit_linear(k.st, sample, m, plan.B, model, &k.sm->lin);
+        launch_fit_sort(k.st, sample, m, model);
         launch_fit(k.st, sample, m, nseg, plan.B, nullptr, model);
     }
     *s1r = learned_spec(plan, 1, model, 0);
@@ -521,6 +525,7 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
     TBJ_TRY(pt.finish(res));
     res->max_r_partition = h.stats[0];
     if (learned) res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
+    res->learned_model = learned ? (h.lin ? 2 : 1) : 0;
     *overflow_out = h.overflow != 0;
     res->matches = h.counters[0];
     res->checksum = h.counters[1];

@@ -49,9 +49,17 @@ constexpr int kSm64MergeTile = kSm64Block * kSm64MergeItems;
 // two ends; keys outside take segment 0's or the last segment's line, the
 // latter never below ymax, the table's value at kmax.  The map stays
 // monotone.
+//
+// A near-linear sample (sample_is_linear) gives the linear model instead:
+// partition = min(umulhi(key - kmin, lmul), 2^bits - 1) over [kmin, kmax]
+// (the sample's range widened by 8 expected gaps per side; keys beyond it
+// take the end partitions), lmul ~ 2^(64 + bits) / (kmax - kmin + 1): key
+// ranges in key order for one multiply, no knots and no cells.
 struct Model64 {
     uint32_t nseg, gshift, ncells, ymax;
     uint64_t kmin, kmax;  // sample key range (the cells span it)
+    uint32_t linear, pad;
+    uint64_t lmul;
     uint64_t x[kSm64Knots + 1];
     double base[kSm64Knots];
     double slope[kSm64Knots];
@@ -84,8 +92,9 @@ struct MergeBlock {
 };
 
 size_t sm64_model_lds(const Bin64& b);
+// *lin_out: 1 when the fit gave the linear model.
 void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint32_t nseg,
-              uint32_t bits, uint64_t* sample, Model64* model);
+              uint32_t bits, uint64_t* sample, Model64* model, uint32_t* lin_out);
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                      uint32_t nbins, uint32_t* ntiles_dev);
 // Tile t's bin counts go to row t of hist (nbins u16: a tile holds at most

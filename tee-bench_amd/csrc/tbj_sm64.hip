@@ -107,6 +107,8 @@ struct Bin64Fn {
     const LdsModel64* lm;
     uint32_t gshift, ymax;
     uint64_t kmin, kmax, lo, hi;
+    uint32_t lin, last;  // the linear model: min(umulhi(key - kmin, lmul), last)
+    uint64_t lmul;
     // the two end segments, for keys outside [kmin, kmax]
     uint64_t x0, xl;
     double base0, slope0, basel, slopel;
@@ -115,6 +117,9 @@ struct Bin64Fn {
 
     __device__ __forceinline__ uint32_t full(uint64_t key) const {
         if (kind == kBinRadix) return (uint32_t)key & radix_mask;
+        if (lin)
+            return key < kmin ? 0u : key > kmax ? last
+                                                : min((uint32_t)__umul64hi(key - kmin, lmul), last);
         if (key < kmin) {  // below the sample: segment 0's line, down to 0
             const double v = base0 - (double)(x0 - key) * slope0;
             return min(v <= 0.0 ? 0u : (uint32_t)v, phi0);
@@ -147,10 +152,18 @@ __device__ __forceinline__ Bin64Fn make_bin64(const Bin64& b, LdsModel64* lm) {
     f.kmin = f.kmax = f.x0 = f.xl = 0;
     f.base0 = f.slope0 = f.basel = f.slopel = 0.0;
     f.phi0 = f.plol = f.phil = 0;
+    f.lin = 0;
+    f.last = b.bits >= 32 ? 0xFFFFFFFFu : (1u << b.bits) - 1;
+    f.lmul = 0;
     f.drop = b.range != nullptr;
     f.lo = f.drop ? b.range[0] : 0;
     f.hi = f.drop ? b.range[1] : ~0ull;
-    if (b.kind == kBinLearned) {
+    if (b.kind == kBinLearned && b.model->linear) {
+        f.lin = 1;
+        f.kmin = b.model->kmin;
+        f.kmax = b.model->kmax;
+        f.lmul = b.model->lmul;
+    } else if (b.kind == kBinLearned) {
         const Model64* m = b.model;
         for (uint32_t i = threadIdx.x; i <= m->ncells; i += blockDim.x) lm->cellv[i] = m->cellv[i];
         const uint32_t l = m->nseg - 1;
@@ -187,7 +200,8 @@ __global__ void k_sample64(const ulonglong2* __restrict__ in, uint64_t n, uint32
 
 __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sample, uint32_t m,
                                                 uint32_t nseg_req, uint32_t bits,
-                                                Model64* __restrict__ model) {
+                                                Model64* __restrict__ model,
+                                                uint32_t* __restrict__ lin_out) {
     __shared__ uint64_t a[kSm64FitMax];
     __shared__ uint64_t knots[kSm64Knots];
     __shared__ uint32_t krank[kSm64Knots];
@@ -195,8 +209,35 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
     __shared__ uint32_t splo[kSm64Knots], sphi[kSm64Knots];
     __shared__ uint32_t scell[kSm64Cells + 1];
     __shared__ uint32_t nk_s;
+    __shared__ unsigned long long lin_lds[kLinBuckets + 5];
     for (uint32_t i = threadIdx.x; i < m; i += 1024) a[i] = sample[i];
     __syncthreads();
+    {
+        // a near-linear sample needs no sort, knots or cells: the linear
+        // model over its range widened by 8 expected gaps per side (span + 1
+        // >= 2^(bits + 1) keeps lmul below 2^64)
+        uint64_t smin, smax;
+        const bool lin = sample_is_linear(a, m, lin_lds, &smin, &smax);
+        const uint64_t gap = 8 * ((smax - smin) / m);
+        const uint64_t lo = smin > gap ? smin - gap : 0;
+        const uint64_t hi = smax < ~0ull - gap ? smax + gap : ~0ull;
+        const double span1 = (double)(hi - lo) + 1.0;
+        if (lin && span1 >= ldexp(1.0, (int)bits + 1)) {
+            if (threadIdx.x == 0) {
+                *lin_out = 1;
+                model->linear = 1;
+                model->lmul = (uint64_t)(ldexp(1.0, 64 + (int)bits) / span1);
+                model->kmin = lo;
+                model->kmax = hi;
+                model->nseg = model->gshift = model->ncells = model->ymax = 0;
+            }
+            return;
+        }
+        if (threadIdx.x == 0) {
+            model->linear = 0;
+            *lin_out = 0;
+        }
+    }
     // bitonic sort, one compare-exchange per pair index q < m / 2 (every
     // lane busy): pair q joins i = its low bits below j and the rest shifted
     // up one place, with i ^ j
@@ -343,8 +384,12 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
     }
     __syncthreads();
     if (mm && threadIdx.x == 0) {
-        atomicMin(&mm[0], bmm[0]);
-        atomicMax(&mm[1], bmm[1]);
+        // the stored range only narrows toward the result: a stale read can
+        // only let an unneeded atomic through, and most tiles need none (one
+        // atomic pair per tile on two words serialises)
+        const volatile unsigned long long* vmm = mm;
+        if (bmm[0] < vmm[0]) atomicMin(&mm[0], bmm[0]);
+        if (bmm[1] > vmm[1]) atomicMax(&mm[1], bmm[1]);
     }
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block)
         hist[(uint64_t)t * nbins + i] = (uint16_t)cnt[i];
@@ -1247,9 +1292,10 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
 size_t sm64_model_lds(const Bin64& b) { return b.kind == kBinLearned ? sizeof(LdsModel64) : 0; }
 
 void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint32_t nseg,
-              uint32_t bits, uint64_t* sample, Model64* model) {
+              uint32_t bits, uint64_t* sample, Model64* model, uint32_t* lin_out) {
     hipLaunchKernelGGL(k_sample64, dim3((m + 255) / 256), dim3(256), 0, st, R, nR, m, sample);
-    hipLaunchKernelGGL(k_fit64, dim3(1), dim3(1024), 0, st, sample, m, nseg, bits, model);
+    hipLaunchKernelGGL(k_fit64, dim3(1), dim3(1024), 0, st, sample, m, nseg, bits, model,
+                       lin_out);
 }
 
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,

@@ -23,7 +23,8 @@ enum Phase { kPhHist1 = 0, kPhScan1, kPhScatter1, kPhSize2, kPhScatter2, kPhTask
 
 struct Small64 {  // device scalars, one allocation
     uint32_t ntiles1, ntiles2, nrtasks, njtasks;
-    uint32_t max_r, total1, total2, pad;
+    uint32_t max_r, total1, total2;
+    uint32_t lin;  // the learned fit gave the linear model
     unsigned long long range[2];     // exact R key range (learned partitioner)
     unsigned long long counters[2];  // matches, checksum
     unsigned long long volume;       // pairs over all chunks
@@ -137,7 +138,7 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
             uint64_t* sample = nullptr;
             TBJ_TRY(W.get("s64_sample", sizeof(uint64_t) * m, (void**)&sample));
             const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kSm64Knots, m / 64));
-            sm64_fit(st, Rin, nR, m, nseg, B, sample, model);
+            sm64_fit(st, Rin, nR, m, nseg, B, sample, model, &sm->lin);
             s1r = bin64(kBinLearned, B, b2, F1, model, nullptr);
             s1s = bin64(kBinLearned, B, b2, F1, model, sm->range);
             s2 = bin64(kBinLearned, B, 0, F2, model, nullptr);
@@ -197,6 +198,7 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
     TBJ_HIP(hipStreamSynchronize(st));
     res->max_r_partition = h.max_r;
     if (learned) res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
+    res->learned_model = learned ? (h.lin ? 2 : 1) : 0;
     SmTask *rtasks = nullptr, *jtasks = nullptr;
     TBJ_TRY(W.get("s64_rtasks", sizeof(SmTask) * std::max<uint32_t>(1, h.nrtasks), (void**)&rtasks));
     TBJ_TRY(W.get("s64_jtasks", sizeof(SmTask) * std::max<uint32_t>(1, h.njtasks), (void**)&jtasks));

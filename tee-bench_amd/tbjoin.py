@@ -42,6 +42,7 @@ class Result(ctypes.Structure):
                 ("max_r_partition", ctypes.c_uint32), ("s_dropped", ctypes.c_uint64),
                 ("ms_partition", ctypes.c_float), ("ms_join", ctypes.c_float),
                 ("ms_total", ctypes.c_float), ("redone_exact", ctypes.c_int32),
+                ("learned_model", ctypes.c_int32),
                 ("phase_ms", ctypes.c_float * 8)]
 
     PHASES = ("hist1", "scan1", "scatter1", "size2", "scatter2", "tasks", "join", "other")

@@ -155,8 +155,53 @@ def test_learned_partitioner_matches_radix(tbj, oracle, z):
     a = assert_join(tbj, oracle, "RHO", R, S, partitioner="learned")
     b = assert_join(tbj, oracle, "RHO", R, S, partitioner="radix")
     assert a.partitioner_used == tbj.PART_LEARNED and a.fallback == 0
+    # R's keys 1..|R| are uniform: the fit is the linear map of the key range
+    assert a.learned_model == 2 and b.learned_model == 0
     assert a.s_dropped == nS // 2 and b.s_dropped == 0
     assert (a.matches, a.checksum) == (b.matches, b.checksum)
+
+
+def skewed_unique_keys(rng, n):
+    """Unique uint32 keys whose CDF is far from a line: half of them dense at
+    the bottom of the range, half spread thinly above (gaps of 500..1500)."""
+    dense = np.arange(1, n // 2 + 1, dtype=np.uint64)
+    sparse = np.uint64(n) + np.cumsum(rng.integers(500, 1500, n - n // 2)).astype(np.uint64)
+    k = np.concatenate([dense, sparse])
+    assert int(k.max()) < 2**32
+    return rng.permutation(k).astype(np.uint32)
+
+
+@pytest.mark.parametrize("sampled", [True, -1])
+def test_learned_piecewise_model_on_skewed_keys(tbj, oracle, sampled):
+    # a sample that is not near-linear: the knots-and-cells model, in the
+    # sample-sized flow (R's scatter measures the range S drops against) and
+    # in the exact flow (the exact range read beside the sample's sort)
+    rng = np.random.default_rng(21)
+    nR = 1 << 18
+    rk = skewed_unique_keys(rng, nR)
+    sk = np.concatenate([rng.choice(rk, 1 << 19),
+                         rng.integers(0, 2**32, 1 << 17, dtype=np.uint64).astype(np.uint32)])
+    res = assert_join(tbj, oracle, "RHO", rel(rk), rel(sk[rng.permutation(len(sk))]),
+                      partitioner="learned", sampled_regions=sampled, radix_bits=14, passes=2)
+    assert res.partitioner_used == tbj.PART_LEARNED and res.learned_model == 1
+    assert res.redone_exact == 0
+
+
+@pytest.mark.parametrize("where", ["bottom", "top"])
+@pytest.mark.parametrize("sampled", [True, -1])
+def test_learned_linear_model_at_range_ends(tbj, oracle, where, sampled):
+    # the linear model's key range widened past the sample's extremes, cut
+    # at 0 and at 2^32 - 1; S keys at the very ends and just outside R's keys
+    rng = np.random.default_rng(22)
+    nR = 200000
+    base = 0 if where == "bottom" else 2**32 - 2**21
+    rk = (base + rng.choice(2**21, nR, replace=False)).astype(np.uint32)
+    ends = np.array([0, 1, 2**32 - 1, 2**32 - 2, base, rk.min(), rk.max()], np.uint64)
+    sk = np.concatenate([rng.choice(rk, 600000).astype(np.uint64), ends,
+                         ends.astype(np.int64).clip(1) - 1]).astype(np.uint32)
+    res = assert_join(tbj, oracle, "RHO", rel(rk), rel(sk[rng.permutation(len(sk))]),
+                      partitioner="learned", sampled_regions=sampled, radix_bits=12, passes=2)
+    assert res.partitioner_used == tbj.PART_LEARNED and res.learned_model == 2
 
 
 def test_learned_fallback_to_radix(tbj, oracle):

@@ -232,3 +232,22 @@ def test_learned_key_spans(tbj, oracle, case):
     R, S = rel64(rk), rel64(sk)
     res, got = run64(tbj, "PSM", R, S, partitioner="learned")
     check_pairs(oracle, R, S, got, res, ordered=True)
+    if case != "narrow":  # uniform inside their span: the linear model
+        assert res.learned_model == 2
+
+
+def test_learned_piecewise_model64_on_skewed_keys(tbj, oracle):
+    # half of R packed into the bottom 2^20 keys, half spread up to 2^60:
+    # far from linear, so the knots-and-cells model; output in key order
+    rng = np.random.default_rng(34)
+    n = 1 << 17
+    rk = np.concatenate([rng.choice(1 << 20, n // 2, replace=False).astype(np.uint64),
+                         (np.uint64(1 << 20) +
+                          rng.choice(1 << 40, n // 2, replace=False).astype(np.uint64)
+                          * np.uint64(1 << 20))])
+    rk = rk[rng.permutation(len(rk))]
+    sk = rk[rng.integers(0, len(rk), 3 * len(rk))]
+    R, S = rel64(rk), rel64(sk)
+    res, got = run64(tbj, "PSM", R, S, partitioner="learned")
+    assert res.learned_model == 1
+    check_pairs(oracle, R, S, got, res, ordered=True)

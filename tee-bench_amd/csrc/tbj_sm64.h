@@ -153,9 +153,13 @@ void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
                       const unsigned long long* chunk_off, const unsigned long long* total,
                       unsigned long long* runs);
 // Emit: tasks of shared chunks claim slots in their runs; a chunk's only
-// task lays its runs out from chunk_off[rchunk] (the scanned volumes).
+// task lays its runs out from chunk_off[rchunk] (the scanned volumes).  Sb
+// holds R's partitions too: unless presorted, a chunk's only task with at
+// most kSm64Block S tuples reads its R chunk there unsorted (the count kernel
+// did not sort it) and orders its S tuples by key instead.
 void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
                const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
-               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap);
+               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap,
+               bool presorted);
 
 }  // namespace tbj

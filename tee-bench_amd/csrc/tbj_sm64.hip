@@ -965,6 +965,58 @@ __device__ __forceinline__ void find_runs(KeyAt key_at, uint32_t n, uint32_t hb,
     }
 }
 
+// A chunk probed by its task alone with at most kSmallS S tuples is not
+// sorted: every R tuple is chained in an LDS hash table as loaded, its S
+// tuples count their matches along the chains, and the emit kernel orders
+// the few S tuples by key instead (so pairs still come out in key order).
+constexpr uint32_t kSmallS = kSm64Block;
+
+__device__ __forceinline__ bool small_single(const SmTask& t) {
+    return !t.multi && t.s_len <= kSmallS;
+}
+
+// LDS index of every tuple of an unsorted chunk of n keys, chained under
+// hash64(key) (1-based, 0 ends a chain).  head must be zero; a barrier must
+// follow.
+template <typename KeyAt>
+__device__ __forceinline__ void index_all(KeyAt key_at, uint32_t n, uint32_t hb, uint32_t* head,
+                                          uint16_t* nxt) {
+    for (uint32_t j = threadIdx.x; j < n; j += kSm64Block)
+        nxt[j] = (uint16_t)atomicExch(&head[hash64(key_at(j), hb)], j + 1);
+}
+
+// Matches of N keys along index_all's chains, walked in lockstep (N
+// independent LDS reads per step); cks += R.payload + S.payload per match.
+template <int N, typename KeyAt, typename PayAt>
+__device__ __forceinline__ void count_all(KeyAt key_at, PayAt pay_at, uint32_t hb,
+                                          const uint32_t* head, const uint16_t* nxt,
+                                          const uint64_t (&k)[N], const uint64_t (&sp)[N],
+                                          const bool (&valid)[N], uint32_t (&cnt)[N],
+                                          uint64_t& cks) {
+    uint32_t e[N];
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        e[q] = valid[q] ? head[hash64(k[q], hb)] : 0;
+        cnt[q] = 0;
+    }
+    for (;;) {
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            if (e[q]) {
+                const uint32_t i = e[q] - 1;
+                if (key_at(i) == k[q]) {
+                    cnt[q]++;
+                    cks += pay_at(i) + sp[q];
+                }
+                e[q] = nxt[i];
+                any |= e[q] != 0;
+            }
+        }
+        if (!any) break;
+    }
+}
+
 // Count.  counters[0] += matches, counters[1] += sum of (R.payload +
 // S.payload).  MAT: runs[g] += S tuples matching the key run starting at
 // sorted R position g; chunk_vol[rchunk] += pairs of the task.
@@ -1003,8 +1055,10 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         uint64_t sk[G], sp[G];
         bool ok[G];
         load_s_group<G>(Sb + tk.s_off, tk.s_len, 0, sk, sp, ok);
-        if (PRESORTED || tk.multi) {
-            // slots past n (re-read copies of the last tuple) are never read
+        const bool small = !PRESORTED && small_single(tk);
+        if (PRESORTED || tk.multi || small) {
+            // presorted, or not sorted at all (small); slots past n (re-read
+            // copies of the last tuple) are never read
 #pragma unroll
             for (uint32_t k = 0; k < kSortItems; k++) L.a[k * kSm64Block + threadIdx.x] = v[k];
             __syncthreads();
@@ -1016,13 +1070,21 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         const uint32_t tn = t + gridDim.x;
         const SmTask tkn = tasks[tn < ntasks ? tn : t];
         load_chunk_clamped(chunk_src(tkn), tkn.r_len, v);
-        index_runs(key_at, n, hb, head, nxt);
+        if (small) index_all(key_at, n, hb, head, nxt);
+        else index_runs(key_at, n, hb, head, nxt);
         if (MAT)
             for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) L.cnt[j] = 0;
         if (threadIdx.x == 0) tm = 0;
         __syncthreads();
         uint64_t mt = 0;
-        for (uint32_t j0 = 0;;) {
+        if (small) {  // one group: s_len <= kSmallS <= G * kSm64Block
+            uint32_t c[G];
+            count_all<G>(key_at, [&](uint32_t m) { return (uint64_t)L.a[m].y; }, hb, head, nxt,
+                         sk, sp, ok, c, cks);
+#pragma unroll
+            for (int q = 0; q < G; q++) mt += c[q];
+        }
+        for (uint32_t j0 = 0; !small;) {
             uint32_t lo[G];
             find_runs<G>(key_at, n, hb, head, nxt, sk, ok, lo);
 #pragma unroll
@@ -1206,7 +1268,7 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
     const ulonglong2* __restrict__ Rsorted, const ulonglong2* __restrict__ Sb,
     const SmTask* __restrict__ tasks, const uint32_t* __restrict__ ntasks_dev,
     unsigned long long* __restrict__ runs, const unsigned long long* __restrict__ chunk_off,
-    ulonglong2* __restrict__ pairs, uint64_t cap) {
+    ulonglong2* __restrict__ pairs, uint64_t cap, uint32_t small_ok) {
     __shared__ uint64_t rk[kSm64Cap];
     __shared__ uint32_t lcnt[kSm64Cap];
     __shared__ uint32_t scratch[kSm64Block / TBJ_WAVE + 1];
@@ -1220,6 +1282,43 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         const SmTask tk = tasks[t];
         const uint32_t n = tk.r_len;
         const uint32_t hb = hash_bits(n);
+        if (small_ok && small_single(tk)) {
+            // unsorted R (the count kernel did not sort it): every tuple
+            // chained, one S tuple per thread ranked by (key, position), the
+            // pairs laid out in that order from the chunk's offset
+            const ulonglong2* Ru = Sb + tk.r_off;  // R's partitions share the buffer
+            for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) rk[j] = Ru[j].x;
+            const bool ok = threadIdx.x < tk.s_len;
+            const ulonglong2 sv = ok ? Sb[tk.s_off + threadIdx.x] : make_ulonglong2(0, 0);
+            __syncthreads();
+            index_all(key_at, n, hb, head, nxt);
+            uint64_t* sks = reinterpret_cast<uint64_t*>(lbase);
+            sks[threadIdx.x] = sv.x;
+            __syncthreads();
+            uint32_t m = 0;
+            if (ok)
+                for (uint32_t e = head[hash64(sv.x, hb)]; e; e = nxt[e - 1]) m += rk[e - 1] == sv.x;
+            uint32_t rank = 0;
+            if (ok)
+                for (uint32_t j = 0; j < tk.s_len; j++) {
+                    const uint64_t kj = sks[j];
+                    rank += kj < sv.x || (kj == sv.x && j < threadIdx.x);
+                }
+            if (ok) lcnt[rank] = m;
+            __syncthreads();
+            block_scan_lds<kSm64Block>(lcnt, tk.s_len, scratch);
+            if (ok && m) {
+                unsigned long long o = chunk_off[tk.rchunk] + lcnt[rank];
+                for (uint32_t e = head[hash64(sv.x, hb)]; e; e = nxt[e - 1])
+                    if (rk[e - 1] == sv.x) {
+                        if (o < cap) pairs[o] = make_ulonglong2(Ru[e - 1].y, sv.y);
+                        o++;
+                    }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < (1u << hb); i += kSm64Block) head[i] = 0;
+            continue;
+        }
         const ulonglong2* Rc = Rsorted + tk.r_off;
         for (uint32_t j = threadIdx.x; j < n; j += kSm64Block) {
             rk[j] = Rc[j].x;
@@ -1460,9 +1559,11 @@ void sm64_run_offsets(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted,
 
 void sm64_emit(hipStream_t st, uint32_t grid, const ulonglong2* Rsorted, const ulonglong2* Sb,
                const SmTask* tasks, const uint32_t* ntasks_dev, unsigned long long* runs,
-               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap) {
+               const unsigned long long* chunk_off, ulonglong2* pairs, uint64_t cap,
+               bool presorted) {
     hipLaunchKernelGGL(k_emit64, dim3(resident_grid(k_emit64, grid)), dim3(kSm64Block), 0, st,
-                       Rsorted, Sb, tasks, ntasks_dev, runs, chunk_off, pairs, cap);
+                       Rsorted, Sb, tasks, ntasks_dev, runs, chunk_off, pairs, cap,
+                       (uint32_t)!presorted);
 }
 
 }  // namespace tbj

@@ -263,7 +263,7 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
         const uint32_t ogrid = std::max<uint32_t>(1, std::min<uint32_t>(h.nrtasks, C.num_cus * 4));
         sm64_run_offsets(st, ogrid, Rsorted, rtasks, h.nrtasks, chunk_vol, &sm->volume, runs);
         sm64_emit(st, jgrid, Rsorted, Pb, jtasks, &sm->njtasks, runs, chunk_vol,
-                  (ulonglong2*)pairs, pairs_cap);
+                  (ulonglong2*)pairs, pairs_cap, presorted);
     }
     TBJ_HIP(hipGetLastError());
     TBJ_TRY(pt.mark(kPhJoin));

@@ -108,6 +108,25 @@ def test_duplicates_and_oversized_partitions(tbj, oracle, algo):
     check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
 
 
+@pytest.mark.parametrize("algo", ["PSM", "RSM"])
+def test_small_single_tasks_with_duplicates(tbj, oracle, algo):
+    # partitions of at most one chunk, each probed by a few dozen S tuples:
+    # the unsorted path (R chained in LDS, the task's S tuples ranked by key),
+    # with keys of 1, 2 and 3 R copies, repeated S keys and S keys that miss
+    rng = np.random.default_rng(41)
+    base = rng.choice(1 << 62, 40000, replace=False).astype(np.uint64) + np.uint64(1)
+    rk = np.concatenate([base, base[:5000], base[:1000]])
+    rk = rk[rng.permutation(len(rk))]
+    sk = np.concatenate([rng.choice(base, 3000), rng.choice(base[:1000], 500),
+                         rng.integers(1, 1 << 62, 800, dtype=np.uint64)])
+    sk = sk[rng.permutation(len(sk))]
+    R, S = rel64(rk), rel64(sk)
+    n = oracle.pairs64(R[:, 0], R[:, 1], S[:, 0], S[:, 1])[0]
+    res, got = run64(tbj, algo, R, S, cap=n)
+    assert res.max_r_partition <= 2048
+    check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
+
+
 def test_learned_oversized_unique_partitions(tbj, oracle):
     # 6 bits for 2^18 unique keys: every range partition spans two chunks
     R = tbj.gen_unique64(9, 1 << 18)

@@ -144,7 +144,7 @@ struct tbj_batch_plan {
                 launch_tile_offsets(st, g.T1, h1, o1, t1, segs1, sm, g.F1, q1, g.L1,
                                     (uint32_t*)partial.p, sm + 6);
                 launch_scatter(st, g.T1, base, (uint2*)c1.p, t1, sm, 0, o1, s1, g.F1,
-                               g.passes == 1, nullptr, nullptr, tune.tile);
+                               g.passes == 1, nullptr, tune.tile);
                 if (g.passes == 1) {
                     batch_bounds(st, q1, segs1, g.nj, g.F1, 1, 1, bnd);
                     Rb = (const uint2*)c1.p;
@@ -164,7 +164,7 @@ struct tbj_batch_plan {
                     launch_tile_offsets(st, g.T2max, h2, o2, t2, s2d, sm + 1, g.F2, q2, g.L2,
                                         (uint32_t*)partial.p, sm + 7);
                     launch_scatter(st, g.T2max, (const uint2*)c1.p, (uint2*)c2.p, t2, sm + 1, 0,
-                                   o2, s2, g.F2, true, nullptr, nullptr, tune.tile);
+                                   o2, s2, g.F2, true, nullptr, tune.tile);
                     batch_bounds(st, q2, s2d, g.nj, g.F1, g.F2, 2, bnd);
                     Rb = (const uint2*)c2.p;
                 }

@@ -74,13 +74,10 @@ struct ClaimSpec {
 // fill most of a wave: lanes sharing a bin then take their ranks from one LDS
 // atomic (in a first pass of two, whose tiles mix the whole key range, the
 // plain per-tuple atomics are cheaper).
-// range_out (optional; {UINT_MAX, 0} beforehand): the exact key range of
-// the launch's tuples.
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims = nullptr, uint32_t* range_out = nullptr,
-                    uint32_t tile = kTile);
+                    const ClaimSpec* claims = nullptr, uint32_t tile = kTile);
 // Exclusive scan of a[0..n) in place; partial needs scan_partials(n) words.
 uint32_t scan_partials(uint64_t n);
 void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,

@@ -347,8 +347,6 @@ struct Claims {
 // round trip sits between the tile's load and its writes.
 constexpr int kScatterBinsPerThread = kMaxBins / kPartBlock;
 
-// range_out (optional, {UINT_MAX, 0} beforehand) receives the exact key
-// range of the launch's tuples (one filtered atomic pair per workgroup).
 // IT: tuples per thread (the tile holds kPartBlock * IT).  With 4096-tuple
 // tiles three workgroups per CU (six waves per SIMD) is what the LDS stage
 // allows, and the register budget is held to it; 8192-tuple tiles fit one
@@ -360,12 +358,10 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
                                                         const uint32_t* __restrict__ ntiles_dev,
                                                         uint32_t tile_base, uint32_t tile_end,
                                                         const uint32_t* __restrict__ offs,
-                                                        Claims cl, BinSpec bs, uint32_t nbins,
-                                                        uint32_t* __restrict__ range_out) {
+                                                        Claims cl, BinSpec bs, uint32_t nbins) {
     __shared__ uint2 stage[kPartBlock * IT];
     __shared__ uint16_t sbin[kPartBlock * IT];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t wrange[2][kPartBlock / TBJ_WAVE];  // range_out only
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
     uint32_t* loff = reinterpret_cast<uint32_t*>(
@@ -426,36 +422,7 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
             else if (b[k] != TBJ_DROP)
                 rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
-        if (range_out) {
-            uint32_t mn = 0xFFFFFFFFu, mx = 0;
-#pragma unroll
-            for (uint32_t k = 0; k < IT; k++) {
-                if (k * kPartBlock + threadIdx.x < td.len) {
-                    mn = min(mn, v[k].x);
-                    mx = max(mx, v[k].x);
-                }
-            }
-            for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
-                mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
-                mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
-            }
-            if (lane_id() == 0) {
-                wrange[0][wave_id()] = mn;
-                wrange[1][wave_id()] = mx;
-            }
-        }
         __syncthreads();
-        if (range_out && threadIdx.x == 0) {
-            uint32_t mn = 0xFFFFFFFFu, mx = 0;
-            for (int w = 0; w < kPartBlock / TBJ_WAVE; w++) {
-                mn = min(mn, wrange[0][w]);
-                mx = max(mx, wrange[1][w]);
-            }
-            // the stored range only narrows toward the result: a stale read
-            // can only let an unneeded atomic through
-            if (mn < range_out[0]) atomicMin(&range_out[0], mn);
-            if (mx > range_out[1]) atomicMax(&range_out[1], mx);
-        }
         // claims: one atomic per non-empty bin, issued now, awaited after the
         // scan and the staging
         uint32_t cnt[kScatterBinsPerThread], base[kScatterBinsPerThread];
@@ -1173,7 +1140,7 @@ void launch_hist(hipStream_t st, uint32_t ntiles, const uint2* in, const TileDes
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
-                    const ClaimSpec* claims, uint32_t* range_out, uint32_t tile) {
+                    const ClaimSpec* claims, uint32_t tile) {
     if (!ntiles) return;
     // one tile per workgroup; the learned model is read in place (cached)
     // rather than copied into LDS per tile
@@ -1204,7 +1171,7 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
         k = learned ? (last_pass ? k_scatter<true, true, false> : k_scatter<true, false, false>)
                     : (last_pass ? k_scatter<false, true, false> : k_scatter<false, false, false>);
     hipLaunchKernelGGL(k, dim3(ntiles), dim3(kPartBlock), lds, st, in, out, tiles, ntiles_dev,
-                       tile_base, tile_base + ntiles, offs, cl, bs, nbins, range_out);
+                       tile_base, tile_base + ntiles, offs, cl, bs, nbins);
 }
 
 void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,

@@ -139,9 +139,9 @@ struct Call {
 // The learned model, fitted on a strided key sample of R, and the partition
 // functions of both passes.  exact_range: the model spans R's exact key range
 // (a pass over R, beside the sample's sort); otherwise it spans the sample's
-// range, and R's exact range -- which decides what S drops -- is measured by
-// R's first scatter into sm->minmax.  Either way the drop test reads
-// sm->minmax.
+// range, and R's exact range -- which decides what S drops -- is read into
+// sm->minmax by the sample-sized flow (beside R's sample and scatter).  Either
+// way the drop test reads sm->minmax.
 int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_range) {
     const PassPlan& plan = k.plan;
     if (!k.learned) {
@@ -367,7 +367,14 @@ int partition_sampled(Call& k) {
     launch_fill_u32(st, cnt1, np1, 0);
     launch_fill_u32(st, cursors1, np1, 0);
     launch_fill_u32(st, cursors2, nsub8, 0);
+    Context& C = k.C;
     if (k.learned) {
+        // R's exact key range (what S drops against) is read on the side
+        // stream beside R's sample, the judge and R's scatter
+        TBJ_HIP(hipEventRecord(C.fork_ev, st));
+        TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
+        launch_minmax(C.side_stream, k.J.R, nR, sm->minmax);
+        TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
         // R first: its sample, then the judge on R's estimated partitions
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
                            fine, 0);
@@ -376,6 +383,7 @@ int partition_sampled(Call& k) {
         TBJ_TRY(judge_learned(k, &fb));
         launch_fill_u32(st, sm->stats, 2, 0);
         if (fb) {
+            TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
             k.learned = false;
             k.res->fallback = 1;
             s1r = s1s = radix_spec(plan, 1);
@@ -399,14 +407,14 @@ int partition_sampled(Call& k) {
         launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
                        &c1s);
     } else {
-        // R's regions and scatter, which measures R's exact key range; then
-        // S's sample, regions and scatter, which drop S keys outside it
+        // R's regions and scatter; then S's sample, regions and scatter,
+        // which drop S keys outside R's range
         TBJ_TRY(k.pt.mark(kPhHist1));
         launch_row_caps(st, cnt1, nR1, starts1);
         launch_scan(st, starts1, (uint64_t)nR1 + 1, partial, &sm->region_total);
         ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
-        launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r,
-                       sm->minmax);
+        launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
+        TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
                            fine, 1);
         launch_row_caps(st, cnt1 + nR1, np1 - nR1, starts1 + nR1);

@@ -54,7 +54,9 @@ constexpr int kSm64MergeTile = kSm64Block * kSm64MergeItems;
 // partition = min(umulhi(key - kmin, lmul), 2^bits - 1) over [kmin, kmax]
 // (the sample's range widened by 8 expected gaps per side; keys beyond it
 // take the end partitions), lmul ~ 2^(64 + bits) / (kmax - kmin + 1): key
-// ranges in key order for one multiply, no knots and no cells.
+// ranges in key order for one multiply, no knots and no cells.  When the
+// widened range is all of [0, 2^64) (linear = 2) lmul is 2^bits and the map
+// is the key's top `bits` bits, taken with a shift.
 struct Model64 {
     uint32_t nseg, gshift, ncells, ymax;
     uint64_t kmin, kmax;  // sample key range (the cells span it)

@@ -108,7 +108,8 @@ struct Bin64Fn {
     uint32_t gshift, ymax;
     uint64_t kmin, kmax, lo, hi;
     uint32_t lin, last;  // the linear model: min(umulhi(key - kmin, lmul), last)
-    uint64_t lmul;
+    uint64_t lmul;       // (lin == 2: the key's top bits, key >> hshift)
+    uint32_t hshift;
     // the two end segments, for keys outside [kmin, kmax]
     uint64_t x0, xl;
     double base0, slope0, basel, slopel;
@@ -117,6 +118,7 @@ struct Bin64Fn {
 
     __device__ __forceinline__ uint32_t full(uint64_t key) const {
         if (kind == kBinRadix) return (uint32_t)key & radix_mask;
+        if (lin == 2) return (uint32_t)(key >> hshift);
         if (lin)
             return key < kmin ? 0u : key > kmax ? last
                                                 : min((uint32_t)__umul64hi(key - kmin, lmul), last);
@@ -153,13 +155,14 @@ __device__ __forceinline__ Bin64Fn make_bin64(const Bin64& b, LdsModel64* lm) {
     f.base0 = f.slope0 = f.basel = f.slopel = 0.0;
     f.phi0 = f.plol = f.phil = 0;
     f.lin = 0;
+    f.hshift = 64 - b.bits;
     f.last = b.bits >= 32 ? 0xFFFFFFFFu : (1u << b.bits) - 1;
     f.lmul = 0;
     f.drop = b.range != nullptr;
     f.lo = f.drop ? b.range[0] : 0;
     f.hi = f.drop ? b.range[1] : ~0ull;
     if (b.kind == kBinLearned && b.model->linear) {
-        f.lin = 1;
+        f.lin = b.model->linear;
         f.kmin = b.model->kmin;
         f.kmax = b.model->kmax;
         f.lmul = b.model->lmul;
@@ -225,7 +228,9 @@ __global__ __launch_bounds__(1024) void k_fit64(const uint64_t* __restrict__ sam
         if (lin && span1 >= ldexp(1.0, (int)bits + 1)) {
             if (threadIdx.x == 0) {
                 *lin_out = 1;
-                model->linear = 1;
+                // a range widened to all 64-bit keys maps a key to its top
+                // `bits` bits (lmul = 2^bits): a shift
+                model->linear = lo == 0 && hi == ~0ull ? 2 : 1;
                 model->lmul = (uint64_t)(ldexp(1.0, 64 + (int)bits) / span1);
                 model->kmin = lo;
                 model->kmax = hi;

@@ -67,6 +67,7 @@ struct ClaimSpec {
     uint32_t* overflow;
     uint32_t cb0, cmul, cstride;
     uint32_t xmul;  // r += xmul * the XCD eighth the tile runs in
+    const uint32_t* agg_flag = nullptr;  // last pass: aggregate ranks only while *agg_flag
 };
 // offs: the per-tile offsets from launch_tile_offsets (claims == NULL), or
 // sample-sized regions claimed per (tile, bin) (claims).  last_pass: the
@@ -134,10 +135,12 @@ void launch_est_part_max(hipStream_t st, const uint32_t* cnt1, uint32_t F1, uint
 // Second-pass region caps, 8 per-XCD sub-regions per (partition, bin):
 // 8 * nseg2 * nbins2 + 1 words, from fine, npart and the partitions' tile
 // spans (first: scanned tile counts in partition order; ntiles2: the total).
+// *hot becomes 1 when a second-pass bin holds over 1/8 of its partition's
+// sample (a hot key: the second pass then aggregates its ranks per wave).
 void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* npart,
                         uint32_t nseg2, uint32_t nbins2, const uint32_t* first,
                         const uint32_t* ntiles2, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
-                        uint32_t* caps);
+                        uint32_t* caps, uint32_t* hot);
 // Second-pass tiles over the n filled first-pass regions, visited in
 // partition order: ntiles (n + 1 words), scanned into first tiles, then the
 // descriptors.

@@ -337,6 +337,7 @@ struct Claims {
     uint32_t* overflow;
     uint32_t cb0, cmul, cstride;
     uint32_t xmul;  // r += xmul * the tile's XCD eighth (block index mod 8)
+    const uint32_t* agg_flag;  // AGG only while *agg_flag (NULL: always)
 };
 
 // CLAIM = false: offs row t = the output offset of each bin's run of tile t.
@@ -415,12 +416,16 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
             }
             bin.template bins<GMODEL>(keys, ok, b);
         }
+        // the aggregated ranks cost a ballot per item: a sample-sized pass
+        // takes them only when its sample saw a hot bin
+        const bool agg = AGG && (!CLAIM || !cl.agg_flag || *cl.agg_flag);
+        if (agg) {
 #pragma unroll
-        for (uint32_t k = 0; k < IT; k++) {
-            if (AGG)
-                rank[k] = agg_inc(loff, b[k]);
-            else if (b[k] != TBJ_DROP)
-                rank[k] = atomicAdd(&loff[b[k]], 1u);
+            for (uint32_t k = 0; k < IT; k++) rank[k] = agg_inc(loff, b[k]);
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < IT; k++)
+                if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
         __syncthreads();
         // claims: one atomic per non-empty bin, issued now, awaited after the
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(256) void k_region_caps(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ ntiles2,
                                                      uint32_t F1, uint32_t nrowR, uint32_t nrowS,
                                                      uint32_t* __restrict__ caps,
-                                                     uint32_t nsub) {
+                                                     uint32_t nsub, uint32_t* __restrict__ hot) {
     __shared__ uint32_t red[256 / TBJ_WAVE];
     const uint32_t s = blockIdx.x;
     const uint32_t* f = fine + (uint64_t)s * nbins2;
@@ -653,6 +658,8 @@ __global__ __launch_bounds__(256) void k_region_caps(const uint32_t* __restrict_
     }
     const double n = (double)npart[s];
     for (uint32_t b = threadIdx.x; b < nbins2; b += 256) {
+        // a second-pass bin with over 1/8 of its partition's sample is hot
+        if (f[b] >= 8 && 8ull * f[b] > tot) atomicOr(hot, 1u);
         const double raw = tot ? n * (double)f[b] / (double)tot : 0.0;
         const double shrunk = n * ((double)f[b] + 8.0) / ((double)tot + 8.0 * nbins2);
         const double e = raw > shrunk ? raw : shrunk;
@@ -1156,6 +1163,7 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
         cl.cmul = claims->cmul;
         cl.cstride = claims->cstride;
         cl.xmul = claims->xmul;
+        cl.agg_flag = claims->agg_flag;
     }
     decltype(&k_scatter<false, false, false>) k;
     if (claim)
@@ -1312,9 +1320,10 @@ void launch_est_part_max(hipStream_t st, const uint32_t* cnt1, uint32_t F1, uint
 void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* npart,
                         uint32_t nseg2, uint32_t nbins2, const uint32_t* first,
                         const uint32_t* ntiles2, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
-                        uint32_t* caps) {
+                        uint32_t* caps, uint32_t* hot) {
+    launch_fill_u32(st, hot, 1, 0);
     hipLaunchKernelGGL(k_region_caps, dim3(nseg2), dim3(256), 0, st, fine, npart, nbins2, first,
-                       ntiles2, F1, nrowR, nrowS, caps, nseg2 * nbins2);
+                       ntiles2, F1, nrowR, nrowS, caps, nseg2 * nbins2, hot);
 }
 
 void launch_region_tiles(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,

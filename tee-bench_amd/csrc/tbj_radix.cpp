@@ -38,7 +38,7 @@ struct Small {  // device scalars, one allocation
     uint32_t total1, total2, overflow, region_total;
     uint32_t ngroups2, est_max;
     uint32_t lin;  // the learned fit gave the linear model
-    uint32_t pad;
+    uint32_t hot2;  // the sample saw a hot second-pass bin
     unsigned long long counters[4];
 };
 
@@ -432,11 +432,11 @@ int partition_sampled(Call& k) {
     launch_region_tiles(st, starts1, cursors1, np1, F1, nrowR, nrowS, first2);
     launch_scan(st, first2, (uint64_t)np1 + 1, partial, &sm->ntiles2);
     launch_region_caps(st, fine, npart, 2 * F1, F2, first2, &sm->ntiles2, F1, nrowR, nrowS,
-                       starts2);
+                       starts2, &sm->hot2);
     launch_scan(st, starts2, (uint64_t)nsub8 + 1, partial, &sm->total2);
     launch_region_tile_descs(st, starts1, cursors1, first2, np1, F1, nrowR, nrowS, tiles2);
     TBJ_TRY(k.pt.mark(kPhSize2));
-    ClaimSpec c2{starts2, cursors2, &sm->overflow, 0, 8 * F2, 1, F2};
+    ClaimSpec c2{starts2, cursors2, &sm->overflow, 0, 8 * F2, 1, F2, &sm->hot2};
     launch_scatter(st, T2max, C1, C2, tiles2, &sm->ntiles2, 0, nullptr, s2, F2, true, &c2);
     // partition p's side rel is the 8 pieces bounds[(rel * nparts + p) * 8 + x]
     launch_bounds_x8(st, starts2, cursors2, F1, F2, k.bounds);

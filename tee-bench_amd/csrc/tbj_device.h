@@ -83,6 +83,38 @@ __device__ __forceinline__ uint64_t wave_inclusive_scan_u64(uint64_t x) {
     return x;
 }
 
+// One DPP step of a 64-bit value whose invalid source lanes read `fill`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64_fill(uint64_t x, uint64_t fill) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(
+        (int)(uint32_t)fill, (int)(uint32_t)x, CTRL, ROWS, 0xf, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(
+        (int)(uint32_t)(fill >> 32), (int)(uint32_t)(x >> 32), CTRL, ROWS, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Minimum and maximum of 64-bit values across a wave, in the scans' DPP
+// steps (no LDS crossbar round trips); every lane receives both.
+__device__ __forceinline__ void wave_minmax_u64(uint64_t& mn, uint64_t& mx) {
+    mn = min(mn, dpp_u64_fill<0x111, 0xf>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x111, 0xf>(mx, 0));
+    mn = min(mn, dpp_u64_fill<0x112, 0xf>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x112, 0xf>(mx, 0));
+    mn = min(mn, dpp_u64_fill<0x114, 0xf>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x114, 0xf>(mx, 0));
+    mn = min(mn, dpp_u64_fill<0x118, 0xf>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x118, 0xf>(mx, 0));
+    mn = min(mn, dpp_u64_fill<0x142, 0xa>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x142, 0xa>(mx, 0));
+    mn = min(mn, dpp_u64_fill<0x143, 0xc>(mn, ~0ull));
+    mx = max(mx, dpp_u64_fill<0x143, 0xc>(mx, 0));
+    // lane 63 holds the wave's extremes
+    mn = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mn >> 32), 63) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mn, 63);
+    mx = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(mx >> 32), 63) << 32) |
+         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mx, 63);
+}
+
 __device__ __forceinline__ uint64_t wave_reduce_u64(uint64_t x) {
 #pragma unroll
     for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) x += __shfl_xor(x, d, TBJ_WAVE);

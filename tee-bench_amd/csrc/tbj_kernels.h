@@ -144,12 +144,16 @@ void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* np
 // Second-pass tiles over the n filled first-pass regions, visited in
 // partition order: ntiles (n + 1 words), scanned into first tiles, then the
 // descriptors.
+// tile: tuples per second-pass tile.
 void launch_region_tiles(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                          uint32_t n, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
-                         uint32_t* ntiles);
+                         uint32_t* ntiles, uint32_t tile = kTile);
+// seg_is_part: a tile's segment is its first-pass partition (rel * F1 + b1)
+// rather than its region.
 void launch_region_tile_descs(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                               const uint32_t* first, uint32_t n, uint32_t F1, uint32_t nrowR,
-                              uint32_t nrowS, TileDesc* tiles);
+                              uint32_t nrowS, TileDesc* tiles, uint32_t tile = kTile,
+                              bool seg_is_part = false);
 void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                        uint32_t n, uint2* bounds);
 // Join bounds (8 pieces per partition side) of the second pass's per-XCD

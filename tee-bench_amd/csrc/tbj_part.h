@@ -54,6 +54,12 @@ constexpr uint32_t kRowTiles = 2048;
 constexpr uint32_t kSampleTiles = 224;
 constexpr uint32_t kChunksPerRow = (kRowTiles + kSampleTiles - 1) / kSampleTiles;
 
+// The sampled 16-tuple group of a 256-tuple block (one in 16, at a hashed
+// position).
+inline __host__ __device__ uint32_t sampled_group(uint32_t block) {
+    return (block * 2654435761u) >> 28;
+}
+
 inline __host__ __device__ uint32_t eighth_start(uint32_t x, uint32_t n) {
     const uint32_t q = n / 8, r = n % 8;
     return x * q + (x < r ? x : r);

@@ -478,9 +478,6 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
 // relation, the sampled tuples of each (first-pass bin, second-pass bin)
 // pair (sizing the second pass's regions inside each first-pass partition).
 
-__device__ __forceinline__ uint32_t sampled_group(uint32_t block) {
-    return (block * 2654435761u) >> 28;  // which of the block's 16 groups
-}
 
 // One workgroup per sample chunk: chunk c of tile row `row` (rows of R first,
 // then S; kChunksPerRow chunks per row).  cnt1 (zeroed) accumulates the
@@ -690,12 +687,13 @@ __device__ __forceinline__ uint32_t region_order(uint32_t i, uint32_t F1, uint32
 
 __global__ void k_region_tiles(const uint32_t* __restrict__ starts,
                                const uint32_t* __restrict__ cursors, uint32_t n, uint32_t F1,
-                               uint32_t nrowR, uint32_t nrowS, uint32_t* __restrict__ ntiles) {
+                               uint32_t nrowR, uint32_t nrowS, uint32_t* __restrict__ ntiles,
+                               uint32_t tile) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const uint32_t len = min(cursors[i], starts[i + 1] - starts[i]);
         uint32_t part;
-        ntiles[region_order(i, F1, nrowR, nrowS, &part)] = (len + kTile - 1) / kTile;
+        ntiles[region_order(i, F1, nrowR, nrowS, &part)] = (len + tile - 1) / tile;
     } else if (i == n) {
         ntiles[i] = 0;
     }
@@ -707,17 +705,18 @@ __global__ void k_region_tile_descs(const uint32_t* __restrict__ starts,
                                     const uint32_t* __restrict__ cursors,
                                     const uint32_t* __restrict__ first, uint32_t n, uint32_t F1,
                                     uint32_t nrowR, uint32_t nrowS,
-                                    TileDesc* __restrict__ tiles) {
+                                    TileDesc* __restrict__ tiles, uint32_t tile,
+                                    uint32_t seg_is_part) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t st = starts[i], len = min(cursors[i], starts[i + 1] - st);
     uint32_t part;
     uint32_t t = first[region_order(i, F1, nrowR, nrowS, &part)];
-    for (uint32_t k = 0; k < len; k += kTile, t++) {
+    for (uint32_t k = 0; k < len; k += tile, t++) {
         TileDesc d;
         d.in_off = st + k;
-        d.len = min((uint32_t)kTile, len - k);
-        d.seg = i;
+        d.len = min(tile, len - k);
+        d.seg = seg_is_part ? part : i;
         d.aux = part;
         tiles[t] = d;
     }
@@ -1328,17 +1327,17 @@ void launch_region_caps(hipStream_t st, const uint32_t* fine, const uint32_t* np
 
 void launch_region_tiles(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                          uint32_t n, uint32_t F1, uint32_t nrowR, uint32_t nrowS,
-                         uint32_t* ntiles) {
+                         uint32_t* ntiles, uint32_t tile) {
     hipLaunchKernelGGL(k_region_tiles, dim3((n + 1 + 255) / 256), dim3(256), 0, st, starts,
-                       cursors, n, F1, nrowR, nrowS, ntiles);
+                       cursors, n, F1, nrowR, nrowS, ntiles, tile);
 }
 
 void launch_region_tile_descs(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                               const uint32_t* first, uint32_t n, uint32_t F1, uint32_t nrowR,
-                              uint32_t nrowS, TileDesc* tiles) {
+                              uint32_t nrowS, TileDesc* tiles, uint32_t tile, bool seg_is_part) {
     if (!n) return;
     hipLaunchKernelGGL(k_region_tile_descs, dim3((n + 255) / 256), dim3(256), 0, st, starts,
-                       cursors, first, n, F1, nrowR, nrowS, tiles);
+                       cursors, first, n, F1, nrowR, nrowS, tiles, tile, (uint32_t)seg_is_part);
 }
 
 void launch_bounds_x8(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,

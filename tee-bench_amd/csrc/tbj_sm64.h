@@ -97,8 +97,28 @@ size_t sm64_model_lds(const Bin64& b);
 // *lin_out: 1 when the fit gave the linear model.
 void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint32_t nseg,
               uint32_t bits, uint64_t* sample, Model64* model, uint32_t* lin_out);
+// rows: each tile's aux = its row (tile_row) for sample-sized first passes.
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
-                     uint32_t nbins, uint32_t* ntiles_dev);
+                     uint32_t nbins, uint32_t* ntiles_dev, bool rows = false);
+// Sample of a sample-sized first pass: cnt1 (zeroed, F1 words per row of R,
+// then per row of S) += the sampled tuples of each (row, bin).  bss must not
+// drop (R's range is not known yet).
+void sm64_sample_rows(hipStream_t st, const ulonglong2* R, uint32_t nR, const ulonglong2* S,
+                      uint32_t nS, uint32_t nrowR, uint32_t nrowS, const Bin64& bsr,
+                      const Bin64& bss, uint32_t F1, uint32_t* cnt1);
+// Second-pass segments (one per first-pass partition, 2 * F1) over the tiles
+// of sample-sized regions (first: their scanned tile counts in partition order).
+void sm64_part_segments(hipStream_t st, const uint32_t* first, uint32_t F1, uint32_t F2,
+                        uint32_t nrowR, uint32_t nrowS, SegDesc* segs2);
+// Claims of a sample-sized first pass (see k_scatter64): regions cb0 +
+// tile row * cmul + bin; mm (optional) receives the launch's key range.
+struct Sm64Claims {
+    const uint32_t* starts;
+    uint32_t* cursors;
+    uint32_t* overflow;
+    uint32_t cb0, cmul;
+    unsigned long long* mm;
+};
 // Tile t's bin counts go to row t of hist (nbins u16: a tile holds at most
 // kSm64Span tuples); mm (optional):
 // receives atomic min / max of the keys of the tiles.
@@ -109,7 +129,8 @@ void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDe
 // per-tile offsets (launch_tile_offsets).
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
                   const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                  const uint32_t* offs, const Bin64& b, uint32_t nbins);
+                  const uint32_t* offs, const Bin64& b, uint32_t nbins,
+                  const Sm64Claims* claims = nullptr);
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,
                    uint32_t nbins2, SegDesc* segs2, TileDesc* tiles2, uint32_t* ntiles2_dev,
                    uint32_t max_tiles2);

@@ -378,10 +378,7 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         agg_add(cnt, j < td.len ? bin(kv[q]) : kNone);
     }
     if (mm) {  // tile min / max: waves -> LDS -> one global atomic pair
-        for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
-            mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, TBJ_WAVE));
-            mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, TBJ_WAVE));
-        }
+        wave_minmax_u64(mn, mx);
         if (lane_id() == 0) {
             atomicMin(&bmm[0], (unsigned long long)mn);
             atomicMax(&bmm[1], (unsigned long long)mx);
@@ -411,19 +408,37 @@ __host__ __device__ constexpr size_t sm64_model_bytes_aligned(uint32_t kind) {
 // (scatter 4.8 -> 3.5 ms radix).  sbin keeps one byte per staged tuple,
 // bin >> sh (sh = log2(nbins) - 8 above 256 bins); the low sh bits come
 // back from the staged bin starts.
+// CLAIM: the first pass of sample-sized regions: tile t's run of bin i goes
+// to region cl.cb0 + tiles[t].aux * cl.cmul + i (a row-major (row, bin)
+// layout, so a tile's claims are contiguous words), claimed from
+// cl.cursors; a run past its region's end sets *cl.overflow and is not
+// written.  cl.mm (optional, {2^64 - 1, 0} beforehand): the exact key range
+// of the launch's tuples.
+struct Claims64 {
+    const uint32_t* starts;
+    uint32_t* cursors;
+    uint32_t* overflow;
+    uint32_t cb0, cmul;
+    unsigned long long* mm;
+};
+
+template <bool CLAIM>
 __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     const ulonglong2* __restrict__ in, ulonglong2* __restrict__ out,
     const TileDesc* __restrict__ tiles, const uint32_t* __restrict__ ntiles_dev,
     uint32_t tile_base, uint32_t tile_end, const uint32_t* __restrict__ offs, Bin64 b,
-    uint32_t nbins) {
+    uint32_t nbins, Claims64 cl) {
+    static_assert(kSm64MaxBins <= kSm64ScBlock, "one bin per thread");
     __shared__ ulonglong2 stage[kSm64ScTile];
     __shared__ uint8_t sbin[kSm64ScTile];
     __shared__ uint32_t scratch[kSm64ScBlock / TBJ_WAVE + 1];
-    // dynamic LDS: the learned model (if any), then the bin starts and the
-    // bins' global offsets (nbins words each)
+    __shared__ unsigned long long bmm[2];
+    // dynamic LDS: the learned model (if any), then the bin starts, the
+    // bins' global offsets and (CLAIM) their region ends (nbins words each)
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     uint32_t* loff = reinterpret_cast<uint32_t*>(dyn_lds + sm64_model_bytes_aligned(b.kind));
     uint32_t* goff = loff + nbins;
+    uint32_t* glim = goff + nbins;
     const uint32_t end = min(tile_end, *ntiles_dev);
     const uint32_t n = end > tile_base ? end - tile_base : 0;
     if (blockIdx.x >= n) return;
@@ -437,6 +452,25 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
         if (j < td.len) v[k] = in[td.in_off + j];
     }
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) loff[i] = 0;
+    if (CLAIM && cl.mm && threadIdx.x == 0) {
+        bmm[0] = ~0ull;
+        bmm[1] = 0;
+    }
+    // this thread's bin (one per thread): its region's bounds, loaded early
+    const uint32_t r0 = CLAIM ? cl.cb0 + td.aux * cl.cmul : 0;
+    uint32_t w0 = 0, w1 = 0;
+    if (CLAIM && threadIdx.x < nbins) {
+        w0 = cl.starts[r0 + threadIdx.x];
+        w1 = cl.starts[r0 + threadIdx.x + 1];
+    }
+    // the stored range, read now so that the filter below does not wait for
+    // it (it only narrows: a stale value lets an unneeded atomic through)
+    unsigned long long smin = 0, smax = ~0ull;
+    if (CLAIM && cl.mm && threadIdx.x == 0) {
+        const volatile unsigned long long* vmm = cl.mm;
+        smin = vmm[0];
+        smax = vmm[1];
+    }
     __syncthreads();
     uint32_t bb[kSm64ScItems], rank[kSm64ScItems];
 #pragma unroll
@@ -446,10 +480,39 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     }
 #pragma unroll
     for (uint32_t k = 0; k < kSm64ScItems; k++) rank[k] = agg_inc(loff, bb[k]);
+    if (CLAIM && cl.mm) {  // tile min / max: waves -> LDS -> one filtered atomic pair
+        uint64_t mn = ~0ull, mx = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kSm64ScItems; k++)
+            if (k * kSm64ScBlock + threadIdx.x < td.len) {
+                mn = min(mn, (uint64_t)v[k].x);
+                mx = max(mx, (uint64_t)v[k].x);
+            }
+        wave_minmax_u64(mn, mx);
+        if (lane_id() == 0) {
+            atomicMin(&bmm[0], (unsigned long long)mn);
+            atomicMax(&bmm[1], (unsigned long long)mx);
+        }
+    }
     __syncthreads();
+    if (CLAIM && cl.mm && threadIdx.x == 0) {
+        if (bmm[0] < smin) atomicMin(&cl.mm[0], bmm[0]);
+        if (bmm[1] > smax) atomicMax(&cl.mm[1], bmm[1]);
+    }
+    // claims: one atomic per non-empty bin, awaited after the scan
+    const uint32_t cnt = CLAIM && threadIdx.x < nbins ? loff[threadIdx.x] : 0;
+    const uint32_t cbase = CLAIM && cnt ? atomicAdd(&cl.cursors[r0 + threadIdx.x], cnt) : 0;
     const uint32_t nvalid = block_scan_lds<kSm64ScBlock>(loff, nbins, scratch);
-    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock)
-        goff[i] = offs[(uint64_t)t * nbins + i] - loff[i];
+    if (CLAIM) {
+        if (threadIdx.x < nbins) {
+            if (cnt && cbase + cnt > w1 - w0) atomicOr(cl.overflow, 1u);
+            goff[threadIdx.x] = w0 + cbase - loff[threadIdx.x];
+            glim[threadIdx.x] = w1;
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock)
+            goff[i] = offs[(uint64_t)t * nbins + i] - loff[i];
+    }
     const uint32_t sh = nbins > 256 ? 23 - __clz(nbins) : 0;  // log2(nbins) - 8
 #pragma unroll
     for (uint32_t k = 0; k < kSm64ScItems; k++) {
@@ -465,13 +528,108 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
         const uint32_t base = (uint32_t)sbin[j] << sh;
         uint32_t bj = base;
         for (uint32_t k = 1; k < (1u << sh); k++) bj += loff[base + k] <= j ? 1u : 0u;
-        out[goff[bj] + j] = stage[j];
+        const uint32_t pos = goff[bj] + j;
+        if (!CLAIM || pos < glim[bj]) out[pos] = stage[j];
+    }
+}
+
+// Sample of the first pass of sample-sized regions: one workgroup per chunk
+// of kS64SampleTiles tiles of one row (rows of R, then of S); exactly one
+// 16-tuple group of every 256-tuple block is read (sampled_group), binned
+// with the pass's function (S without the drop test: R's range is not known
+// yet) and counted per (row, bin) into cnt1[(rel ? F1 * nrowR : 0) + row *
+// F1 + bin].
+constexpr uint32_t kS64SampleTiles = 112;
+constexpr uint32_t kS64ChunksPerRow = (kRowTiles + kS64SampleTiles - 1) / kS64SampleTiles;
+
+__global__ __launch_bounds__(256) void k_sample_rows64(
+    const ulonglong2* __restrict__ R, uint32_t nR, const ulonglong2* __restrict__ S, uint32_t nS,
+    uint32_t nrowR, uint32_t nrowS, Bin64 bsr, Bin64 bss, uint32_t F1,
+    uint32_t* __restrict__ cnt1) {
+    __shared__ uint32_t c1[kSm64MaxBins];
+    extern __shared__ __align__(16) uint8_t dyn_lds[];
+    const uint32_t rowg = blockIdx.x / kS64ChunksPerRow, chunk = blockIdx.x % kS64ChunksPerRow;
+    const uint32_t rel = rowg < nrowR ? 0 : 1;
+    const uint32_t row = rel ? rowg - nrowR : rowg;
+    const ulonglong2* in = rel ? S : R;
+    const uint32_t n = rel ? nS : nR;
+    const uint32_t ntiles = (n + kSm64Span - 1) / kSm64Span;
+    const uint32_t rpe = rows_per_eighth(ntiles);
+    const uint32_t x = row / rpe, c = row % rpe;
+    const uint32_t e0 = eighth_start(x, ntiles), e1 = eighth_start(x + 1, ntiles);
+    const uint32_t r0 = min(e0 + c * kRowTiles, e1), r1 = min(r0 + kRowTiles, e1);
+    const uint32_t t0 = min(r0 + chunk * kS64SampleTiles, r1), t1 = min(t0 + kS64SampleTiles, r1);
+    if (t0 == t1) return;  // whole workgroup
+    const Bin64Fn bin = make_bin64(rel ? bss : bsr, reinterpret_cast<LdsModel64*>(dyn_lds));
+    for (uint32_t i = threadIdx.x; i < F1; i += 256) c1[i] = 0;
+    __syncthreads();
+    const uint32_t lo = t0 * kSm64Span, hi = min(t1 * kSm64Span, n);
+    for (uint32_t blk = lo / 256 + threadIdx.x; blk < (hi + 255) / 256; blk += 256) {
+        const uint32_t first = blk * 256 + sampled_group(blk) * 16;
+        uint64_t key[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {  // all loads in flight before any use
+            const uint32_t i = first + k;
+            key[k] = in[i >= lo && i < hi ? i : lo].x;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) {
+            const uint32_t i = first + k;
+            if (i >= lo && i < hi) {
+                const uint32_t bb = bin(key[k]);
+                if (bb != kNone) atomicAdd(&c1[bb], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t pbase = rel ? F1 * nrowR : 0;
+    for (uint32_t i = threadIdx.x; i < F1; i += 256)
+        if (c1[i]) atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
+}
+
+// Second-pass segments of sample-sized first-pass regions: segment p = first-
+// pass partition (rel, b1) = (p / F1, p % F1), whose tiles are the tiles of
+// its rows' regions, [first[pos0], first[pos0 + nrow]) with pos0 = the
+// partition's first region in partition order (k_region_tiles' scan).
+__global__ __launch_bounds__(kScanBlock) void k_part_segments64(
+    const uint32_t* __restrict__ first, uint32_t F1, uint32_t F2, uint32_t nrowR, uint32_t nrowS,
+    SegDesc* __restrict__ segs2) {
+    __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t gcarry_s;
+    if (threadIdx.x == 0) gcarry_s = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < 2 * F1; base += kScanBlock) {
+        const uint32_t p = base + threadIdx.x;
+        uint32_t ft = 0, nt = 0;
+        if (p < 2 * F1) {
+            const uint32_t rel = p / F1, b1 = p % F1, nrow = rel ? nrowS : nrowR;
+            const uint32_t pos0 = (rel ? F1 * nrowR : 0) + b1 * nrow;
+            ft = first[pos0];
+            nt = first[pos0 + nrow] - ft;
+        }
+        const uint32_t ng = seg_groups(ft, nt);
+        uint32_t gtotal;
+        const uint32_t gex = block_exclusive_scan<kScanBlock>(ng, scratch, &gtotal);
+        if (p < 2 * F1) {
+            SegDesc d;
+            d.in_off = 0;  // the tiles carry their own offsets
+            d.n = 0;
+            d.ntiles = nt;
+            d.first_tile = ft;
+            d.ngroups = ng;
+            d.aux = p;
+            d.hist_base = (uint64_t)(gcarry_s + gex) * F2;
+            segs2[p] = d;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) gcarry_s += gtotal;
+        __syncthreads();
     }
 }
 
 __global__ void k_make_tiles64(TileDesc* __restrict__ tiles, SegDesc* __restrict__ segs,
                                uint32_t nR, uint32_t nS, uint32_t nbins,
-                               uint32_t* __restrict__ ntiles_dev) {
+                               uint32_t* __restrict__ ntiles_dev, uint32_t rows) {
     const uint32_t tR = (nR + kSm64Span - 1) / kSm64Span, tS = (nS + kSm64Span - 1) / kSm64Span;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < tR + tS) {
@@ -481,7 +639,7 @@ __global__ void k_make_tiles64(TileDesc* __restrict__ tiles, SegDesc* __restrict
         d.in_off = k * kSm64Span;
         d.len = min((uint32_t)kSm64Span, n - k * kSm64Span);
         d.seg = r ? 0 : 1;
-        d.aux = 0;
+        d.aux = rows ? tile_row(k, r ? tR : tS) : 0;  // sample-sized: the tile's row
         tiles[i] = d;
     }
     if (i == 0) {
@@ -742,10 +900,7 @@ __device__ void chunk_sort(const ulonglong2 (&v)[kSortItems], uint32_t n, SortLd
             mx = max(mx, (uint64_t)v[k].x);
         }
     }
-    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
-        mn = min(mn, (uint64_t)__shfl_xor((unsigned long long)mn, d, TBJ_WAVE));
-        mx = max(mx, (uint64_t)__shfl_xor((unsigned long long)mx, d, TBJ_WAVE));
-    }
+    wave_minmax_u64(mn, mx);
     __syncthreads();
     if (lane_id() == 0) {
         atomicMin(&L.mm[0], (unsigned long long)mn);
@@ -1403,10 +1558,25 @@ void sm64_fit(hipStream_t st, const ulonglong2* R, uint64_t nR, uint32_t m, uint
 }
 
 void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
-                     uint32_t nbins, uint32_t* ntiles_dev) {
+                     uint32_t nbins, uint32_t* ntiles_dev, bool rows) {
     const uint32_t t = (nR + kSm64Span - 1) / kSm64Span + (nS + kSm64Span - 1) / kSm64Span;
     hipLaunchKernelGGL(k_make_tiles64, dim3(t ? (t + 255) / 256 : 1), dim3(256), 0, st, tiles,
-                       segs, nR, nS, nbins, ntiles_dev);
+                       segs, nR, nS, nbins, ntiles_dev, (uint32_t)rows);
+}
+
+void sm64_sample_rows(hipStream_t st, const ulonglong2* R, uint32_t nR, const ulonglong2* S,
+                      uint32_t nS, uint32_t nrowR, uint32_t nrowS, const Bin64& bsr,
+                      const Bin64& bss, uint32_t F1, uint32_t* cnt1) {
+    const uint32_t rows = nrowR + nrowS;
+    if (!rows) return;
+    hipLaunchKernelGGL(k_sample_rows64, dim3(rows * kS64ChunksPerRow), dim3(256),
+                       sm64_model_lds(bsr), st, R, nR, S, nS, nrowR, nrowS, bsr, bss, F1, cnt1);
+}
+
+void sm64_part_segments(hipStream_t st, const uint32_t* first, uint32_t F1, uint32_t F2,
+                        uint32_t nrowR, uint32_t nrowS, SegDesc* segs2) {
+    hipLaunchKernelGGL(k_part_segments64, dim3(1), dim3(kScanBlock), 0, st, first, F1, F2, nrowR,
+                       nrowS, segs2);
 }
 
 void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDesc* tiles,
@@ -1419,11 +1589,24 @@ void sm64_hist(hipStream_t st, uint32_t grid, const ulonglong2* in, const TileDe
 
 void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong2* out,
                   const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
-                  const uint32_t* offs, const Bin64& b, uint32_t nbins) {
+                  const uint32_t* offs, const Bin64& b, uint32_t nbins,
+                  const Sm64Claims* claims) {
     if (!grid) return;
-    const size_t lds = sm64_model_bytes_aligned(b.kind) + 2 * sizeof(uint32_t) * nbins;
-    hipLaunchKernelGGL(k_scatter64, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
-                       tiles, ntiles_dev, tile_base, tile_base + grid, offs, b, nbins);
+    const size_t lds = sm64_model_bytes_aligned(b.kind) + 3 * sizeof(uint32_t) * nbins;
+    Claims64 cl{};
+    if (claims) {
+        cl.starts = claims->starts;
+        cl.cursors = claims->cursors;
+        cl.overflow = claims->overflow;
+        cl.cb0 = claims->cb0;
+        cl.cmul = claims->cmul;
+        cl.mm = claims->mm;
+        hipLaunchKernelGGL(k_scatter64<true>, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
+                           tiles, ntiles_dev, tile_base, tile_base + grid, offs, b, nbins, cl);
+    } else {
+        hipLaunchKernelGGL(k_scatter64<false>, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
+                           tiles, ntiles_dev, tile_base, tile_base + grid, offs, b, nbins, cl);
+    }
 }
 
 void sm64_segments(hipStream_t st, const uint32_t* scanned1, const SegDesc* segs1, uint32_t nbins1,

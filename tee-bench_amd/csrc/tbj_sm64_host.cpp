@@ -25,6 +25,9 @@ struct Small64 {  // device scalars, one allocation
     uint32_t ntiles1, ntiles2, nrtasks, njtasks;
     uint32_t max_r, total1, total2;
     uint32_t lin;  // the learned fit gave the linear model
+    uint32_t overflow;      // a sample-sized region was too small
+    uint32_t region_total;  // sample-sized first pass: region capacity in all
+    uint32_t ntiles2_pad[2];
     unsigned long long range[2];     // exact R key range (learned partitioner)
     unsigned long long counters[2];  // matches, checksum
     unsigned long long volume;       // pairs over all chunks
@@ -50,9 +53,17 @@ Bin64 bin64(uint32_t kind, uint32_t B, uint32_t shift, uint32_t fanout, const Mo
 
 }  // namespace
 
-int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tbj_tuple64* S,
-             uint64_t nS64, tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt,
-             tbj_result* res, hipStream_t st) {
+// Joins of at least this many tuples (|R| + |S|) size their first pass from
+// a sample when tbj_options.sampled_regions is 0 (auto).
+constexpr uint64_t kSampledMin64 = 1ull << 24;
+
+// One run; *overflow_out: a sample-sized region was too small (the run then
+// stops before the join, to be redone with exact histograms).
+static int run_sm64_once(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64,
+                  const tbj_tuple64* S, uint64_t nS64, tbj_pair64* pairs, uint64_t pairs_cap,
+                  const tbj_options& opt, bool allow_sampled, tbj_result* res, hipStream_t st,
+                  bool* overflow_out) {
+    *overflow_out = false;
     if (nR64 + nS64 >= (1ull << 32) - (1ull << 20)) return TBJ_ERR_TOO_LARGE;
     bool learned = opt.partitioner == TBJ_PART_LEARNED ||
                    (opt.partitioner == TBJ_PART_AUTO && algo == TBJ_PSM);
@@ -83,8 +94,20 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
     Small64* sm = nullptr;
     TBJ_TRY(W.get("s64_small", sizeof(Small64), (void**)&sm));
     const uint64_t ntot = (uint64_t)nR + nS;
+    // sample-sized first pass (two-pass plans): its regions leave slack, so C1
+    // takes the largest sum of capacities the rule allows (sum of 16 x counts
+    // <= |R| + |S| plus the sampled blocks' rounding; sum of sqrt <=
+    // sqrt(n x sum))
+    const uint32_t tR = (nR + kSm64Span - 1) / kSm64Span, tS = (nS + kSm64Span - 1) / kSm64Span;
+    const uint32_t nrowR = 8 * rows_per_eighth(tR), nrowS = 8 * rows_per_eighth(tS);
+    const uint32_t np1 = F1 * (nrowR + nrowS);
+    const double e1 = (double)ntot + 256.0 * (nrowR + nrowS);
+    const double cap1 = 1.1 * e1 + 48.0 * sqrt((double)np1 * e1) + 640.0 * np1 + 1;
+    const bool sampled = allow_sampled && passes == 2 && cap1 < 4294967295.0 &&
+                         (opt.sampled_regions > 0 ||
+                          (opt.sampled_regions == 0 && ntot >= kSampledMin64));
     ulonglong2 *C1 = nullptr, *C2 = nullptr;
-    TBJ_TRY(W.get("c1", sizeof(ulonglong2) * ntot, (void**)&C1));
+    TBJ_TRY(W.get("c1", sizeof(ulonglong2) * (sampled ? (uint64_t)cap1 : ntot), (void**)&C1));
     TBJ_TRY(W.get("c2", sizeof(ulonglong2) * ntot, (void**)&C2));
     uint2* bounds = nullptr;
     TBJ_TRY(W.get("bounds", sizeof(uint2) * 2 * nparts, (void**)&bounds));
@@ -108,7 +131,6 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
         Tmp = C2;
         TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(nparts + 1), (void**)&partial));
     } else {
-        const uint32_t tR = (nR + kSm64Span - 1) / kSm64Span, tS = (nS + kSm64Span - 1) / kSm64Span;
         const uint32_t T1 = tR + tS;
         const uint32_t T2max = passes == 2 ? (uint32_t)(ntot / kSm64Span) + 2 * F1 + 2 : 0;
         // group-sum lengths: R's and S's groups; second pass at most
@@ -146,6 +168,61 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
             s1r = s1s = bin64(kBinRadix, B, 0, F1, nullptr, nullptr);
             s2 = bin64(kBinRadix, B, b1, F2, nullptr, nullptr);
         }
+        if (sampled) {
+            // ---- pass 1 from a sample: regions per (row, bin), runs claimed
+            uint32_t *cnt1 = nullptr, *starts1 = nullptr, *cursors1 = nullptr, *first2 = nullptr,
+                     *npart = nullptr;
+            const uint32_t T2s = (uint32_t)(ntot / kSm64Span) + np1 + 1;
+            const uint64_t L2s = (uint64_t)F2 * (T2s / kGroupTiles + 2ull * (2 * F1) + 1) + 1;
+            TBJ_TRY(W.get("s64_cnt1", sizeof(uint32_t) * np1, (void**)&cnt1));
+            TBJ_TRY(W.get("s64_starts1", sizeof(uint32_t) * (np1 + 1), (void**)&starts1));
+            TBJ_TRY(W.get("s64_cursors1", sizeof(uint32_t) * np1, (void**)&cursors1));
+            TBJ_TRY(W.get("s64_first2", sizeof(uint32_t) * (np1 + 1), (void**)&first2));
+            TBJ_TRY(W.get("s64_npart", sizeof(uint32_t) * 2 * F1, (void**)&npart));
+            TBJ_TRY(W.get("tiles2", sizeof(TileDesc) * T2s, (void**)&tiles2));
+            TBJ_TRY(W.get("segs2", sizeof(SegDesc) * 2 * F1, (void**)&segs2));
+            TBJ_TRY(W.get("hist2", sizeof(uint16_t) * F2 * T2s, (void**)&hist2));
+            TBJ_TRY(W.get("offs2", sizeof(uint32_t) * F2 * T2s, (void**)&offs2));
+            TBJ_TRY(W.get("gs2", sizeof(uint32_t) * L2s, (void**)&gs2));
+            TBJ_TRY(W.get("partial2",
+                          sizeof(uint32_t) * scan_partials(std::max<uint64_t>(
+                                                 std::max<uint64_t>(np1 + 1, L2s), nparts + 1)),
+                          (void**)&partial));
+            launch_fill_u32(st, cnt1, np1, 0);
+            launch_fill_u32(st, cursors1, np1, 0);
+            sm64_make_tiles(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1, true);
+            Bin64 s1s_sample = s1s;
+            s1s_sample.range = nullptr;  // R's range comes with R's scatter
+            sm64_sample_rows(st, Rin, nR, Sin, nS, nrowR, nrowS, s1r, s1s_sample, F1, cnt1);
+            TBJ_TRY(pt.mark(kPhHist1));
+            launch_row_caps(st, cnt1, np1, starts1);
+            launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
+            TBJ_TRY(pt.mark(kPhScan1));
+            // R's scatter measures R's exact range (learned: S drops keys outside it)
+            const Sm64Claims c1r{starts1, cursors1, &sm->overflow, 0, F1,
+                                 learned ? sm->range : nullptr};
+            const Sm64Claims c1s{starts1, cursors1, &sm->overflow, F1 * nrowR, F1, nullptr};
+            sm64_scatter(st, tR, Rin, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, &c1r);
+            sm64_scatter(st, tS, Sin, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, &c1s);
+            TBJ_TRY(pt.mark(kPhScatter1));
+            // ---- pass 2 (exact) over the filled regions, one segment per
+            // first-pass partition
+            launch_part_sizes(st, starts1, cursors1, F1, nrowR, nrowS, npart, &sm->total1);
+            launch_region_tiles(st, starts1, cursors1, np1, F1, nrowR, nrowS, first2, kSm64Span);
+            launch_scan(st, first2, (uint64_t)np1 + 1, partial, &sm->ntiles2);
+            launch_region_tile_descs(st, starts1, cursors1, first2, np1, F1, nrowR, nrowS, tiles2,
+                                     kSm64Span, true);
+            sm64_part_segments(st, first2, F1, F2, nrowR, nrowS, segs2);
+            sm64_hist(st, T2s, C1, tiles2, &sm->ntiles2, 0, hist2, s2, F2, nullptr);
+            launch_tile_offsets(st, T2s, hist2, offs2, tiles2, segs2, &sm->ntiles2, F2, gs2, L2s,
+                                partial, &sm->total2);
+            TBJ_TRY(pt.mark(kPhSize2));
+            sm64_scatter(st, T2s, C1, C2, tiles2, &sm->ntiles2, 0, offs2, s2, F2);
+            sm64_bounds(st, gs2, segs2, F1, F2, 2, nR, nS, bounds);
+            TBJ_TRY(pt.mark(kPhScatter2));
+            Pb = C2;
+            Tmp = C1;
+        } else {
         // ---- pass 1
         sm64_make_tiles(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
         sm64_hist(st, tR, Rin, tiles1, &sm->ntiles1, 0, hist1, s1r, F1,
@@ -183,6 +260,7 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
             Pb = C2;
             Tmp = C1;
         }
+        }  // exact passes
     }
     TBJ_HIP(hipEventRecord(C.ev[1], st));
 
@@ -196,6 +274,10 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
     Small64 h;
     TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small64), hipMemcpyDeviceToHost, st));
     TBJ_HIP(hipStreamSynchronize(st));
+    if (h.overflow) {  // redo with exact histograms before any join work
+        *overflow_out = true;
+        return TBJ_OK;
+    }
     res->max_r_partition = h.max_r;
     if (learned) res->s_dropped = (uint64_t)nS - ((uint64_t)h.total1 - nR);
     res->learned_model = learned ? (h.lin ? 2 : 1) : 0;
@@ -281,6 +363,22 @@ int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR64, const tb
     res->ms_join = ms;
     TBJ_HIP(hipEventElapsedTime(&ms, C.ev[0], C.ev[2]));
     res->ms_total = ms;
+    return TBJ_OK;
+}
+
+int run_sm64(Context& C, int algo, const tbj_tuple64* R, uint64_t nR, const tbj_tuple64* S,
+             uint64_t nS, tbj_pair64* pairs, uint64_t pairs_cap, const tbj_options& opt,
+             tbj_result* res, hipStream_t st) {
+    bool overflow = false;
+    TBJ_TRY(run_sm64_once(C, algo, R, nR, S, nS, pairs, pairs_cap, opt, opt.sampled_regions >= 0,
+                          res, st, &overflow));
+    if (!overflow) return TBJ_OK;
+    const int32_t learned_model = res->learned_model;
+    memset(res, 0, sizeof(*res));
+    TBJ_TRY(run_sm64_once(C, algo, R, nR, S, nS, pairs, pairs_cap, opt, false, res, st,
+                          &overflow));
+    res->redone_exact = 1;
+    if (!res->learned_model) res->learned_model = learned_model;
     return TBJ_OK;
 }
 

@@ -127,6 +127,53 @@ def test_small_single_tasks_with_duplicates(tbj, oracle, algo):
     check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
 
 
+def sampled_positions(n):
+    """Tuples a sample-sized first pass reads: exactly one 16-tuple group in
+    every block of 256 tuples, at a hashed position."""
+    i = np.arange(n, dtype=np.uint64)
+    block = i >> np.uint64(8)
+    pick = ((block * np.uint64(2654435761)) & np.uint64(0xFFFFFFFF)) >> np.uint64(28)
+    return ((i >> np.uint64(4)) & np.uint64(15)) == pick
+
+
+@pytest.mark.parametrize("algo", ["PSM", "RSM"])
+@pytest.mark.parametrize("z", [0.5, 1.5])
+def test_sample_sized_first_pass(tbj, oracle, algo, z):
+    # the first pass sized from a 1/16 sample (regions per tile row and bin,
+    # runs claimed), the second pass exact over the filled regions
+    R = tbj.gen_unique64(21, 1 << 18)
+    S = tbj.gen_zipf_fk64(22, 1 << 19, R, z)
+    res, got = run64(tbj, algo, R, S, sampled_regions=True, radix_bits=16, passes=2)
+    assert res.passes == 2 and res.redone_exact == 0
+    check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
+
+
+def test_sample_sized_overflow_redoes_exact(tbj, oracle):
+    # the tuples the sample sees carry one key and the others another, whose
+    # regions the sample believes empty
+    rng = np.random.default_rng(23)
+    rk = rng.choice(1 << 62, 1 << 14, replace=False).astype(np.uint64) + np.uint64(1)
+    nS = 1 << 17
+    sk = np.where(sampled_positions(nS), rk[0], rk[1]).astype(np.uint64)
+    R, S = rel64(rk), rel64(sk)
+    res, got = run64(tbj, "RSM", R, S, sampled_regions=True, radix_bits=12, passes=2)
+    assert res.redone_exact == 1 and res.matches == nS
+    check_pairs(oracle, R, S, got, res, ordered=False)
+
+
+def test_sample_sized_learned_drops_outside_range(tbj, oracle):
+    # R's range measured by R's claiming scatter; S keys outside it dropped
+    rng = np.random.default_rng(24)
+    rk = rng.integers(1 << 40, 1 << 41, 200000, dtype=np.uint64)
+    rk = np.unique(rk)
+    sk = np.concatenate([rng.choice(rk, 300000), rng.integers(0, 1 << 40, 40000, dtype=np.uint64),
+                         rng.integers(1 << 41, 1 << 63, 60000, dtype=np.uint64)])
+    R, S = rel64(rk), rel64(sk[rng.permutation(len(sk))])
+    res, got = run64(tbj, "PSM", R, S, sampled_regions=True, radix_bits=14, passes=2)
+    assert res.passes == 2 and res.redone_exact == 0 and res.s_dropped == 100000
+    check_pairs(oracle, R, S, got, res, ordered=True)
+
+
 def test_learned_oversized_unique_partitions(tbj, oracle):
     # 6 bits for 2^18 unique keys: every range partition spans two chunks
     R = tbj.gen_unique64(9, 1 << 18)

@@ -177,9 +177,11 @@ def measured_traffic(kernel, workload_name):
     return None
 
 
-def measured_step_bytes(workload_name):
-    """HBM bytes of one whole step (all kernels) from the committed counter summary."""
-    path = os.path.join(ROOT, "profiles", f"traffic_step_{workload_name}.txt")
+def measured_step_bytes(workload_name, algo="RHO"):
+    """HBM bytes of one whole step (all kernels) from the committed counter
+    summary of that algorithm (RHO's is traffic_step_<workload>.txt)."""
+    tag = workload_name if algo == "RHO" else f"{algo.lower()}_{workload_name}"
+    path = os.path.join(ROOT, "profiles", f"traffic_step_{tag}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -197,11 +199,11 @@ def roofline(kbytes, kms, ksym, wl, **extra):
     return d
 
 
-def step_roofline(b_alg, ms_per_step, wl):
+def step_roofline(b_alg, ms_per_step, wl, algo):
     a = b_alg / (ms_per_step * 1e-3) / 1e9
     return {"achieved": round(a, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(a / HBM_PEAK_GBS, 4), "algorithmic_bytes": b_alg,
-            "measured_bytes": measured_step_bytes(wl)}
+            "measured_bytes": measured_step_bytes(wl, algo)}
 
 
 def calibrate_copy():
@@ -343,7 +345,7 @@ def bench_join32(args, tb):
                    "phase_ms": {k: round(v, 4) for k, v in phases.items()},
                    "gen_seconds": round(gen_s, 1)},
         "roofline": roofline(kbytes, kms, ksym, args.workload),
-        "step_roofline": step_roofline(b_alg, ms_per_step, args.workload),
+        "step_roofline": step_roofline(b_alg, ms_per_step, args.workload, algo),
         "cpu_baseline": None,
     }
     if comparison:
@@ -484,7 +486,7 @@ def bench_batch(args, tb):
                    "gen_seconds": round(B["gen_s"], 1)},
         "roofline": roofline(b_alg, dev_ms, "batch", "c4",
                              note="whole batch (all kernels of the graph), not one kernel"),
-        "step_roofline": step_roofline(b_alg, ms_per_step, "c4"),
+        "step_roofline": step_roofline(b_alg, ms_per_step, "c4", "batch"),
         "cpu_baseline": None,
     }
 
@@ -561,7 +563,7 @@ def bench_join64(args, tb):
                    "comparison": side(tb, res2, el2, 2 * n, other),
                    "gen_seconds": round(gen_s, 1)},
         "roofline": roofline(kbytes, kms, ksym, "c5"),
-        "step_roofline": step_roofline(b_alg, ms_per_step, "c5"),
+        "step_roofline": step_roofline(b_alg, ms_per_step, "c5", algo),
         "cpu_baseline": None,
     }
 

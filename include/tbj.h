@@ -94,7 +94,13 @@ typedef struct tbj_options {
                                    |R| + |S| >= 2^24), 1 = always, -1 = never */
     int32_t profile;            /* 1: record per-phase device times in phase_ms */
     int32_t output;             /* enum tbj_output */
-    int32_t reserved[2];
+    int32_t bulk_build;         /* PHT / NPO_st: build the hash table from R
+                                   partitioned on the bucket's low bits (one
+                                   contiguous block of buckets per partition,
+                                   filled in LDS; no global atomic per tuple)
+                                   instead of one atomic insert per tuple.
+                                   0 = auto (|R| >= 2^20), 1 = always, -1 = never */
+    int32_t reserved[1];
 } tbj_options;
 
 typedef struct tbj_result {

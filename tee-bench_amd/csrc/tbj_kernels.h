@@ -194,9 +194,18 @@ size_t pht_bucket_bytes();
 size_t pht_node_bytes();
 void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, uint32_t mask,
                       void* pool, uint32_t* pool_ctr);
+// Bulk build: one block of 2^hbits buckets (hbits <= kPhtPlaceHBits) per
+// partition of Rb (partition_relation on the low pbits bits, first pass b1
+// bits; bounds: its partition bounds), overflow nodes claimed from *pool_ctr
+// (zeroed).  The table (2^(pbits + hbits) buckets) needs no clearing.
+constexpr uint32_t kPhtPlaceHBits = 10;
+void launch_pht_place(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
+                      uint32_t pbits, uint32_t hbits, void* tab, void* pool, uint32_t* pool_ctr);
+// pbits, hbits: the table's layout (pht_slot); pbits = 0 for a table built by
+// launch_pht_build.
 void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,
-                      uint32_t mask, const void* pool, uint2* pairs, uint64_t cap,
-                      unsigned long long* counters);
+                      uint32_t mask, uint32_t pbits, uint32_t hbits, const void* pool,
+                      uint2* pairs, uint64_t cap, unsigned long long* counters);
 // CHT state on the device: overflow count (zeroed by the caller), phase-A
 // spills, whether the dense array is exactly full, and the number of set bits.
 struct ChtState {

@@ -2,7 +2,9 @@
 // hash table), each built in HBM with global atomics and probed by streaming S.
 //
 // PHT restates no_partitioning_join.h:119-356,219-259 with nextpow2(|R|/2)
-// buckets (the table of tbj_pht.h).
+// buckets (the table of tbj_pht.h).  Large tables are built in bulk: R is
+// partitioned on the buckets' low bits first, and each partition's buckets,
+// one contiguous block, are filled in LDS and written whole.
 //
 // CHT restates CHT.hpp:26-254 / CHTJoin.hpp:46-322: a bitmap of
 // nextpow2(|R|) bits in 32-bit words with a running popcount per word, a
@@ -49,6 +51,88 @@ __global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, 
     }
 }
 
+// Bulk build.  R is partitioned beforehand on the low pbits bits of the key
+// (partition p holds the keys whose bits [0, b1) are p / F2 and bits [b1,
+// pbits) are p % F2), and the table stores bucket b at pht_slot(b, pbits,
+// hbits), so partition p's 2^hbits buckets form the block starting at its
+// low bits << hbits.  One workgroup per partition: counts its tuples per
+// bucket (LDS), claims pool nodes for the tuples past each bucket's two slots
+// with one atomic (a bucket's overflow tuples take consecutive nodes, chained
+// in order), places the tuples in an LDS copy of the block and writes the
+// block out whole.  Every bucket is written, so the table needs no clearing.
+// The buckets hold what the per-tuple inserts of k_pht_build would (counts,
+// the same tuples), in another order inside each bucket.
+constexpr int kPlaceBlock = 256;
+
+__global__ __launch_bounds__(kPlaceBlock) void k_pht_place(
+    const uint2* __restrict__ Rb, const uint2* __restrict__ bounds, uint32_t b1, uint32_t F2,
+    uint32_t pbits, uint32_t hbits, PhtBucket* __restrict__ tab, OvNode* __restrict__ pool,
+    uint32_t* __restrict__ pool_ctr) {
+    __shared__ PhtBucket blk[1u << kPhtPlaceHBits];
+    __shared__ uint32_t cnt[1u << kPhtPlaceHBits];
+    __shared__ uint32_t ovp[1u << kPhtPlaceHBits];
+    __shared__ uint32_t pos[1u << kPhtPlaceHBits];
+    __shared__ uint32_t scratch[kPlaceBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t base_s;
+    const uint32_t p = blockIdx.x;
+    const uint32_t nb = 1u << hbits, hmask = nb - 1;
+    const uint32_t low = (p / F2) | ((p % F2) << b1);
+    const uint2 bd = bounds[p];
+    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) {
+        cnt[i] = 0;
+        pos[i] = 0;
+    }
+    __syncthreads();
+    // counts (a block-uniform loop: every lane of a wave calls agg_add)
+    for (uint32_t j0 = bd.x; j0 < bd.y; j0 += kPlaceBlock) {
+        const uint32_t j = j0 + threadIdx.x;
+        agg_add(cnt, j < bd.y ? (Rb[j].x >> pbits) & hmask : TBJ_DROP);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) ovp[i] = cnt[i] > 2 ? cnt[i] - 2 : 0;
+    __syncthreads();
+    const uint32_t nov = block_scan_lds<kPlaceBlock>(ovp, nb, scratch);
+    if (threadIdx.x == 0) base_s = nov ? atomicAdd(pool_ctr, nov) : 0;
+    __syncthreads();
+    const uint32_t base = base_s;
+    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) {
+        PhtBucket e;
+        e.count = cnt[i];
+        e.ovf = cnt[i] > 2 ? base + ovp[i] + 1 : 0;
+        e.slot[0] = e.slot[1] = make_uint2(0, 0);
+        e.pad[0] = e.pad[1] = 0;
+        blk[i] = e;
+    }
+    __syncthreads();
+    for (uint32_t j0 = bd.x; j0 < bd.y; j0 += kPlaceBlock) {
+        const uint32_t j = j0 + threadIdx.x;
+        uint2 t = make_uint2(0, 0);
+        uint32_t b = TBJ_DROP;
+        if (j < bd.y) {
+            t = Rb[j];
+            b = (t.x >> pbits) & hmask;
+        }
+        const uint32_t r = agg_inc(pos, b);
+        if (b == TBJ_DROP) continue;
+        if (r < 2) {
+            blk[b].slot[r] = t;
+        } else {
+            // node n; the next one (1-based n + 2) unless this is the
+            // bucket's last overflow tuple (r - 2 == cnt - 3)
+            const uint32_t n = base + ovp[b] + (r - 2);
+            OvNode nd;
+            nd.t = t;
+            nd.next = r + 1 < cnt[b] ? n + 2 : 0;
+            nd.pad = 0;
+            pool[n] = nd;
+        }
+    }
+    __syncthreads();
+    uint4* dst = reinterpret_cast<uint4*>(tab + ((uint64_t)low << hbits));
+    const uint4* src = reinterpret_cast<const uint4*>(blk);
+    for (uint32_t i = threadIdx.x; i < 2 * nb; i += kPlaceBlock) dst[i] = src[i];
+}
+
 // Block-aggregated output: every round each thread handles kProbeItems S
 // tuples; all their buckets are loaded before any is used (independent
 // loads in flight together), matches are counted, the block claims one
@@ -60,7 +144,8 @@ constexpr int kProbeItems = 8;
 template <bool MAT>
 __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
     const uint2* __restrict__ S, uint64_t nS, const PhtBucket* __restrict__ tab, uint32_t mask,
-    const OvNode* __restrict__ pool, uint2* __restrict__ pairs, uint64_t cap,
+    uint32_t pbits, uint32_t hbits, const OvNode* __restrict__ pool, uint2* __restrict__ pairs,
+    uint64_t cap,
     unsigned long long* __restrict__ counters) {
     __shared__ uint32_t scratch[kProbeBlock / TBJ_WAVE + 1];
     __shared__ unsigned long long obase_s;
@@ -81,7 +166,7 @@ __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(
         for (int k = 0; k < kProbeItems; k++) {
             const uint64_t j = base + k * kProbeBlock + threadIdx.x;
             if (j < nS) {
-                bk[k] = tab[sv[k].x & mask];
+                bk[k] = tab[pht_slot(sv[k].x & mask, pbits, hbits)];
             } else {
                 bk[k].count = 0;
                 bk[k].ovf = 0;
@@ -416,16 +501,25 @@ void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, ui
                        dim3(256), 0, st, R, nR, (PhtBucket*)tab, mask, (OvNode*)pool, pool_ctr);
 }
 
+void launch_pht_place(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
+                      uint32_t pbits, uint32_t hbits, void* tab, void* pool, uint32_t* pool_ctr) {
+    const uint32_t F2 = 1u << (pbits - b1);
+    hipLaunchKernelGGL(k_pht_place, dim3(1u << pbits), dim3(kPlaceBlock), 0, st, Rb, bounds, b1,
+                       F2, pbits, hbits, (PhtBucket*)tab, (OvNode*)pool, pool_ctr);
+}
+
 void launch_pht_probe(hipStream_t st, bool mat, const uint2* S, uint64_t nS, const void* tab,
-                      uint32_t mask, const void* pool, uint2* pairs, uint64_t cap,
-                      unsigned long long* counters) {
+                      uint32_t mask, uint32_t pbits, uint32_t hbits, const void* pool,
+                      uint2* pairs, uint64_t cap, unsigned long long* counters) {
     const uint32_t g = grid_for(nS, kProbeBlock * kProbeItems, 16384);
     if (mat)
         hipLaunchKernelGGL(k_pht_probe<true>, dim3(g), dim3(kProbeBlock), 0, st, S, nS,
-                           (const PhtBucket*)tab, mask, (const OvNode*)pool, pairs, cap, counters);
+                           (const PhtBucket*)tab, mask, pbits, hbits, (const OvNode*)pool, pairs,
+                           cap, counters);
     else
         hipLaunchKernelGGL(k_pht_probe<false>, dim3(g), dim3(kProbeBlock), 0, st, S, nS,
-                           (const PhtBucket*)tab, mask, (const OvNode*)pool, pairs, cap, counters);
+                           (const PhtBucket*)tab, mask, pbits, hbits, (const OvNode*)pool, pairs,
+                           cap, counters);
 }
 
 void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmask,

@@ -42,8 +42,15 @@ int finish(Context& C, hipStream_t st, Small* sm, bool mat, uint64_t cap, tbj_re
 
 }  // namespace
 
+// Relations at least this large build the PHT table in bulk when
+// tbj_options.bulk_build is 0 (auto).
+constexpr uint64_t kBulkMin = 1ull << 20;
+
 // PHT (no_partitioning_join.h:543-623) and NPO_st (474-541): one shared
-// table of nextpow2(|R| / 2) buckets (allocate_hashtable, 119-151).
+// table of nextpow2(|R| / 2) buckets (allocate_hashtable, 119-151), built by
+// one atomic insert per R tuple or, in bulk, from R partitioned on the
+// buckets' low bits (res->radix_bits and res->passes then report that
+// partitioning), and probed by streaming S.
 int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
             tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
             hipStream_t st) {
@@ -60,14 +67,39 @@ int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
         return finish(C, st, sm, mat, pairs_cap, res);
     }
     const uint32_t nb = std::max<uint32_t>(1, next_pow2(nR / 2));
+    uint32_t lg = 0;
+    while ((1u << lg) < nb) lg++;
+    const bool bulk = nR < (1ull << 31) &&
+                      (opt.bulk_build > 0 || (opt.bulk_build == 0 && nR >= kBulkMin));
     void *tab = nullptr, *pool = nullptr;
     TBJ_TRY(W.get("pht_tab", pht_bucket_bytes() * nb, &tab));
     TBJ_TRY(W.get("pht_pool", pht_node_bytes() * (nR + 1), &pool));
-    launch_fill_u4(st, tab, pht_bucket_bytes() * nb, 0);
-    launch_pht_build(st, (const uint2*)R, nR, tab, nb - 1, pool, &sm->pool_ctr);
+    uint32_t pbits = 0, hbits = lg;
+    if (bulk) {
+        hbits = std::min<uint32_t>(lg, kPhtPlaceHBits);
+        pbits = lg - hbits;
+        const uint2* Rb = (const uint2*)R;
+        const uint2* bounds = nullptr;
+        uint32_t b1 = 0;
+        if (pbits) {
+            TBJ_TRY(partition_relation(C, R, (uint32_t)nR, pbits, st, &Rb, &bounds, &b1));
+        } else {
+            // one block of buckets: all of R
+            uint2* bd = nullptr;
+            TBJ_TRY(W.get("pht_bound", sizeof(uint2) * 2, (void**)&bd));
+            launch_part_bounds(st, nullptr, nullptr, 1, 1, 0, (uint32_t)nR, 0, bd);
+            bounds = bd;
+        }
+        launch_pht_place(st, Rb, bounds, b1, pbits, hbits, tab, pool, &sm->pool_ctr);
+        res->radix_bits = (int32_t)pbits;
+        res->passes = pbits == 0 ? 0 : pbits > 8 ? 2 : 1;
+    } else {
+        launch_fill_u4(st, tab, pht_bucket_bytes() * nb, 0);
+        launch_pht_build(st, (const uint2*)R, nR, tab, nb - 1, pool, &sm->pool_ctr);
+    }
     TBJ_HIP(hipEventRecord(C.ev[1], st));
-    launch_pht_probe(st, mat, (const uint2*)S, nS, tab, nb - 1, pool, (uint2*)pairs,
-                     mat ? pairs_cap : 0, sm->counters);
+    launch_pht_probe(st, mat, (const uint2*)S, nS, tab, nb - 1, pbits, hbits, pool,
+                     (uint2*)pairs, mat ? pairs_cap : 0, sm->counters);
     TBJ_HIP(hipGetLastError());
     res->num_partitions = nb;
     return finish(C, st, sm, mat, pairs_cap, res);

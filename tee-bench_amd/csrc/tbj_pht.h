@@ -23,6 +23,14 @@ struct __align__(16) OvNode {
     uint32_t pad;
 };
 
+// Where bucket b sits in the table.  A table built in bulk (from R
+// partitioned on the bucket's low pbits bits) keeps the 2^hbits buckets of
+// one partition as one contiguous block: b -> ((b & (2^pbits - 1)) << hbits)
+// | (b >> pbits).  pbits = 0 is the plain array (b -> b).
+__device__ __forceinline__ uint32_t pht_slot(uint32_t b, uint32_t pbits, uint32_t hbits) {
+    return ((b & ((1u << pbits) - 1u)) << hbits) | (b >> pbits);
+}
+
 // Stores tuple t in bucket b, where it took position pos (the bucket's count
 // before this tuple's atomicAdd): one of the two slots, else a pool node at
 // the head of the bucket's overflow chain.

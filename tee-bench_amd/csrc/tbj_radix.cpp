@@ -550,6 +550,36 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
 
 }  // namespace
 
+int partition_relation(Context& C, const tbj_tuple* R, uint32_t nR, uint32_t bits,
+                       hipStream_t st, const uint2** Rb, const uint2** bounds, uint32_t* b1) {
+    if (bits == 0 || bits > 2 * (uint32_t)kMaxPassBits || nR == 0) return TBJ_ERR_INVALID;
+    Job J{};
+    J.algo = TBJ_RHO;
+    J.R = J.S = (const uint2*)R;
+    J.nR = nR;
+    J.nS = 0;
+    J.mode = kModeChain;
+    tbj_options o;
+    memset(&o, 0, sizeof(o));
+    o.partitioner = TBJ_PART_RADIX;
+    o.radix_bits = (int32_t)bits;
+    o.sampled_regions = -1;
+    tbj_result res{};
+    PhaseTimer pt(C, st, false);
+    Small* sm = nullptr;
+    TBJ_TRY(C.ws.get("prel_small", sizeof(Small), (void**)&sm));
+    TBJ_HIP(hipMemsetAsync(sm, 0, sizeof(Small), st));
+    Call k{C, C.ws, J, o, &res, st, pt, sm, {}, 0, nullptr, nullptr, nullptr, false, J.R, J.S, 1};
+    k.plan = plan_passes(bits, 0);
+    k.nparts = k.plan.F1 * k.plan.F2;
+    TBJ_TRY(C.ws.get("prel_bounds", sizeof(uint2) * 2 * k.nparts, (void**)&k.bounds));
+    TBJ_TRY(partition_exact(k));
+    *Rb = k.Rb;
+    *bounds = k.bounds;
+    *b1 = k.plan.b1;
+    return TBJ_OK;
+}
+
 int run_radix_join(Context& C, int algo, const tbj_tuple* R, uint64_t nR64, const tbj_tuple* S,
                    uint64_t nS64, void* pairs, uint64_t pairs_cap, const tbj_options& opt,
                    tbj_result* res, hipStream_t st) {

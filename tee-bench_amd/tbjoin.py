@@ -31,7 +31,8 @@ class Options(ctypes.Structure):
                 ("workspace_cap", ctypes.c_uint64), ("learned_bound", ctypes.c_uint32),
                 ("learned_samples", ctypes.c_int32), ("s_chunk", ctypes.c_int32),
                 ("sampled_regions", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("output", ctypes.c_int32), ("reserved", ctypes.c_int32 * 2)]
+                ("output", ctypes.c_int32), ("bulk_build", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 class Result(ctypes.Structure):
@@ -114,7 +115,7 @@ def init(device=0):
 
 def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
                  workspace_cap=0, learned_bound=0, learned_samples=0, s_chunk=0,
-                 sampled_regions=False, profile=False, output=OUT_PAIRS):
+                 sampled_regions=False, profile=False, output=OUT_PAIRS, bulk_build=0):
     o = Options()
     lib().tbj_default_options(ctypes.byref(o))
     o.partitioner = PARTITIONERS[partitioner] if isinstance(partitioner, str) else int(partitioner)
@@ -126,6 +127,8 @@ def make_options(partitioner="auto", radix_bits=0, passes=0, materialize=False,
     o.sampled_regions = int(sampled_regions)
     o.profile = 1 if profile else 0
     o.output = int(output)
+    # PHT / NPO_st table build: 0 = auto, 1 = always bulk, -1 = atomic inserts
+    o.bulk_build = int(bulk_build)
     return o
 
 

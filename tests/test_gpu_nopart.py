@@ -68,3 +68,46 @@ def test_cht_slot_past_last_bit(tbj, oracle):
     ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
     assert res.matches == 4 and np.array_equal(ga, ra) and np.array_equal(gb, sa)
     assert res.checksum == gu.checksum(ra, sa)
+
+
+def join_all_pairs(tbj, oracle, algo, R, S, **opts):
+    """Every pair written (output sized to the oracle's count) and compared."""
+    n = oracle.count(R[:, 0], S[:, 0])
+    res, got = gu.run(tbj, algo, R, S, cap=max(n, 1), **opts)
+    _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert res.matches == n and res.pairs_written == n
+    assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+    assert res.checksum == gu.checksum(ra, sa)
+    return res
+
+
+def pht_layout_bits(nR):
+    """Partition bits of PHT's bulk build: the table has nextpow2(|R| / 2)
+    buckets, blocks of at most 2^10 of them."""
+    lg = max(0, int(max(1, nR // 2) - 1).bit_length())
+    return max(0, lg - 10)
+
+
+@pytest.mark.parametrize("bulk", [1, -1])
+@pytest.mark.parametrize("nR", [1, 3, 1000, 5000, 70001, 1 << 20])
+def test_pht_bulk_and_atomic_builds(tbj, oracle, bulk, nR):
+    # blocks of buckets filled from R partitioned on the buckets' low bits (no
+    # partitioning, one pass and two passes) against per-tuple inserts; keys
+    # with a narrow range repeat, so buckets overflow into chains
+    rng = np.random.default_rng(nR)
+    rk = rng.integers(0, max(2, nR // 2), nR)
+    rk[: min(nR, 3)] = [0xFFFFFFFF, 0, 7][: min(nR, 3)]
+    sk = np.concatenate([rng.integers(0, max(2, nR // 2) + 50, 3 * nR + 5), [0xFFFFFFFF, 0]])
+    res = join_all_pairs(tbj, oracle, "PHT", rel(rk), rel(sk), bulk_build=bulk)
+    assert res.radix_bits == (pht_layout_bits(nR) if bulk == 1 else 0)
+
+
+@pytest.mark.parametrize("algo", ["PHT", "NPO_st"])
+def test_pht_bulk_build_long_chains(tbj, oracle, algo):
+    # one key 10000 times (a bucket whose chain spans many pool nodes, placed
+    # by one workgroup) next to distinct keys, through two partition passes
+    rk = np.concatenate([np.full(10000, 42), np.arange(1, 1 << 20)])
+    sk = np.concatenate([np.full(7, 42), np.arange(0, 1 << 20, 3)])
+    res = join_all_pairs(tbj, oracle, algo, rel(rk), rel(sk), bulk_build=1)
+    assert res.radix_bits == pht_layout_bits(len(rk)) and res.passes == 2

@@ -1,8 +1,8 @@
 // Batches of joins: partition bounds over many joins at once, a persistent
 // count-only LDS hash join whose tasks are decoded from a per-partition
-// prefix (no task list, no host round trip), the PHT joins of a wave built
-// and probed in one launch each, and the device generator of the batch
-// workload.
+// prefix (no task list, no host round trip), the bulk build of the PHT
+// joins' tables (a block of buckets per workgroup) and their probe in one
+// launch, and the device generator of the batch workload.
 //
 // The join semantics are RHO's (radix partition, then a per-partition
 // bucket-chained hash join: Joins/radix_join.c:373-440, 1557-1561) and PHT's
@@ -205,42 +205,42 @@ __global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(4)))
 // next job, so the search never stops on one).
 
 __device__ __forceinline__ uint32_t pht_job_of(const PhtJob* __restrict__ jobs, uint32_t njobs,
-                                               uint32_t c, bool probe) {
+                                               uint32_t c) {
     uint32_t lo = 0, hi = njobs;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        if ((probe ? jobs[mid].sc0 : jobs[mid].rc0) <= c) lo = mid;
+        if (jobs[mid].sc0 <= c) lo = mid;
         else hi = mid;
     }
     return lo;
 }
 
 constexpr int kPhtBlock = 256;
-constexpr int kPhtBuildItems = kPhtBuildChunk / kPhtBlock;  // inserts in flight per thread
-constexpr int kPhtProbeItems = 8;                           // bucket loads in flight per thread
+constexpr int kPhtProbeItems = 8;  // bucket loads in flight per thread
 
-__global__ __launch_bounds__(kPhtBlock) void k_bpht_build(
-    const uint2* __restrict__ arena, const PhtJob* __restrict__ jobs, uint32_t njobs,
-    uint32_t nchunks, PhtBucket* __restrict__ tab, OvNode* __restrict__ pool,
-    uint32_t* __restrict__ pool_ctr) {
-    for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const PhtJob jb = jobs[pht_job_of(jobs, njobs, c, false)];
-        const uint32_t base = (c - jb.rc0) * kPhtBuildChunk + threadIdx.x;
-        PhtBucket* t0 = tab + jb.tab;
-        uint2 t[kPhtBuildItems];
-        uint32_t pos[kPhtBuildItems];
-#pragma unroll
-        for (int k = 0; k < kPhtBuildItems; k++) {
-            const uint32_t i = base + k * kPhtBlock;
-            t[k] = i < jb.nR ? arena[jb.r_off + i] : make_uint2(0, 0);
-        }
-#pragma unroll
-        for (int k = 0; k < kPhtBuildItems; k++)
-            pos[k] = base + k * kPhtBlock < jb.nR ? atomicAdd(&t0[t[k].x & jb.mask].count, 1u)
-                                                  : ~0u;
-#pragma unroll
-        for (int k = 0; k < kPhtBuildItems; k++)
-            if (pos[k] != ~0u) pht_place(t0 + (t[k].x & jb.mask), t[k], pos[k], pool, pool_ctr);
+// Bulk build of the tables of a wave's PHT joins, one block of 2^hbits
+// buckets per workgroup (pht_place_block).  Partitioned: a part of joins with
+// the same pbits, whose R ranges were partitioned in one pass on the low pbits
+// bits into src; block k is partition k & (2^pbits - 1) of the part's segment
+// (join) k >> pbits, its bounds taken from the scanned group sums.  DIRECT
+// (pbits = 0, tables of at most 2^kPhtPlaceHBits buckets): block k is all of
+// join k's R, read from the arena.  jobs: the part's joins.
+template <bool DIRECT>
+__global__ __launch_bounds__(kPhtPlaceBlock) void k_bpht_place(
+    const uint2* __restrict__ src, const PhtJob* __restrict__ jobs,
+    const SegDesc* __restrict__ segs, const uint32_t* __restrict__ gsum, uint32_t pbits,
+    PhtBucket* __restrict__ tab, OvNode* __restrict__ pool, uint32_t* __restrict__ pool_ctr) {
+    const uint32_t k = blockIdx.x;
+    const uint32_t s = k >> pbits, b = k & ((1u << pbits) - 1);
+    const PhtJob jb = jobs[s];
+    if (DIRECT) {
+        pht_place_block(src + jb.r_off, 0, jb.nR, 0, jb.hbits, tab + jb.tab, pool, pool_ctr);
+    } else {
+        const SegDesc sd = segs[s];
+        const uint32_t lo = gsum[sd.hist_base + (uint64_t)b * sd.ngroups];
+        const uint32_t hi = gsum[sd.hist_base + (uint64_t)(b + 1) * sd.ngroups];
+        pht_place_block(src, lo, hi, pbits, jb.hbits, tab + jb.tab + ((uint64_t)b << jb.hbits),
+                        pool, pool_ctr);
     }
 }
 
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kPhtBlock) void k_bpht_probe(
     uint32_t nchunks, const PhtBucket* __restrict__ tab, const OvNode* __restrict__ pool,
     unsigned long long* __restrict__ counts) {
     for (uint32_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
-        const PhtJob jb = jobs[pht_job_of(jobs, njobs, c, true)];
+        const PhtJob jb = jobs[pht_job_of(jobs, njobs, c)];
         const PhtBucket* t0 = tab + jb.tab;
         const uint32_t base = (c - jb.sc0) * kPhtProbeChunk + threadIdx.x;
         uint32_t m = 0;
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kPhtBlock) void k_bpht_probe(
 #pragma unroll
             for (int k = 0; k < kPhtProbeItems; k++) {
                 if (base + r + k * kPhtBlock < jb.nS) {
-                    bk[k] = t0[sv[k].x & jb.mask];
+                    bk[k] = t0[pht_slot(sv[k].x & jb.mask, jb.pbits, jb.hbits)];
                 } else {
                     bk[k].count = 0;
                     bk[k].ovf = 0;
@@ -418,19 +418,25 @@ void batch_join(hipStream_t st, uint32_t grid, const uint2* Rb, const uint2* Sb,
                        task_off, ntasks_dev, task_ctr, s_chunk, hshift, join_ids, counts);
 }
 
-void batch_pht(hipStream_t st, uint32_t grid, const uint2* arena, const PhtJob* jobs,
-               uint32_t njobs, uint32_t nrc, uint32_t nsc, void* tab, uint64_t nbuckets,
-               void* pool, uint32_t* pool_ctr, unsigned long long* counts) {
+void batch_pht_place(hipStream_t st, const uint2* src, const PhtJob* jobs, uint32_t njobs,
+                     const SegDesc* segs, const uint32_t* gsum, uint32_t pbits, void* tab,
+                     void* pool, uint32_t* pool_ctr) {
     if (!njobs) return;
-    // a bucket is four 8-byte words; kernels, not memsets, so that the
-    // launches can be captured into a graph
-    batch_zero_u64(st, reinterpret_cast<unsigned long long*>(tab), 4 * nbuckets);
-    launch_fill_u32(st, pool_ctr, 1, 0);
-    if (nrc)
-        hipLaunchKernelGGL(k_bpht_build, dim3(min(grid, nrc)), dim3(kPhtBlock), 0, st, arena, jobs,
-                           njobs, nrc, reinterpret_cast<PhtBucket*>(tab),
+    const uint32_t nblk = njobs << pbits;
+    if (pbits == 0)
+        hipLaunchKernelGGL(k_bpht_place<true>, dim3(nblk), dim3(kPhtPlaceBlock), 0, st, src, jobs,
+                           segs, gsum, 0u, reinterpret_cast<PhtBucket*>(tab),
                            reinterpret_cast<OvNode*>(pool), pool_ctr);
-    if (nsc)
+    else
+        hipLaunchKernelGGL(k_bpht_place<false>, dim3(nblk), dim3(kPhtPlaceBlock), 0, st, src,
+                           jobs, segs, gsum, pbits, reinterpret_cast<PhtBucket*>(tab),
+                           reinterpret_cast<OvNode*>(pool), pool_ctr);
+}
+
+void batch_pht_probe(hipStream_t st, uint32_t grid, const uint2* arena, const PhtJob* jobs,
+                     uint32_t njobs, uint32_t nsc, const void* tab, const void* pool,
+                     unsigned long long* counts) {
+    if (njobs && nsc)
         hipLaunchKernelGGL(k_bpht_probe, dim3(min(grid, nsc)), dim3(kPhtBlock), 0, st, arena, jobs,
                            njobs, nsc, reinterpret_cast<const PhtBucket*>(tab),
                            reinterpret_cast<const OvNode*>(pool), counts);

@@ -9,7 +9,10 @@
 // the second one segment per (relation, first-pass bin), exactly as one
 // join's second pass does.  Per-join results come from the join kernel,
 // which maps a partition back to its join.  A PHT group builds the tables
-// of all of its joins in one launch and probes them in another.
+// of all of its joins in bulk -- per part (its joins with the same table
+// layout) the joins' R ranges partitioned in one pass on the buckets' low
+// bits, then one workgroup per block of buckets -- and probes them all in one
+// launch.
 #include <math.h>
 #include <string.h>
 
@@ -21,6 +24,7 @@
 #include "../../include/tbj_gen.h"
 #include "tbj_batch.h"
 #include "tbj_kernels.h"
+#include "tbj_pht.h"
 #include "tbj_runtime.h"
 
 namespace {
@@ -44,8 +48,23 @@ struct Group {
                            // (PHT groups: word 0 = the overflow pool counter)
     // PHT groups
     size_t jobs_off;
-    uint32_t njobs, nrc, nsc;
+    uint32_t njobs, nsc;
     uint64_t nbuckets, npool;
+    uint32_t part0, part1;  // its parts in tbj_batch_plan::parts
+};
+
+// The joins of a PHT group whose tables share pbits (PhtJob): jobs [j0, j1)
+// of the group.  pbits > 0: their R ranges (segments, relative to the wave
+// window at base_off) are partitioned in one pass on the low pbits bits
+// before the blocks are placed.
+struct PhtPart {
+    uint32_t pbits;
+    uint32_t j0, j1;
+    uint32_t T1;
+    uint64_t L1;
+    uint64_t base_off;
+    size_t segs_off;
+    uint32_t small_idx;  // word 0: its tile count (set at planning)
 };
 
 struct DevBuf {
@@ -95,6 +114,7 @@ struct tbj_batch_plan {
     unsigned long long* counts = nullptr;
     tbj_batch_tuning tune{};
     std::vector<Group> groups;
+    std::vector<PhtPart> parts;
     uint32_t waves = 0;
     uint64_t tuples = 0;
     DevBuf c1, c2, tiles1, hist1, offs1, gs1, tiles2, hist2, offs2, gs2, segs2, bounds, task_off,
@@ -118,9 +138,38 @@ struct tbj_batch_plan {
         for (const Group& g : groups) {
             if (g.kind == kGroupPht) {
                 const PhtJob* jobs = reinterpret_cast<const PhtJob*>((char*)konst.p + g.jobs_off);
-                batch_pht(st, pht_grid, reinterpret_cast<const uint2*>(arena), jobs, g.njobs, g.nrc,
-                          g.nsc, ptab.p, g.nbuckets, ppool.p, (uint32_t*)small.p + 8 * g.small_idx,
-                          counts);
+                uint32_t* pool_ctr = (uint32_t*)small.p + 8 * g.small_idx;
+                const uint2* arena2 = reinterpret_cast<const uint2*>(arena);
+                launch_fill_u32(st, pool_ctr, 1, 0);
+                for (uint32_t k = g.part0; k < g.part1; k++) {
+                    const PhtPart& pp = parts[k];
+                    if (pp.pbits == 0) {
+                        batch_pht_place(st, arena2, jobs + pp.j0, pp.j1 - pp.j0, nullptr, nullptr,
+                                        0, ptab.p, ppool.p, pool_ctr);
+                        continue;
+                    }
+                    const uint2* base = arena2 + pp.base_off;
+                    const SegDesc* segs = reinterpret_cast<const SegDesc*>((char*)konst.p + pp.segs_off);
+                    uint32_t* sm = (uint32_t*)small.p + 8 * pp.small_idx;
+                    TileDesc* t1 = (TileDesc*)tiles1.p;
+                    uint16_t* h1 = (uint16_t*)hist1.p;
+                    uint32_t* o1 = (uint32_t*)offs1.p;
+                    uint32_t* q1 = (uint32_t*)gs1.p;
+                    const uint32_t F = 1u << pp.pbits;
+                    const BinSpec bs = radix_pass(pp.pbits, 0, F);
+                    const uint32_t nseg = pp.j1 - pp.j0;
+                    launch_tiles_from_segs(st, segs, nseg, sm, t1, pp.T1, tune.tile);
+                    launch_hist(st, pp.T1, base, t1, sm, 0, h1, bs, F, tune.tile);
+                    launch_fill_u32(st, q1 + (pp.L1 - 1), 1, 0);
+                    launch_tile_offsets(st, pp.T1, h1, o1, t1, segs, sm, F, q1, pp.L1,
+                                        (uint32_t*)partial.p, sm + 6);
+                    launch_scatter(st, pp.T1, base, (uint2*)c1.p, t1, sm, 0, o1, bs, F, true,
+                                   nullptr, tune.tile);
+                    batch_pht_place(st, (const uint2*)c1.p, jobs + pp.j0, nseg, segs, q1,
+                                    pp.pbits, ptab.p, ppool.p, pool_ctr);
+                }
+                batch_pht_probe(st, pht_grid, arena2, jobs, g.njobs, g.nsc, ptab.p, ppool.p,
+                                counts);
                 continue;
             }
             const uint2* base = reinterpret_cast<const uint2*>(arena) + g.base_off;
@@ -217,6 +266,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     std::vector<char> konst;
     size_t max_tup = 0, max_T1 = 0, max_T2 = 0, max_L1 = 0, max_L2 = 0, max_s2 = 0, max_NP = 0;
     size_t max_H1 = 0, max_H2 = 0, max_bk = 0, max_pool = 0;
+    uint32_t nsmall = 0;  // 8-word device slots of the groups and PHT parts
     auto align = [&](size_t a) { konst.resize((konst.size() + a - 1) / a * a); };
     size_t w0 = 0;
     while (w0 < n) {
@@ -297,7 +347,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             g.ids_off = konst.size();
             konst.resize(konst.size() + js.size() * sizeof(uint32_t));
             memcpy(konst.data() + g.ids_off, js.data(), js.size() * sizeof(uint32_t));
-            g.small_idx = (uint32_t)P->groups.size();
+            g.small_idx = nsmall++;
             if (g.passes) max_tup = std::max<size_t>(max_tup, g.tuples);
             max_T1 = std::max<size_t>(max_T1, g.T1);
             max_T2 = std::max<size_t>(max_T2, g.T2max);
@@ -315,6 +365,17 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
             Group g{};
             g.kind = kGroupPht;
             g.nj = g.njobs = (uint32_t)pht.size();
+            // table layout of each join: blocks of at most 2^kPhtPlaceHBits
+            // buckets, one per value of the buckets' low pbits bits
+            auto lg_of = [&](uint32_t i) {
+                return ceil_log2(std::max<uint64_t>(1, items[i].nR / 2));
+            };
+            auto pbits_of = [&](uint32_t i) {
+                const uint32_t lg = lg_of(i);
+                return lg > kPhtPlaceHBits ? lg - kPhtPlaceHBits : 0u;
+            };
+            std::stable_sort(pht.begin(), pht.end(),
+                             [&](uint32_t a, uint32_t b) { return pbits_of(a) < pbits_of(b); });
             std::vector<PhtJob> jobs(pht.size());
             for (uint32_t j = 0; j < g.njobs; j++) {
                 const tbj_batch_item& it = items[pht[j]];
@@ -323,24 +384,69 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
                 jb.s_off = it.s_off;
                 jb.nR = (uint32_t)it.nR;
                 jb.nS = (uint32_t)it.nS;
-                const uint64_t nb = 1ull << ceil_log2(std::max<uint64_t>(1, it.nR / 2));
-                jb.mask = (uint32_t)(nb - 1);
+                const uint32_t lg = lg_of(pht[j]);
+                jb.mask = (uint32_t)((1ull << lg) - 1);
+                jb.pbits = pbits_of(pht[j]);
+                jb.hbits = lg - jb.pbits;
                 jb.tab = g.nbuckets;
                 jb.id = pht[j];
-                jb.rc0 = g.nrc;
                 jb.sc0 = g.nsc;
-                g.nbuckets += nb;
-                g.nrc += (uint32_t)((it.nR + kPhtBuildChunk - 1) / kPhtBuildChunk);
+                g.nbuckets += 1ull << lg;
                 // no R tuple, no match: S is not probed
                 if (it.nR) g.nsc += (uint32_t)((it.nS + kPhtProbeChunk - 1) / kPhtProbeChunk);
                 g.npool += it.nR;
                 g.tuples += it.nR + it.nS;
             }
+            // parts: runs of joins with the same pbits; those with pbits > 0
+            // get the R segments of their partition pass
+            g.part0 = (uint32_t)P->parts.size();
+            for (uint32_t j0 = 0; j0 < g.njobs;) {
+                uint32_t j1 = j0 + 1;
+                while (j1 < g.njobs && jobs[j1].pbits == jobs[j0].pbits) j1++;
+                PhtPart pp{};
+                pp.pbits = jobs[j0].pbits;
+                pp.j0 = j0;
+                pp.j1 = j1;
+                pp.base_off = wlo;
+                if (pp.pbits) {
+                    const uint32_t F = 1u << pp.pbits;
+                    std::vector<SegDesc> segs(j1 - j0);
+                    uint32_t tiles = 0;
+                    uint64_t groups = 0, tup = 0;
+                    for (uint32_t j = j0; j < j1; j++) {
+                        SegDesc& d = segs[j - j0];
+                        d.in_off = (uint32_t)(jobs[j].r_off - wlo);
+                        d.n = jobs[j].nR;
+                        d.ntiles = (d.n + tile - 1) / tile;
+                        d.first_tile = tiles;
+                        d.ngroups = seg_groups(tiles, d.ntiles);
+                        d.aux = 0;
+                        d.hist_base = groups * F;
+                        tiles += d.ntiles;
+                        groups += d.ngroups;
+                        tup += d.n;
+                    }
+                    pp.T1 = tiles;
+                    pp.L1 = (uint64_t)F * groups + 1;
+                    align(64);
+                    pp.segs_off = konst.size();
+                    konst.resize(konst.size() + segs.size() * sizeof(SegDesc));
+                    memcpy(konst.data() + pp.segs_off, segs.data(), segs.size() * sizeof(SegDesc));
+                    pp.small_idx = nsmall++;
+                    max_tup = std::max<size_t>(max_tup, tup);
+                    max_T1 = std::max<size_t>(max_T1, pp.T1);
+                    max_H1 = std::max<size_t>(max_H1, (size_t)F * pp.T1);
+                    max_L1 = std::max<size_t>(max_L1, pp.L1);
+                }
+                P->parts.push_back(pp);
+                j0 = j1;
+            }
+            g.part1 = (uint32_t)P->parts.size();
             align(64);
             g.jobs_off = konst.size();
             konst.resize(konst.size() + jobs.size() * sizeof(PhtJob));
             memcpy(konst.data() + g.jobs_off, jobs.data(), jobs.size() * sizeof(PhtJob));
-            g.small_idx = (uint32_t)P->groups.size();
+            g.small_idx = nsmall++;
             max_bk = std::max<size_t>(max_bk, g.nbuckets);
             max_pool = std::max<size_t>(max_pool, g.npool);
             P->groups.push_back(g);
@@ -364,15 +470,17 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
     TBJ_TRY(P->partial.get(sizeof(uint32_t) *
                            scan_partials(std::max({max_L1, max_L2, max_NP + 1}))));
     TBJ_TRY(P->konst.get(konst.size()));
-    TBJ_TRY(P->small.get(sizeof(uint32_t) * 8 * std::max<size_t>(1, P->groups.size())));
+    TBJ_TRY(P->small.get(sizeof(uint32_t) * 8 * std::max<size_t>(1, nsmall)));
     TBJ_TRY(P->total.get(sizeof(unsigned long long)));
     TBJ_TRY(P->ptab.get(32 * max_bk));
     TBJ_TRY(P->ppool.get(16 * max_pool));
     if (!konst.empty())
         TBJ_HIP(hipMemcpy(P->konst.p, konst.data(), konst.size(), hipMemcpyHostToDevice));
-    std::vector<uint32_t> sm(8 * std::max<size_t>(1, P->groups.size()), 0);
+    std::vector<uint32_t> sm(8 * std::max<size_t>(1, nsmall), 0);
     for (const Group& g : P->groups)
         if (g.kind == kGroupRadix) sm[8 * g.small_idx] = g.T1;
+    for (const PhtPart& pp : P->parts)
+        if (pp.pbits) sm[8 * pp.small_idx] = pp.T1;
     TBJ_HIP(hipMemcpy(P->small.p, sm.data(), sm.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
     TBJ_HIP(hipEventCreate(&P->e0));
     TBJ_HIP(hipEventCreate(&P->e1));

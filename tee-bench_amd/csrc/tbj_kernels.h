@@ -194,11 +194,11 @@ size_t pht_bucket_bytes();
 size_t pht_node_bytes();
 void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, uint32_t mask,
                       void* pool, uint32_t* pool_ctr);
-// Bulk build: one block of 2^hbits buckets (hbits <= kPhtPlaceHBits) per
-// partition of Rb (partition_relation on the low pbits bits, first pass b1
-// bits; bounds: its partition bounds), overflow nodes claimed from *pool_ctr
-// (zeroed).  The table (2^(pbits + hbits) buckets) needs no clearing.
-constexpr uint32_t kPhtPlaceHBits = 10;
+// Bulk build: one block of 2^hbits buckets (hbits <= kPhtPlaceHBits of
+// tbj_pht.h) per partition of Rb (partition_relation on the low pbits bits,
+// first pass b1 bits; bounds: its partition bounds), overflow nodes claimed
+// from *pool_ctr (zeroed).  The table (2^(pbits + hbits) buckets) needs no
+// clearing.
 void launch_pht_place(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
                       uint32_t pbits, uint32_t hbits, void* tab, void* pool, uint32_t* pool_ctr);
 // pbits, hbits: the table's layout (pht_slot); pbits = 0 for a table built by

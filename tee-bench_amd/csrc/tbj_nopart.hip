@@ -55,82 +55,16 @@ __global__ __launch_bounds__(256) void k_pht_build(const uint2* __restrict__ R, 
 // (partition p holds the keys whose bits [0, b1) are p / F2 and bits [b1,
 // pbits) are p % F2), and the table stores bucket b at pht_slot(b, pbits,
 // hbits), so partition p's 2^hbits buckets form the block starting at its
-// low bits << hbits.  One workgroup per partition: counts its tuples per
-// bucket (LDS), claims pool nodes for the tuples past each bucket's two slots
-// with one atomic (a bucket's overflow tuples take consecutive nodes, chained
-// in order), places the tuples in an LDS copy of the block and writes the
-// block out whole.  Every bucket is written, so the table needs no clearing.
-// The buckets hold what the per-tuple inserts of k_pht_build would (counts,
-// the same tuples), in another order inside each bucket.
-constexpr int kPlaceBlock = 256;
-
-__global__ __launch_bounds__(kPlaceBlock) void k_pht_place(
+// low bits << hbits; one workgroup per partition (pht_place_block).  Every
+// bucket is written, so the table needs no clearing.
+__global__ __launch_bounds__(kPhtPlaceBlock) void k_pht_place(
     const uint2* __restrict__ Rb, const uint2* __restrict__ bounds, uint32_t b1, uint32_t F2,
     uint32_t pbits, uint32_t hbits, PhtBucket* __restrict__ tab, OvNode* __restrict__ pool,
     uint32_t* __restrict__ pool_ctr) {
-    __shared__ PhtBucket blk[1u << kPhtPlaceHBits];
-    __shared__ uint32_t cnt[1u << kPhtPlaceHBits];
-    __shared__ uint32_t ovp[1u << kPhtPlaceHBits];
-    __shared__ uint32_t pos[1u << kPhtPlaceHBits];
-    __shared__ uint32_t scratch[kPlaceBlock / TBJ_WAVE + 1];
-    __shared__ uint32_t base_s;
     const uint32_t p = blockIdx.x;
-    const uint32_t nb = 1u << hbits, hmask = nb - 1;
     const uint32_t low = (p / F2) | ((p % F2) << b1);
     const uint2 bd = bounds[p];
-    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) {
-        cnt[i] = 0;
-        pos[i] = 0;
-    }
-    __syncthreads();
-    // counts (a block-uniform loop: every lane of a wave calls agg_add)
-    for (uint32_t j0 = bd.x; j0 < bd.y; j0 += kPlaceBlock) {
-        const uint32_t j = j0 + threadIdx.x;
-        agg_add(cnt, j < bd.y ? (Rb[j].x >> pbits) & hmask : TBJ_DROP);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) ovp[i] = cnt[i] > 2 ? cnt[i] - 2 : 0;
-    __syncthreads();
-    const uint32_t nov = block_scan_lds<kPlaceBlock>(ovp, nb, scratch);
-    if (threadIdx.x == 0) base_s = nov ? atomicAdd(pool_ctr, nov) : 0;
-    __syncthreads();
-    const uint32_t base = base_s;
-    for (uint32_t i = threadIdx.x; i < nb; i += kPlaceBlock) {
-        PhtBucket e;
-        e.count = cnt[i];
-        e.ovf = cnt[i] > 2 ? base + ovp[i] + 1 : 0;
-        e.slot[0] = e.slot[1] = make_uint2(0, 0);
-        e.pad[0] = e.pad[1] = 0;
-        blk[i] = e;
-    }
-    __syncthreads();
-    for (uint32_t j0 = bd.x; j0 < bd.y; j0 += kPlaceBlock) {
-        const uint32_t j = j0 + threadIdx.x;
-        uint2 t = make_uint2(0, 0);
-        uint32_t b = TBJ_DROP;
-        if (j < bd.y) {
-            t = Rb[j];
-            b = (t.x >> pbits) & hmask;
-        }
-        const uint32_t r = agg_inc(pos, b);
-        if (b == TBJ_DROP) continue;
-        if (r < 2) {
-            blk[b].slot[r] = t;
-        } else {
-            // node n; the next one (1-based n + 2) unless this is the
-            // bucket's last overflow tuple (r - 2 == cnt - 3)
-            const uint32_t n = base + ovp[b] + (r - 2);
-            OvNode nd;
-            nd.t = t;
-            nd.next = r + 1 < cnt[b] ? n + 2 : 0;
-            nd.pad = 0;
-            pool[n] = nd;
-        }
-    }
-    __syncthreads();
-    uint4* dst = reinterpret_cast<uint4*>(tab + ((uint64_t)low << hbits));
-    const uint4* src = reinterpret_cast<const uint4*>(blk);
-    for (uint32_t i = threadIdx.x; i < 2 * nb; i += kPlaceBlock) dst[i] = src[i];
+    pht_place_block(Rb, bd.x, bd.y, pbits, hbits, tab + ((uint64_t)low << hbits), pool, pool_ctr);
 }
 
 // Block-aggregated output: every round each thread handles kProbeItems S
@@ -504,7 +438,7 @@ void launch_pht_build(hipStream_t st, const uint2* R, uint64_t nR, void* tab, ui
 void launch_pht_place(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
                       uint32_t pbits, uint32_t hbits, void* tab, void* pool, uint32_t* pool_ctr) {
     const uint32_t F2 = 1u << (pbits - b1);
-    hipLaunchKernelGGL(k_pht_place, dim3(1u << pbits), dim3(kPlaceBlock), 0, st, Rb, bounds, b1,
+    hipLaunchKernelGGL(k_pht_place, dim3(1u << pbits), dim3(kPhtPlaceBlock), 0, st, Rb, bounds, b1,
                        F2, pbits, hbits, (PhtBucket*)tab, (OvNode*)pool, pool_ctr);
 }
 

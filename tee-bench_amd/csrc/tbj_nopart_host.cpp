@@ -4,6 +4,7 @@
 #include <algorithm>
 
 #include "tbj_kernels.h"
+#include "tbj_pht.h"
 #include "tbj_runtime.h"
 
 namespace tbj {

@@ -833,9 +833,10 @@ __global__ void k_sample_keys(const uint2* __restrict__ in, uint64_t n, uint32_t
                               uint32_t* __restrict__ sample) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    // evenly spaced positions with a per-slot offset inside the stride
-    uint64_t stride = n / m;
-    uint64_t pos = (uint64_t)i * stride + ((uint64_t)i * 2654435761u) % (stride ? stride : 1);
+    // slot i covers positions [i n / m, (i + 1) n / m) -- the slots tile all
+    // of the input, its tail included -- and takes one at a hashed offset
+    const uint64_t a = (uint64_t)i * n / m, b = (uint64_t)(i + 1) * n / m;
+    uint64_t pos = a + ((uint64_t)i * 2654435761u) % (b > a ? b - a : 1);
     if (pos >= n) pos = n - 1;
     sample[i] = in[pos].x;
 }

@@ -75,8 +75,11 @@ struct Bin64 {
     uint32_t pass_shift;
     uint32_t pass_mask;
     const Model64* model;
-    // non-null: keys outside [range[0], range[1]] are dropped (S against R's range)
+    // non-null: keys outside [range[0], range[1]] are dropped (S against R's
+    // range), unless gate is set and *gate <= gate_min
     const unsigned long long* range;
+    const uint32_t* gate;
+    uint32_t gate_min;
 };
 
 struct SmTask {
@@ -102,22 +105,27 @@ void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR
                      uint32_t nbins, uint32_t* ntiles_dev, bool rows = false);
 // Sample of a sample-sized first pass: cnt1 (zeroed, F1 words per row of R,
 // then per row of S) += the sampled tuples of each (row, bin).  bss must not
-// drop (R's range is not known yet).
+// drop (R's range is not known yet).  s_out (optional, zeroed; bss learned):
+// += the sampled S keys outside the model's key range [kmin, kmax] -- the
+// keys that R's exact range may let S drop.
 void sm64_sample_rows(hipStream_t st, const ulonglong2* R, uint32_t nR, const ulonglong2* S,
                       uint32_t nS, uint32_t nrowR, uint32_t nrowS, const Bin64& bsr,
-                      const Bin64& bss, uint32_t F1, uint32_t* cnt1);
+                      const Bin64& bss, uint32_t F1, uint32_t* cnt1, uint32_t* s_out = nullptr);
 // Second-pass segments (one per first-pass partition, 2 * F1) over the tiles
 // of sample-sized regions (first: their scanned tile counts in partition order).
 void sm64_part_segments(hipStream_t st, const uint32_t* first, uint32_t F1, uint32_t F2,
                         uint32_t nrowR, uint32_t nrowS, SegDesc* segs2);
 // Claims of a sample-sized first pass (see k_scatter64): regions cb0 +
-// tile row * cmul + bin; mm (optional) receives the launch's key range.
+// tile row * cmul + bin; mm (optional) receives the launch's key range,
+// unless gate is set and *gate <= gate_min.
 struct Sm64Claims {
     const uint32_t* starts;
     uint32_t* cursors;
     uint32_t* overflow;
     uint32_t cb0, cmul;
     unsigned long long* mm;
+    const uint32_t* gate = nullptr;
+    uint32_t gate_min = 0;
 };
 // Tile t's bin counts go to row t of hist (nbins u16: a tile holds at most
 // kSm64Span tuples); mm (optional):

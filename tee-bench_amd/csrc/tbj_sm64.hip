@@ -158,7 +158,7 @@ __device__ __forceinline__ Bin64Fn make_bin64(const Bin64& b, LdsModel64* lm) {
     f.hshift = 64 - b.bits;
     f.last = b.bits >= 32 ? 0xFFFFFFFFu : (1u << b.bits) - 1;
     f.lmul = 0;
-    f.drop = b.range != nullptr;
+    f.drop = b.range != nullptr && (b.gate == nullptr || *b.gate > b.gate_min);
     f.lo = f.drop ? b.range[0] : 0;
     f.hi = f.drop ? b.range[1] : ~0ull;
     if (b.kind == kBinLearned && b.model->linear) {
@@ -195,8 +195,10 @@ __global__ void k_sample64(const ulonglong2* __restrict__ in, uint64_t n, uint32
                            uint64_t* __restrict__ sample) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    const uint64_t stride = n / m;  // m <= n
-    uint64_t pos = (uint64_t)i * stride + ((uint64_t)i * 2654435761u) % stride;
+    // slot i covers positions [i n / m, (i + 1) n / m) (m <= n: the slots
+    // tile all of R, its tail included) and takes one at a hashed offset
+    const uint64_t a = (uint64_t)i * n / m, b = (uint64_t)(i + 1) * n / m;
+    uint64_t pos = a + ((uint64_t)i * 2654435761u) % (b > a ? b - a : 1);
     if (pos >= n) pos = n - 1;
     sample[i] = in[pos].x;
 }
@@ -419,7 +421,9 @@ struct Claims64 {
     uint32_t* cursors;
     uint32_t* overflow;
     uint32_t cb0, cmul;
-    unsigned long long* mm;
+    unsigned long long* mm;  // the launch's key range, while !gate || *gate > gate_min
+    const uint32_t* gate;
+    uint32_t gate_min;
 };
 
 template <bool CLAIM>
@@ -444,6 +448,7 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     if (blockIdx.x >= n) return;
     const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
     const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
+    const bool measure = CLAIM && cl.mm && (!cl.gate || *cl.gate > cl.gate_min);
     const TileDesc td = tiles[t];
     ulonglong2 v[kSm64ScItems];
 #pragma unroll
@@ -452,7 +457,7 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
         if (j < td.len) v[k] = in[td.in_off + j];
     }
     for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) loff[i] = 0;
-    if (CLAIM && cl.mm && threadIdx.x == 0) {
+    if (measure && threadIdx.x == 0) {
         bmm[0] = ~0ull;
         bmm[1] = 0;
     }
@@ -466,7 +471,7 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     // the stored range, read now so that the filter below does not wait for
     // it (it only narrows: a stale value lets an unneeded atomic through)
     unsigned long long smin = 0, smax = ~0ull;
-    if (CLAIM && cl.mm && threadIdx.x == 0) {
+    if (measure && threadIdx.x == 0) {
         const volatile unsigned long long* vmm = cl.mm;
         smin = vmm[0];
         smax = vmm[1];
@@ -480,7 +485,7 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     }
 #pragma unroll
     for (uint32_t k = 0; k < kSm64ScItems; k++) rank[k] = agg_inc(loff, bb[k]);
-    if (CLAIM && cl.mm) {  // tile min / max: waves -> LDS -> one filtered atomic pair
+    if (measure) {  // tile min / max: waves -> LDS -> one filtered atomic pair
         uint64_t mn = ~0ull, mx = 0;
 #pragma unroll
         for (uint32_t k = 0; k < kSm64ScItems; k++)
@@ -495,7 +500,7 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
         }
     }
     __syncthreads();
-    if (CLAIM && cl.mm && threadIdx.x == 0) {
+    if (measure && threadIdx.x == 0) {
         if (bmm[0] < smin) atomicMin(&cl.mm[0], bmm[0]);
         if (bmm[1] > smax) atomicMax(&cl.mm[1], bmm[1]);
     }
@@ -545,7 +550,7 @@ constexpr uint32_t kS64ChunksPerRow = (kRowTiles + kS64SampleTiles - 1) / kS64Sa
 __global__ __launch_bounds__(256) void k_sample_rows64(
     const ulonglong2* __restrict__ R, uint32_t nR, const ulonglong2* __restrict__ S, uint32_t nS,
     uint32_t nrowR, uint32_t nrowS, Bin64 bsr, Bin64 bss, uint32_t F1,
-    uint32_t* __restrict__ cnt1) {
+    uint32_t* __restrict__ cnt1, uint32_t* __restrict__ s_out) {
     __shared__ uint32_t c1[kSm64MaxBins];
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     const uint32_t rowg = blockIdx.x / kS64ChunksPerRow, chunk = blockIdx.x % kS64ChunksPerRow;
@@ -564,6 +569,10 @@ __global__ __launch_bounds__(256) void k_sample_rows64(
     for (uint32_t i = threadIdx.x; i < F1; i += 256) c1[i] = 0;
     __syncthreads();
     const uint32_t lo = t0 * kSm64Span, hi = min(t1 * kSm64Span, n);
+    // S against the learned model's key range (the block-uniform test keeps
+    // the wave reduction below with every lane)
+    const bool count_out = rel && s_out && bss.kind == kBinLearned;
+    uint32_t nout = 0;
     for (uint32_t blk = lo / 256 + threadIdx.x; blk < (hi + 255) / 256; blk += 256) {
         const uint32_t first = blk * 256 + sampled_group(blk) * 16;
         uint64_t key[16];
@@ -578,8 +587,13 @@ __global__ __launch_bounds__(256) void k_sample_rows64(
             if (i >= lo && i < hi) {
                 const uint32_t bb = bin(key[k]);
                 if (bb != kNone) atomicAdd(&c1[bb], 1u);
+                nout += key[k] < bin.kmin || key[k] > bin.kmax;
             }
         }
+    }
+    if (count_out) {
+        const uint64_t w = wave_reduce_u64(nout);
+        if (lane_id() == 0 && w) atomicAdd(s_out, (uint32_t)w);
     }
     __syncthreads();
     const uint32_t pbase = rel ? F1 * nrowR : 0;
@@ -1566,11 +1580,12 @@ void sm64_make_tiles(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR
 
 void sm64_sample_rows(hipStream_t st, const ulonglong2* R, uint32_t nR, const ulonglong2* S,
                       uint32_t nS, uint32_t nrowR, uint32_t nrowS, const Bin64& bsr,
-                      const Bin64& bss, uint32_t F1, uint32_t* cnt1) {
+                      const Bin64& bss, uint32_t F1, uint32_t* cnt1, uint32_t* s_out) {
     const uint32_t rows = nrowR + nrowS;
     if (!rows) return;
     hipLaunchKernelGGL(k_sample_rows64, dim3(rows * kS64ChunksPerRow), dim3(256),
-                       sm64_model_lds(bsr), st, R, nR, S, nS, nrowR, nrowS, bsr, bss, F1, cnt1);
+                       sm64_model_lds(bsr), st, R, nR, S, nS, nrowR, nrowS, bsr, bss, F1, cnt1,
+                       s_out);
 }
 
 void sm64_part_segments(hipStream_t st, const uint32_t* first, uint32_t F1, uint32_t F2,
@@ -1601,6 +1616,8 @@ void sm64_scatter(hipStream_t st, uint32_t grid, const ulonglong2* in, ulonglong
         cl.cb0 = claims->cb0;
         cl.cmul = claims->cmul;
         cl.mm = claims->mm;
+        cl.gate = claims->gate;
+        cl.gate_min = claims->gate_min;
         hipLaunchKernelGGL(k_scatter64<true>, dim3(grid), dim3(kSm64ScBlock), lds, st, in, out,
                            tiles, ntiles_dev, tile_base, tile_base + grid, offs, b, nbins, cl);
     } else {

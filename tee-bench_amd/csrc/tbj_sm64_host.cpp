@@ -27,7 +27,8 @@ struct Small64 {  // device scalars, one allocation
     uint32_t lin;  // the learned fit gave the linear model
     uint32_t overflow;      // a sample-sized region was too small
     uint32_t region_total;  // sample-sized first pass: region capacity in all
-    uint32_t ntiles2_pad[2];
+    uint32_t s_out;  // sampled S keys outside the learned model's key range
+    uint32_t ntiles2_pad[1];
     unsigned long long range[2];     // exact R key range (learned partitioner)
     unsigned long long counters[2];  // matches, checksum
     unsigned long long volume;       // pairs over all chunks
@@ -193,14 +194,22 @@ static int run_sm64_once(Context& C, int algo, const tbj_tuple64* R, uint64_t nR
             sm64_make_tiles(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1, true);
             Bin64 s1s_sample = s1s;
             s1s_sample.range = nullptr;  // R's range comes with R's scatter
-            sm64_sample_rows(st, Rin, nR, Sin, nS, nrowR, nrowS, s1r, s1s_sample, F1, cnt1);
+            sm64_sample_rows(st, Rin, nR, Sin, nS, nrowR, nrowS, s1r, s1s_sample, F1, cnt1,
+                             learned ? &sm->s_out : nullptr);
             TBJ_TRY(pt.mark(kPhHist1));
             launch_row_caps(st, cnt1, np1, starts1);
             launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
             TBJ_TRY(pt.mark(kPhScan1));
-            // R's scatter measures R's exact range (learned: S drops keys outside it)
-            const Sm64Claims c1r{starts1, cursors1, &sm->overflow, 0, F1,
-                                 learned ? sm->range : nullptr};
+            // Learned: R's scatter measures R's exact range and S drops the keys
+            // outside it -- when the sample saw more than 1/64 of S outside the
+            // model's key range (otherwise hardly any S key could be dropped,
+            // and neither pays; the join's result is the same either way).
+            const uint32_t gate_min = nS / 1024;
+            Sm64Claims c1r{starts1, cursors1, &sm->overflow, 0, F1, learned ? sm->range : nullptr};
+            c1r.gate = &sm->s_out;
+            c1r.gate_min = gate_min;
+            s1s.gate = &sm->s_out;
+            s1s.gate_min = gate_min;
             const Sm64Claims c1s{starts1, cursors1, &sm->overflow, F1 * nrowR, F1, nullptr};
             sm64_scatter(st, tR, Rin, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, &c1r);
             sm64_scatter(st, tS, Sin, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, &c1s);

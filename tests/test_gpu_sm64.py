@@ -174,6 +174,21 @@ def test_sample_sized_learned_drops_outside_range(tbj, oracle):
     check_pairs(oracle, R, S, got, res, ordered=True)
 
 
+def test_sample_sized_learned_keeps_rare_outside_keys(tbj, oracle):
+    # 20 of 400020 S keys outside R's range: the sample sees too few of them
+    # for R's range to be measured, so nothing is dropped and those keys
+    # simply find no match
+    rng = np.random.default_rng(25)
+    rk = np.unique(rng.integers(1 << 40, 1 << 41, 200000, dtype=np.uint64))
+    sk = np.concatenate([rng.choice(rk, 400000), rng.integers(0, 1 << 40, 10, dtype=np.uint64),
+                         rng.integers(1 << 41, 1 << 63, 10, dtype=np.uint64)])
+    R, S = rel64(rk), rel64(sk[rng.permutation(len(sk))])
+    res, got = run64(tbj, "PSM", R, S, sampled_regions=True, radix_bits=14, passes=2)
+    assert res.passes == 2 and res.redone_exact == 0 and res.s_dropped == 0
+    assert res.matches == 400000
+    check_pairs(oracle, R, S, got, res, ordered=True)
+
+
 def test_learned_oversized_unique_partitions(tbj, oracle):
     # 6 bits for 2^18 unique keys: every range partition spans two chunks
     R = tbj.gen_unique64(9, 1 << 18)

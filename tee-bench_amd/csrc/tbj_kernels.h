@@ -218,6 +218,16 @@ void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmas
                       uint32_t pmask, uint2* words, uint32_t* cnt, uint32_t* partial,
                       uint32_t* taken, uint2* arr, uint32_t* spill, uint32_t* spilled,
                       ChtState* state);
+// CHT bulk build: Rb = R partitioned (partition_relation) on bitmap position
+// bits [sbits, sbits + pbits) (sbits <= 12, and no more than log2 of the CHT
+// partition size), bounds its partition bounds, b1 its first pass's bits;
+// words (zeroed or not), segcnt (2^pbits + 1), partial
+// (scan_partials(2^pbits + 1)), taken (zeroed), arr, spill (Rb positions),
+// spilled (zeroed), state (zeroed).
+void launch_cht_build_bulk(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
+                           uint32_t pbits, uint32_t sbits, uint32_t bmask, uint32_t pmask,
+                           uint2* words, uint32_t* segcnt, uint32_t* partial, uint32_t* taken,
+                           uint2* arr, uint32_t* spill, uint32_t* spilled, ChtState* state);
 void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,
                          const ChtState* state, uint32_t nspill, uint32_t ov_mask,
                          uint32_t* ov_used, uint2* ov);

@@ -256,10 +256,10 @@ __global__ __launch_bounds__(256) void k_cht_place(const uint2* __restrict__ R, 
 // phase C and none went past the last set bit's slot.  nset tuples then fill
 // slots [0, nset) and nothing lies beyond, so a probe can tell a taken slot
 // by its index alone.
-__global__ void k_cht_dense(const uint32_t* __restrict__ cnt, uint32_t nwords,
+__global__ void k_cht_dense(const uint32_t* __restrict__ nset_src,
                             const uint32_t* __restrict__ taken, ChtState* __restrict__ state) {
     if (threadIdx.x != 0) return;
-    const uint32_t nset = cnt[nwords];
+    const uint32_t nset = *nset_src;
     const bool past = (taken[nset >> 5] >> (nset & 31)) & 1u;
     state->nset = nset;
     state->dense = (state->spill_ctr == state->spill_a && !past) ? 1u : 0u;
@@ -272,6 +272,135 @@ __global__ void k_cht_mark_spilled(const uint32_t* __restrict__ spill,
     const uint32_t n = state->spill_ctr;
     if (i == 0) state->spill_a = n;
     if (i < n) atomicOr(&spilled[spill[i] >> 5], 1u << (spill[i] & 31));
+}
+
+// ---------------------------------------------------------------------------
+// CHT bulk build.  R is partitioned beforehand on the high pbits bits of its
+// bitmap position h = key & bmask, so that partition p holds the tuples whose
+// home lies in segment q (the p -> q map of seg_of) = positions [q << sbits,
+// (q + 1) << sbits); sbits <= kChtSegBits and no segment straddles a CHT
+// partition (sbits <= log2 of its size).  Phase A, one workgroup per segment,
+// in LDS: set the home bit, or the next one in the CHT partition when that
+// one lies in the segment too, else spill; write the segment's bitmap words
+// and its number of set bits.  A scan of those numbers gives each segment
+// the rank of its first set bit.  Phase C, again per segment: the words'
+// ranks, and each tuple at the dense slot of its home bit's rank or the next
+// one inside the segment's dense range (staged in LDS, written whole); the
+// rest spill.  The result is the structure the per-tuple kernels build -- the
+// same bitmap words, ranks and dense slots, read by the same probe -- except
+// that a second choice leaving the segment spills to the overflow table.
+constexpr uint32_t kChtSegBits = 12;
+constexpr int kChtSegBlock = 256;
+
+// Segment of partition p (p = (position bits [sbits, sbits + b1)) * F2 +
+// (the bits above)).
+__device__ __forceinline__ uint32_t cht_seg_of(uint32_t p, uint32_t b1, uint32_t F2) {
+    return (p / F2) | ((p % F2) << b1);
+}
+
+__global__ __launch_bounds__(kChtSegBlock) void k_cht_seg_bits(
+    const uint2* __restrict__ Rb, const uint2* __restrict__ bounds, uint32_t b1, uint32_t F2,
+    uint32_t sbits, ChtParams cp, uint2* __restrict__ words, uint32_t* __restrict__ segcnt,
+    uint32_t* __restrict__ spill, uint32_t* __restrict__ spilled, ChtState* __restrict__ state) {
+    __shared__ uint32_t bw[1u << (kChtSegBits - 5)];
+    __shared__ uint32_t red[kChtSegBlock / TBJ_WAVE];
+    const uint32_t p = blockIdx.x, q = cht_seg_of(p, b1, F2);
+    const uint32_t nw = 1u << (sbits - 5), smask = (1u << sbits) - 1;
+    const uint2 bd = bounds[p];
+    for (uint32_t w = threadIdx.x; w < nw; w += kChtSegBlock) bw[w] = 0;
+    __syncthreads();
+    for (uint32_t j = bd.x + threadIdx.x; j < bd.y; j += kChtSegBlock) {
+        const uint32_t h = Rb[j].x & cp.bmask;
+        const uint32_t hl = h & smask, m = 1u << (hl & 31);
+        if (!(atomicOr(&bw[hl >> 5], m) & m)) continue;
+        const uint32_t h2 = cht_next_in_part(h, cp.pmask);
+        if ((h2 >> sbits) == q) {
+            const uint32_t hl2 = h2 & smask, m2 = 1u << (hl2 & 31);
+            if (!(atomicOr(&bw[hl2 >> 5], m2) & m2)) continue;
+        }
+        spill[atomicAdd(&state->spill_ctr, 1u)] = j;
+        atomicOr(&spilled[j >> 5], 1u << (j & 31));
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t w = threadIdx.x; w < nw; w += kChtSegBlock) {
+        words[((uint64_t)q << (sbits - 5)) + w].x = bw[w];
+        c += __popc(bw[w]);
+    }
+    c = (uint32_t)wave_reduce_u64(c);
+    if (lane_id() == 0) red[wave_id()] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int w = 0; w < kChtSegBlock / TBJ_WAVE; w++) t += red[w];
+        segcnt[q] = t;
+    }
+}
+
+// Phase A's spills are the overflow table's first entries.
+__global__ void k_cht_spill_a(ChtState* __restrict__ state) {
+    if (threadIdx.x == 0) state->spill_a = state->spill_ctr;
+}
+
+// segbase: the scanned segcnt (segment q's first rank; segbase[nseg] = all
+// set bits).
+__global__ __launch_bounds__(kChtSegBlock) void k_cht_seg_place(
+    const uint2* __restrict__ Rb, const uint2* __restrict__ bounds, uint32_t b1, uint32_t F2,
+    uint32_t sbits, ChtParams cp, uint2* __restrict__ words,
+    const uint32_t* __restrict__ segbase, const uint32_t* __restrict__ spilled,
+    uint32_t* __restrict__ taken, uint2* __restrict__ arr, uint32_t* __restrict__ spill,
+    ChtState* __restrict__ state) {
+    __shared__ uint32_t bw[1u << (kChtSegBits - 5)], rk[1u << (kChtSegBits - 5)];
+    __shared__ uint32_t tk[1u << (kChtSegBits - 5)];
+    __shared__ uint2 dense[1u << kChtSegBits];
+    __shared__ uint32_t scratch[kChtSegBlock / TBJ_WAVE + 1];
+    const uint32_t p = blockIdx.x, q = cht_seg_of(p, b1, F2);
+    const uint32_t nw = 1u << (sbits - 5), smask = (1u << sbits) - 1;
+    const uint64_t w0 = (uint64_t)q << (sbits - 5);
+    const uint32_t base = segbase[q], n = segbase[q + 1] - base;
+    const uint2 bd = bounds[p];
+    for (uint32_t w = threadIdx.x; w < nw; w += kChtSegBlock) {
+        const uint32_t x = words[w0 + w].x;
+        bw[w] = x;
+        rk[w] = __popc(x);
+        tk[w] = 0;
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += kChtSegBlock) dense[i] = make_uint2(0, 0);
+    __syncthreads();
+    block_scan_lds<kChtSegBlock>(rk, nw, scratch);  // local ranks; ends with a barrier
+    for (uint32_t w = threadIdx.x; w < nw; w += kChtSegBlock)
+        words[w0 + w] = make_uint2(bw[w], base + rk[w]);
+    for (uint32_t j = bd.x + threadIdx.x; j < bd.y; j += kChtSegBlock) {
+        if ((spilled[j >> 5] >> (j & 31)) & 1u) continue;
+        const uint2 t = Rb[j];
+        const uint32_t hl = t.x & cp.bmask & smask, w = hl >> 5;
+        const uint32_t lp = rk[w] + __popc(bw[w] & ((1u << (hl & 31)) - 1u));
+        const uint32_t m = 1u << (lp & 31);
+        if (!(atomicOr(&tk[lp >> 5], m) & m)) {
+            dense[lp] = t;
+            continue;
+        }
+        const uint32_t lp2 = lp + 1;
+        if (lp2 < n) {
+            const uint32_t m2 = 1u << (lp2 & 31);
+            if (!(atomicOr(&tk[lp2 >> 5], m2) & m2)) {
+                dense[lp2] = t;
+                continue;
+            }
+        }
+        spill[atomicAdd(&state->spill_ctr, 1u)] = j;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += kChtSegBlock) arr[base + i] = dense[i];
+    // the taken bits of slots [base, base + n): local word lw covers global
+    // bits base + 32 lw ..., which straddle two global words unless aligned
+    for (uint32_t lw = threadIdx.x; lw * 32 < n; lw += kChtSegBlock) {
+        const uint32_t v = tk[lw];
+        if (!v) continue;
+        const uint32_t g = base + 32 * lw, sh = g & 31;
+        atomicOr(&taken[g >> 5], v << sh);
+        if (sh && (v >> (32 - sh))) atomicOr(&taken[(g >> 5) + 1], v >> (32 - sh));
+    }
 }
 
 // Overflow table: open addressing, linear probing, (key, payload) with an
@@ -473,7 +602,23 @@ void launch_cht_build(hipStream_t st, const uint2* R, uint64_t nR, uint32_t bmas
                        spill, state, spilled);
     hipLaunchKernelGGL(k_cht_place, dim3(grid_for(nR, 256, 16384)), dim3(256), 0, st, R, nR, p,
                        words, spilled, taken, arr, spill, state);
-    hipLaunchKernelGGL(k_cht_dense, dim3(1), dim3(64), 0, st, cnt, nwords, taken, state);
+    hipLaunchKernelGGL(k_cht_dense, dim3(1), dim3(64), 0, st, cnt + nwords, taken, state);
+}
+
+void launch_cht_build_bulk(hipStream_t st, const uint2* Rb, const uint2* bounds, uint32_t b1,
+                           uint32_t pbits, uint32_t sbits, uint32_t bmask, uint32_t pmask,
+                           uint2* words, uint32_t* segcnt, uint32_t* partial, uint32_t* taken,
+                           uint2* arr, uint32_t* spill, uint32_t* spilled, ChtState* state) {
+    ChtParams p{bmask, pmask, 0};
+    const uint32_t nseg = 1u << pbits, F2 = 1u << (pbits - b1);
+    hipLaunchKernelGGL(k_cht_seg_bits, dim3(nseg), dim3(kChtSegBlock), 0, st, Rb, bounds, b1, F2,
+                       sbits, p, words, segcnt, spill, spilled, state);
+    hipLaunchKernelGGL(k_cht_spill_a, dim3(1), dim3(64), 0, st, state);
+    launch_fill_u32(st, segcnt + nseg, 1, 0);
+    launch_scan(st, segcnt, (uint64_t)nseg + 1, partial, &state->nset);
+    hipLaunchKernelGGL(k_cht_seg_place, dim3(nseg), dim3(kChtSegBlock), 0, st, Rb, bounds, b1, F2,
+                       sbits, p, words, segcnt, spilled, taken, arr, spill, state);
+    hipLaunchKernelGGL(k_cht_dense, dim3(1), dim3(64), 0, st, segcnt + nseg, taken, state);
 }
 
 void launch_cht_overflow(hipStream_t st, const uint2* R, const uint32_t* spill,

@@ -83,7 +83,7 @@ int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
         const uint2* bounds = nullptr;
         uint32_t b1 = 0;
         if (pbits) {
-            TBJ_TRY(partition_relation(C, R, (uint32_t)nR, pbits, st, &Rb, &bounds, &b1));
+            TBJ_TRY(partition_relation(C, R, (uint32_t)nR, pbits, 0, st, &Rb, &bounds, &b1));
         } else {
             // one block of buckets: all of R
             uint2* bd = nullptr;
@@ -107,7 +107,10 @@ int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
 }
 
 // CHT (CHTJoinWrapper.cpp:56-114 with CHTJoin<128 partitions>): bitmap of
-// nextpow2(|R|) bits, partitions of bitmap/128 bits.
+// nextpow2(|R|) bits, partitions of bitmap/128 bits.  Built per R tuple with
+// global atomics or, in bulk (tbj_options.bulk_build), per 4096-position
+// segment of the bitmap in LDS from R partitioned on the positions' high
+// bits (res->radix_bits and res->passes then report that partitioning).
 int run_cht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
             tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
             hipStream_t st) {
@@ -138,11 +141,30 @@ int run_cht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
     TBJ_TRY(W.get("cht_arr", 8ull * nslots, (void**)&arr));
     TBJ_TRY(W.get("cht_spill", 4ull * nR, (void**)&spill));
     TBJ_TRY(W.get("cht_spilled", 4ull * swords, (void**)&spilled));
-    launch_fill_u32(st, (uint32_t*)words, 2ull * nwords, 0);
+    const bool bulk = opt.bulk_build > 0 || (opt.bulk_build == 0 && nR >= kBulkMin);
     launch_fill_u32(st, taken, twords, 0);
     launch_fill_u32(st, spilled, swords, 0);
-    launch_cht_build(st, (const uint2*)R, nR, bmsize - 1, psize - 1, words, cnt, partial, taken,
-                     arr, spill, spilled, &sm->cht);
+    const uint2* Rsrc = (const uint2*)R;  // what the spill list indexes
+    if (bulk) {
+        uint32_t lg = 0, lgp = 0;
+        while ((1u << lg) < bmsize) lg++;
+        while ((1u << lgp) < psize) lgp++;
+        const uint32_t sbits = std::min<uint32_t>(12, lgp), pbits = lg - sbits;
+        const uint2* bounds = nullptr;
+        uint32_t b1 = 0;
+        TBJ_TRY(partition_relation(C, R, (uint32_t)nR, pbits, sbits, st, &Rsrc, &bounds, &b1));
+        uint32_t *segcnt = nullptr, *spart = nullptr;
+        TBJ_TRY(W.get("cht_segcnt", 4ull * ((1ull << pbits) + 1), (void**)&segcnt));
+        TBJ_TRY(W.get("cht_spart", 4ull * scan_partials((1ull << pbits) + 1), (void**)&spart));
+        launch_cht_build_bulk(st, Rsrc, bounds, b1, pbits, sbits, bmsize - 1, psize - 1, words,
+                              segcnt, spart, taken, arr, spill, spilled, &sm->cht);
+        res->radix_bits = (int32_t)pbits;
+        res->passes = pbits > 8 ? 2 : 1;
+    } else {
+        launch_fill_u32(st, (uint32_t*)words, 2ull * nwords, 0);
+        launch_cht_build(st, (const uint2*)R, nR, bmsize - 1, psize - 1, words, cnt, partial,
+                         taken, arr, spill, spilled, &sm->cht);
+    }
     Small h;
     TBJ_HIP(hipMemcpyAsync(&h, sm, sizeof(Small), hipMemcpyDeviceToHost, st));
     TBJ_HIP(hipStreamSynchronize(st));
@@ -153,7 +175,7 @@ int run_cht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uin
     TBJ_TRY(W.get("cht_ov_used", 4ull * ovslots, (void**)&ov_used));
     TBJ_TRY(W.get("cht_ov", 8ull * ovslots, (void**)&ov));
     launch_fill_u32(st, ov_used, ovslots, 0);
-    launch_cht_overflow(st, (const uint2*)R, spill, &sm->cht, nspill, ovslots - 1, ov_used, ov);
+    launch_cht_overflow(st, Rsrc, spill, &sm->cht, nspill, ovslots - 1, ov_used, ov);
     TBJ_HIP(hipEventRecord(C.ev[1], st));
     launch_cht_probe(st, mat, h.cht.dense != 0, (const uint2*)S, nS, bmsize - 1, psize - 1,
                      ovslots - 1, words, taken, arr, nslots, ov_used, ov, &sm->cht, (uint2*)pairs,

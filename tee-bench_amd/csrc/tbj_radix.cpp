@@ -83,11 +83,13 @@ PassPlan plan_passes(uint32_t B, int req_passes) {
     return p;
 }
 
-BinSpec radix_spec(const PassPlan& p, int pass) {
+// key_shift: the plan's bits start at this key bit (0 but for
+// partition_relation).
+BinSpec radix_spec(const PassPlan& p, int pass, uint32_t key_shift = 0) {
     BinSpec s{};
     s.kind = kBinRadix;
-    s.bits = p.B;
-    s.pass_shift = pass == 1 ? 0 : p.b1;
+    s.bits = p.B + key_shift;
+    s.pass_shift = (pass == 1 ? 0 : p.b1) + key_shift;
     s.pass_mask = (pass == 1 ? p.F1 : p.F2) - 1;
     return s;
 }
@@ -134,6 +136,7 @@ struct Call {
     const uint2* Rb;  // the join's inputs: the partitioned relations
     const uint2* Sb;
     uint32_t npieces;  // ranges per partition side in bounds (1, or 8 per-XCD pieces)
+    uint32_t key_shift = 0;  // radix plans: the partition bits start at this key bit
 };
 
 // The learned model, fitted on a strided key sample of R, and the partition
@@ -145,8 +148,8 @@ struct Call {
 int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_range) {
     const PassPlan& plan = k.plan;
     if (!k.learned) {
-        *s1r = *s1s = radix_spec(plan, 1);
-        *s2 = radix_spec(plan, 2);
+        *s1r = *s1s = radix_spec(plan, 1, k.key_shift);
+        *s2 = radix_spec(plan, 2, k.key_shift);
         return TBJ_OK;
     }
     Context& C = k.C;
@@ -551,8 +554,10 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
 }  // namespace
 
 int partition_relation(Context& C, const tbj_tuple* R, uint32_t nR, uint32_t bits,
-                       hipStream_t st, const uint2** Rb, const uint2** bounds, uint32_t* b1) {
-    if (bits == 0 || bits > 2 * (uint32_t)kMaxPassBits || nR == 0) return TBJ_ERR_INVALID;
+                       uint32_t shift, hipStream_t st, const uint2** Rb, const uint2** bounds,
+                       uint32_t* b1) {
+    if (bits == 0 || bits > 2 * (uint32_t)kMaxPassBits || shift + bits > 32 || nR == 0)
+        return TBJ_ERR_INVALID;
     Job J{};
     J.algo = TBJ_RHO;
     J.R = J.S = (const uint2*)R;
@@ -572,6 +577,7 @@ int partition_relation(Context& C, const tbj_tuple* R, uint32_t nR, uint32_t bit
     Call k{C, C.ws, J, o, &res, st, pt, sm, {}, 0, nullptr, nullptr, nullptr, false, J.R, J.S, 1};
     k.plan = plan_passes(bits, 0);
     k.nparts = k.plan.F1 * k.plan.F2;
+    k.key_shift = shift;
     TBJ_TRY(C.ws.get("prel_bounds", sizeof(uint2) * 2 * k.nparts, (void**)&k.bounds));
     TBJ_TRY(partition_exact(k));
     *Rb = k.Rb;

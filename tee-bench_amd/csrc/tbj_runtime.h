@@ -125,13 +125,15 @@ void set_last_message(const char* msg);
 int run_radix_join(Context& C, int algo, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S,
                    uint64_t nS, void* out, uint64_t out_cap, const tbj_options& opt,
                    tbj_result* res, hipStream_t st);
-// R alone through the exact radix passes on the low `bits` bits of the key
+// R alone through the exact radix passes on key bits [shift, shift + bits)
 // (1 <= bits <= 22; the bulk build of a no-partitioning join's table):
 // *Rb = the partitioned copy (session workspace), (*bounds)[p] = {start, end}
-// of partition p = (key bits [0, b1)) * 2^(bits - b1) + (key bits [b1, bits)),
-// p < 2^bits; *b1 = the first pass's bits (bits itself for one pass).
+// of partition p = (key bits [shift, shift + b1)) * 2^(bits - b1) + (key bits
+// [shift + b1, shift + bits)), p < 2^bits; *b1 = the first pass's bits (bits
+// itself for one pass).
 int partition_relation(Context& C, const tbj_tuple* R, uint32_t nR, uint32_t bits,
-                       hipStream_t st, const uint2** Rb, const uint2** bounds, uint32_t* b1);
+                       uint32_t shift, hipStream_t st, const uint2** Rb, const uint2** bounds,
+                       uint32_t* b1);
 int run_pht(Context& C, const tbj_tuple* R, uint64_t nR, const tbj_tuple* S, uint64_t nS,
             tbj_pair* pairs, uint64_t pairs_cap, const tbj_options& opt, tbj_result* res,
             hipStream_t st);

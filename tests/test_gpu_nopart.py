@@ -111,3 +111,40 @@ def test_pht_bulk_build_long_chains(tbj, oracle, algo):
     sk = np.concatenate([np.full(7, 42), np.arange(0, 1 << 20, 3)])
     res = join_all_pairs(tbj, oracle, algo, rel(rk), rel(sk), bulk_build=1)
     assert res.radix_bits == pht_layout_bits(len(rk)) and res.passes == 2
+
+
+@pytest.mark.parametrize("bulk", [1, -1])
+def test_cht_bulk_and_atomic_builds(tbj, oracle, bulk):
+    # duplicates and misses (spills in both phases) through the bitmap built
+    # per segment in LDS and through the per-tuple atomics
+    rng = np.random.default_rng(31)
+    rk = np.concatenate([rng.integers(0, 1 << 19, 300000), [0, 0, 0xFFFFFFFF]])
+    sk = np.concatenate([rng.integers(0, 1 << 19, 500000), [0, 0xFFFFFFFF]])
+    res = join_all_pairs(tbj, oracle, "CHT", rel(rk), rel(sk), bulk_build=bulk)
+    assert (res.radix_bits > 0) == (bulk == 1)
+
+
+def test_cht_bulk_second_choice_leaves_segment(tbj, oracle):
+    # |R| > 2^19: a bitmap of 2^20 bits, CHT partitions of 8192 bits, two
+    # 4096-bit segments each.  Key 4095 + 2^20 shares home 4095 (the last of
+    # segment 0) with key 4095: its next bit, 4096, is free but lies in
+    # segment 1, so it spills.  Key 8191 + 2^20 shares home 8191, the last of
+    # partition 0, whose next bit wraps to 0 in segment 0: it spills too.
+    bm = 1 << 20
+    base = np.arange(1, 600000)
+    base = base[base != 4096]
+    rk = np.concatenate([base, [4095 + bm, 8191 + bm]])
+    sk = np.concatenate([np.arange(0, 610000, 7), [4095, 4095 + bm, 4096, 8191, 8191 + bm,
+                                                   4095 + 2 * bm]])
+    res = join_all_pairs(tbj, oracle, "CHT", rel(rk), rel(sk), bulk_build=1)
+    assert res.radix_bits == 8 and res.max_r_partition >= 2  # tuples in the overflow table
+
+
+def test_cht_bulk_checksum_matches_reference_structure(tbj, oracle):
+    nR, nS = 1 << 20, 1 << 21
+    R = tbj.gen_pk(11111, nR)
+    S = tbj.gen_zipf(22222, nS, nR, 0.9)
+    m, cks, unresolved = oracle.cht(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    assert unresolved == 0
+    res, _ = gu.run(tbj, "CHT", R, S, materialize=False, bulk_build=1)
+    assert (res.matches, res.checksum) == (m, cks) and res.radix_bits == 8

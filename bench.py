@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--extra-steps", type=int, default=8, help="timed steps of c3/c5 per side")
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--joins", type=int, default=4096, help="c4: joins in the batch")
+    ap.add_argument("--ceilings", action="store_true",
+                    help="instead of a workload: time the chip's ceilings for the path's "
+                         "access patterns (copy, scatter runs, random reads and atomics)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="after the timed steps, run one plain 1 GiB device copy (a "
                          "known-size kernel for scaling HBM byte counters)")
@@ -568,6 +571,55 @@ def bench_join64(args, tb):
     }
 
 
+def ceilings(tb):
+    """Rates of the access patterns that bound the path's kernels, on this
+    device: a plain copy; a partition pass's writes as runs of `run` 8-byte
+    tuples into `bins` regions per XCD (tiles of bins x run tuples); random
+    8- and 32-byte reads and u32 atomics into tables of growing size."""
+    import torch
+    n = 1 << 29
+    src = torch.ones(n, dtype=torch.int64, device="cuda:0")
+    dst = torch.empty_like(src)
+
+    def best(f, reps=3):
+        f()
+        return min(f() for _ in range(reps))
+
+    def copy_ms():
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    out = {"copy": {"bytes": 16 * n, "ms": round(best(copy_ms), 4)}}
+    out["copy"]["TB/s"] = round(16 * n / out["copy"]["ms"] / 1e9, 3)
+    runs = []
+    for bins, run in ((64, 64), (128, 32), (256, 16), (256, 32), (512, 8), (512, 16),
+                      (2048, 2), (2048, 4), (2048, 8), (8192, 1), (8192, 2), (16384, 1)):
+        ms = best(lambda: tb.calibrate_runs(src, dst, bins, run))
+        runs.append({"bins_per_xcd": bins, "run_tuples": run, "run_bytes": 8 * run,
+                     "ms": round(ms, 4), "TB/s": round(16 * n / ms / 1e9, 3)})
+    out["scatter_runs"] = runs
+    del dst
+    table = torch.full((1 << 28,), 0x01010101, dtype=torch.int64, device="cuda:0")
+    sink = torch.zeros(1, dtype=torch.int64, device="cuda:0")
+    nprobe = 1 << 28
+    rnd = []
+    for mb in (2, 8, 32, 256, 1024, 2048):
+        row = {"table_MB": mb}
+        for kind, name in ((tb.CAL_READ8, "read8"), (tb.CAL_READ32, "read32"),
+                           (tb.CAL_ATOMIC, "atomic_u32")):
+            ms = best(lambda: tb.calibrate_random(table, mb << 20, nprobe, kind, sink))
+            row[name + "_G/s"] = round(nprobe / ms / 1e6, 1)
+        rnd.append(row)
+    out["random"] = rnd
+    return {"ceilings": out, "device": torch.cuda.get_device_name(0),
+            "note": ("scatter_runs: TB/s counts 8 B read + 8 B written per tuple; random: "
+                     "accesses per second (2^28 per launch, 8 in flight per thread)")}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -576,6 +628,9 @@ def main():
         sys.exit(2)
     import tbjoin as tb
     tb.init(0)
+    if args.ceilings:
+        print(json.dumps(ceilings(tb)), flush=True)
+        return
     if args.workload == "c4":
         line = bench_batch(args, tb)
     elif args.workload == "c5":

@@ -100,6 +100,12 @@ def lib():
         L.tbj_join_noalloc.restype = ctypes.c_int
         L.tbj_release_stream.argtypes = [vp]
         L.tbj_release_stream.restype = ctypes.c_int
+        L.tbj_calibrate_runs.argtypes = [vp, vp, u64, ctypes.c_uint32, ctypes.c_uint32, vp,
+                                         ctypes.POINTER(ctypes.c_float)]
+        L.tbj_calibrate_runs.restype = ctypes.c_int
+        L.tbj_calibrate_random.argtypes = [vp, u64, u64, ctypes.c_int, vp, vp,
+                                           ctypes.POINTER(ctypes.c_float)]
+        L.tbj_calibrate_random.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -457,3 +463,32 @@ def gen_batch_device(arena, specs):
     import torch
     stream = torch.cuda.current_stream(arena.device).cuda_stream
     check(_batch_lib().tbj_gen_batch_device(arena.data_ptr(), specs, len(specs), stream))
+
+
+def calibrate_runs(src, dst, bins, run):
+    """Device ms of a partition-pass write pattern over the int64 device
+    tensors src -> dst (same length n): tiles of bins * run tuples, each
+    written as one run of `run` tuples into each of `bins` regions of its XCD
+    (include/tbj_calib.h)."""
+    import torch
+    ms = ctypes.c_float()
+    check(lib().tbj_calibrate_runs(src.data_ptr(), dst.data_ptr(), src.numel(), bins, run,
+                                   torch.cuda.current_stream(src.device).cuda_stream,
+                                   ctypes.byref(ms)))
+    return ms.value
+
+
+CAL_READ8, CAL_READ32, CAL_ATOMIC = 0, 1, 2
+
+
+def calibrate_random(table, nbytes, nprobe, kind, sink):
+    """Device ms of nprobe random accesses (kind CAL_*) into the first nbytes
+    (a power of two) of the device tensor `table`; sink: a device tensor of
+    at least 8 bytes."""
+    import torch
+    assert table.numel() * table.element_size() >= nbytes
+    ms = ctypes.c_float()
+    check(lib().tbj_calibrate_random(table.data_ptr(), nbytes, nprobe, kind, sink.data_ptr(),
+                                     torch.cuda.current_stream(table.device).cuda_stream,
+                                     ctypes.byref(ms)))
+    return ms.value

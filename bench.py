@@ -193,11 +193,42 @@ def measured_step_bytes(workload_name, algo="RHO"):
     return None
 
 
+def pattern_ceiling(ksym):
+    """The committed calibration's rate (GB/s) for the access pattern of a
+    kernel (profiles/ceilings_mi355x.md, from bench.py --ceilings): a
+    scatter's 128-byte runs into 256 regions per XCD, or a plain copy for a
+    kernel that streams its input and output.  None if not calibrated."""
+    path = os.path.join(ROOT, "profiles", "ceilings_mi355x.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    if ksym == "k_scatter":
+        for line in text.splitlines():
+            cells = [c.strip() for c in line.strip("|").split("|")]
+            if len(cells) == 5 and cells[:3] == ["256", "16", "128"]:
+                return {"GB/s": float(cells[4]) * 1e3, "pattern": "runs of 128 B, 256 bins/XCD"}
+    if ksym in ("k_join_mat", "k_join", "k_join_sort"):
+        # the faster of torch's copy and one 32-KB run per tile (a streamed copy)
+        best = None
+        for line in text.splitlines():
+            if line.startswith("Plain device copy"):
+                best = float(line.split(":")[1].split()[0]) * 1e3
+            cells = [c.strip() for c in line.strip("|").split("|")]
+            if len(cells) == 5 and cells[:2] == ["1", "4096"]:
+                best = max(best or 0.0, float(cells[4]) * 1e3)
+        return {"GB/s": best, "pattern": "streamed copy"} if best else None
+    return None
+
+
 def roofline(kbytes, kms, ksym, wl, **extra):
     achieved = kbytes / (kms * 1e-3) / 1e9
     d = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(ksym, wl),
          "kernel": ksym, "kernel_ms": round(kms, 4), "algorithmic_bytes": kbytes}
+    pc = pattern_ceiling(ksym)
+    if pc:
+        # the same launch against the measured rate of its access pattern
+        d["pattern_ceiling"] = dict(pc, frac=round(achieved / pc["GB/s"], 4))
     d.update(extra)
     return d
 
@@ -596,7 +627,7 @@ def ceilings(tb):
     out = {"copy": {"bytes": 16 * n, "ms": round(best(copy_ms), 4)}}
     out["copy"]["TB/s"] = round(16 * n / out["copy"]["ms"] / 1e9, 3)
     runs = []
-    for bins, run in ((64, 64), (128, 32), (256, 16), (256, 32), (512, 8), (512, 16),
+    for bins, run in ((1, 4096), (64, 64), (128, 32), (256, 16), (256, 32), (512, 8), (512, 16),
                       (2048, 2), (2048, 4), (2048, 8), (8192, 1), (8192, 2), (16384, 1)):
         ms = best(lambda: tb.calibrate_runs(src, dst, bins, run))
         runs.append({"bins_per_xcd": bins, "run_tuples": run, "run_bytes": 8 * run,

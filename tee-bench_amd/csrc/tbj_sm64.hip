@@ -140,6 +140,35 @@ struct Bin64Fn {
         if (drop && (key < lo || key > hi)) return kNone;
         return (full(key) >> pass_shift) & pass_mask;
     }
+
+    // The pass's bins of N keys (kNone where valid is false or the key is
+    // dropped).  The choice of map is the same for every key, so it is made
+    // once: the radix map and the top-bits map (the linear model over all
+    // 64-bit keys) then run as straight-line code, not as N copies of
+    // full()'s branches.
+    template <int N>
+    __device__ __forceinline__ void bins(const uint64_t (&key)[N], const bool (&valid)[N],
+                                         uint32_t (&out)[N]) const {
+        bool keep[N];
+#pragma unroll
+        for (int q = 0; q < N; q++)
+            keep[q] = valid[q] && !(drop && (key[q] < lo || key[q] > hi));
+        if (kind == kBinRadix) {
+#pragma unroll
+            for (int q = 0; q < N; q++)
+                out[q] = keep[q] ? ((((uint32_t)key[q] & radix_mask) >> pass_shift) & pass_mask)
+                                 : kNone;
+        } else if (lin == 2) {
+#pragma unroll
+            for (int q = 0; q < N; q++)
+                out[q] = keep[q] ? (((uint32_t)(key[q] >> hshift) >> pass_shift) & pass_mask)
+                                 : kNone;
+        } else {
+#pragma unroll
+            for (int q = 0; q < N; q++)
+                out[q] = keep[q] ? ((full(key[q]) >> pass_shift) & pass_mask) : kNone;
+        }
+    }
 };
 
 // Loads the model into LDS (no barrier: the caller's next barrier covers it).
@@ -353,13 +382,8 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
     extern __shared__ __align__(16) uint8_t dyn_lds[];
     const uint32_t t = tile_base + blockIdx.x;
     if (t >= *ntiles_dev) return;
-    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
-    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) cnt[i] = 0;
-    if (threadIdx.x == 0) {
-        bmm[0] = ~0ull;
-        bmm[1] = 0;
-    }
-    __syncthreads();
+    // the tile's key loads go out before the partition function's set-up,
+    // which waits on loads of its own
     const TileDesc td = tiles[t];
     const uint64_t* keys = reinterpret_cast<const uint64_t*>(in + td.in_off);
     uint64_t mn = ~0ull, mx = 0;
@@ -370,6 +394,20 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
         const uint32_t j = q * kSm64Block + threadIdx.x;
         kv[q] = j < td.len ? keys[2 * (uint64_t)j] : 0;
     }
+    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64Block) cnt[i] = 0;
+    if (threadIdx.x == 0) {
+        bmm[0] = ~0ull;
+        bmm[1] = 0;
+    }
+    __syncthreads();
+    uint32_t hb[kHistItems];
+    {
+        bool ok[kHistItems];
+#pragma unroll
+        for (int q = 0; q < kHistItems; q++) ok[q] = q * kSm64Block + threadIdx.x < td.len;
+        bin.bins(kv, ok, hb);
+    }
 #pragma unroll
     for (int q = 0; q < kHistItems; q++) {
         const uint32_t j = q * kSm64Block + threadIdx.x;
@@ -377,7 +415,7 @@ __global__ __launch_bounds__(kSm64Block) void k_hist64(
             mn = min(mn, kv[q]);
             mx = max(mx, kv[q]);
         }
-        agg_add(cnt, j < td.len ? bin(kv[q]) : kNone);
+        agg_add(cnt, hb[q]);
     }
     if (mm) {  // tile min / max: waves -> LDS -> one global atomic pair
         wave_minmax_u64(mn, mx);
@@ -447,8 +485,9 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     const uint32_t n = end > tile_base ? end - tile_base : 0;
     if (blockIdx.x >= n) return;
     const uint32_t t = tile_base + xcd_tile(blockIdx.x, n);
-    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
-    const bool measure = CLAIM && cl.mm && (!cl.gate || *cl.gate > cl.gate_min);
+    // the tile's loads go out first: the partition function's set-up waits
+    // on loads of its own (a learned model's fields, the drop gate), and one
+    // workgroup per CU leaves nothing else to cover that wait
     const TileDesc td = tiles[t];
     ulonglong2 v[kSm64ScItems];
 #pragma unroll
@@ -456,17 +495,19 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
         const uint32_t j = k * kSm64ScBlock + threadIdx.x;
         if (j < td.len) v[k] = in[td.in_off + j];
     }
-    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) loff[i] = 0;
-    if (measure && threadIdx.x == 0) {
-        bmm[0] = ~0ull;
-        bmm[1] = 0;
-    }
     // this thread's bin (one per thread): its region's bounds, loaded early
     const uint32_t r0 = CLAIM ? cl.cb0 + td.aux * cl.cmul : 0;
     uint32_t w0 = 0, w1 = 0;
     if (CLAIM && threadIdx.x < nbins) {
         w0 = cl.starts[r0 + threadIdx.x];
         w1 = cl.starts[r0 + threadIdx.x + 1];
+    }
+    const Bin64Fn bin = make_bin64(b, reinterpret_cast<LdsModel64*>(dyn_lds));
+    const bool measure = CLAIM && cl.mm && (!cl.gate || *cl.gate > cl.gate_min);
+    for (uint32_t i = threadIdx.x; i < nbins; i += kSm64ScBlock) loff[i] = 0;
+    if (measure && threadIdx.x == 0) {
+        bmm[0] = ~0ull;
+        bmm[1] = 0;
     }
     // the stored range, read now so that the filter below does not wait for
     // it (it only narrows: a stale value lets an unneeded atomic through)
@@ -478,10 +519,15 @@ __global__ __launch_bounds__(kSm64ScBlock) void k_scatter64(
     }
     __syncthreads();
     uint32_t bb[kSm64ScItems], rank[kSm64ScItems];
+    {
+        uint64_t keys[kSm64ScItems];
+        bool ok[kSm64ScItems];
 #pragma unroll
-    for (uint32_t k = 0; k < kSm64ScItems; k++) {
-        const uint32_t j = k * kSm64ScBlock + threadIdx.x;
-        bb[k] = j < td.len ? bin(v[k].x) : kNone;
+        for (uint32_t k = 0; k < kSm64ScItems; k++) {
+            keys[k] = v[k].x;
+            ok[k] = k * kSm64ScBlock + threadIdx.x < td.len;
+        }
+        bin.bins(keys, ok, bb);
     }
 #pragma unroll
     for (uint32_t k = 0; k < kSm64ScItems; k++) rank[k] = agg_inc(loff, bb[k]);
@@ -581,14 +627,15 @@ __global__ __launch_bounds__(256) void k_sample_rows64(
             const uint32_t i = first + k;
             key[k] = in[i >= lo && i < hi ? i : lo].x;
         }
+        bool ok[16];
+        uint32_t bb[16];
+#pragma unroll
+        for (uint32_t k = 0; k < 16; k++) ok[k] = first + k >= lo && first + k < hi;
+        bin.bins(key, ok, bb);
 #pragma unroll
         for (uint32_t k = 0; k < 16; k++) {
-            const uint32_t i = first + k;
-            if (i >= lo && i < hi) {
-                const uint32_t bb = bin(key[k]);
-                if (bb != kNone) atomicAdd(&c1[bb], 1u);
-                nout += key[k] < bin.kmin || key[k] > bin.kmax;
-            }
+            if (bb[k] != kNone) atomicAdd(&c1[bb[k]], 1u);
+            nout += ok[k] && (key[k] < bin.kmin || key[k] > bin.kmax);
         }
     }
     if (count_out) {

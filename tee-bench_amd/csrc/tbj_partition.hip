@@ -69,9 +69,21 @@ struct BinFn {
     template <bool GMODEL, int N>
     __device__ __forceinline__ void fulls(const uint32_t (&key)[N], const bool (&valid)[N],
                                           uint32_t (&out)[N]) const {
-        if (bs.kind == kBinRadix || lin) {
+        // the map is the same for every key: chosen once, then straight-line
+        // code for all N keys (not N copies of full()'s branches)
+        if (bs.kind == kBinRadix) {
+            const uint32_t mask = bs.bits >= 32 ? 0xFFFFFFFFu : (1u << bs.bits) - 1;
 #pragma unroll
-            for (int q = 0; q < N; q++) out[q] = valid[q] ? full(key[q]) : TBJ_DROP;
+            for (int q = 0; q < N; q++) out[q] = valid[q] ? (key[q] & mask) : TBJ_DROP;
+            return;
+        }
+        if (lin) {
+#pragma unroll
+            for (int q = 0; q < N; q++) {
+                const uint32_t k = key[q];
+                const bool drop = !valid[q] || (bs.drop && (k < dlo || k > dhi));
+                out[q] = drop ? TBJ_DROP : linear(k);
+            }
             return;
         }
         typedef __attribute__((address_space(1))) const uint32_t gu32;

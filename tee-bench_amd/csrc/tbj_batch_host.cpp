@@ -56,7 +56,10 @@ struct Group {
 // The joins of a PHT group whose tables share pbits (PhtJob): jobs [j0, j1)
 // of the group.  pbits > 0: their R ranges (segments, relative to the wave
 // window at base_off) are partitioned in one pass on the low pbits bits
-// before the blocks are placed.
+// before the blocks are placed, in tiles of kPhtTile tuples whatever the
+// radix joins' tile (up to 2^11 bins: the largest tile keeps the runs
+// longest).
+constexpr uint32_t kPhtTile = tbj::kMaxTile;
 struct PhtPart {
     uint32_t pbits;
     uint32_t j0, j1;
@@ -158,13 +161,13 @@ struct tbj_batch_plan {
                     const uint32_t F = 1u << pp.pbits;
                     const BinSpec bs = radix_pass(pp.pbits, 0, F);
                     const uint32_t nseg = pp.j1 - pp.j0;
-                    launch_tiles_from_segs(st, segs, nseg, sm, t1, pp.T1, tune.tile);
-                    launch_hist(st, pp.T1, base, t1, sm, 0, h1, bs, F, tune.tile);
+                    launch_tiles_from_segs(st, segs, nseg, sm, t1, pp.T1, kPhtTile);
+                    launch_hist(st, pp.T1, base, t1, sm, 0, h1, bs, F, kPhtTile);
                     launch_fill_u32(st, q1 + (pp.L1 - 1), 1, 0);
                     launch_tile_offsets(st, pp.T1, h1, o1, t1, segs, sm, F, q1, pp.L1,
                                         (uint32_t*)partial.p, sm + 6);
                     launch_scatter(st, pp.T1, base, (uint2*)c1.p, t1, sm, 0, o1, bs, F, true,
-                                   nullptr, tune.tile);
+                                   nullptr, kPhtTile);
                     batch_pht_place(st, (const uint2*)c1.p, jobs + pp.j0, nseg, segs, q1,
                                     pp.pbits, ptab.p, ppool.p, pool_ctr);
                 }
@@ -417,7 +420,7 @@ int plan_build(tbj_batch_plan* P, const tbj_batch_item* items) {
                         SegDesc& d = segs[j - j0];
                         d.in_off = (uint32_t)(jobs[j].r_off - wlo);
                         d.n = jobs[j].nR;
-                        d.ntiles = (d.n + tile - 1) / tile;
+                        d.ntiles = (d.n + kPhtTile - 1) / kPhtTile;
                         d.first_tile = tiles;
                         d.ngroups = seg_groups(tiles, d.ntiles);
                         d.aux = 0;

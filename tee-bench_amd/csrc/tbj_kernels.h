@@ -108,6 +108,14 @@ void launch_fit(hipStream_t st, const uint32_t* sorted, uint32_t m, uint32_t nse
                 uint32_t bits, const uint32_t* minmax, CdfModel* model);
 void launch_minmax(hipStream_t st, const uint2* in, uint64_t n, uint32_t* minmax);
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
+// Zeroes several u32 arrays in one launch.
+constexpr uint32_t kZeroArrays = 6;
+struct ZeroList {
+    uint32_t* p[kZeroArrays];
+    uint64_t n[kZeroArrays];
+    uint32_t k;
+};
+void launch_zero_multi(hipStream_t st, const ZeroList& z);
 // Sample-sized passes (no histogram read).  k_sample_rows: per first-pass
 // tile row the sampled counts of each first-pass bin (cnt1[pbase(rel) + row *
 // F1 + b1], pbase(S) = F1 * nrowR) and per relation the sampled (b1, b2)

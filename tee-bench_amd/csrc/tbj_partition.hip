@@ -1277,6 +1277,14 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Zeroes up to kZeroArrays arrays in one launch (grid-stride over each).
+__global__ void k_zero_multi(ZeroList z) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t a = 0; a < z.k; a++)
+        for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.n[a]; i += step)
+            z.p[a][i] = 0;
+}
+
 void launch_make_tiles1(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,
                         uint32_t nbins, uint32_t* ntiles_dev) {
     const uint32_t t = (nR + kTile - 1) / kTile + (nS + kTile - 1) / kTile;
@@ -1369,6 +1377,14 @@ void launch_bounds_est(hipStream_t st, const uint32_t* starts, const uint32_t* c
 void launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (!n) return;
     hipLaunchKernelGGL(k_fill_u32, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
+}
+
+void launch_zero_multi(hipStream_t st, const ZeroList& z) {
+    uint64_t most = 0;
+    for (uint32_t a = 0; a < z.k; a++) most = z.n[a] > most ? z.n[a] : most;
+    if (!most) return;
+    const uint64_t grid = std::min<uint64_t>((most + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_zero_multi, dim3((uint32_t)grid), dim3(256), 0, st, z);
 }
 
 }  // namespace tbj

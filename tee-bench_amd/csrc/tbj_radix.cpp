@@ -366,10 +366,8 @@ int partition_sampled(Call& k) {
     TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, false));
     const uint32_t nR1 = F1 * nrowR;  // R's first-pass regions (S's follow)
     launch_make_tiles1(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
-    launch_fill_u32(st, fine, nsub, 0);
-    launch_fill_u32(st, cnt1, np1, 0);
-    launch_fill_u32(st, cursors1, np1, 0);
-    launch_fill_u32(st, cursors2, nsub8, 0);
+    launch_zero_multi(st, ZeroList{{fine, cnt1, cursors1, cursors2},
+                                   {nsub, np1, np1, nsub8}, 4});
     Context& C = k.C;
     if (k.learned) {
         // R's exact key range (what S drops against) is read on the side
@@ -391,8 +389,7 @@ int partition_sampled(Call& k) {
             k.res->fallback = 1;
             s1r = s1s = radix_spec(plan, 1);
             s2 = radix_spec(plan, 2);
-            launch_fill_u32(st, fine, nsub, 0);
-            launch_fill_u32(st, cnt1, np1, 0);
+            launch_zero_multi(st, ZeroList{{fine, cnt1}, {nsub, np1}, 2});
         }
     }
     if (!k.learned) {

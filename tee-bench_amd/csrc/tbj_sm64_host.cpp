@@ -189,8 +189,7 @@ static int run_sm64_once(Context& C, int algo, const tbj_tuple64* R, uint64_t nR
                           sizeof(uint32_t) * scan_partials(std::max<uint64_t>(
                                                  std::max<uint64_t>(np1 + 1, L2s), nparts + 1)),
                           (void**)&partial));
-            launch_fill_u32(st, cnt1, np1, 0);
-            launch_fill_u32(st, cursors1, np1, 0);
+            launch_zero_multi(st, ZeroList{{cnt1, cursors1}, {np1, np1}, 2});
             sm64_make_tiles(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1, true);
             Bin64 s1s_sample = s1s;
             s1s_sample.range = nullptr;  // R's range comes with R's scatter

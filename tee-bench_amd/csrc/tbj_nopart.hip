@@ -73,7 +73,7 @@ __global__ __launch_bounds__(kPhtPlaceBlock) void k_pht_place(
 // output range, and the buckets still in registers are walked again to
 // write the matches.
 constexpr int kProbeBlock = 256;
-constexpr int kProbeItems = 8;
+constexpr int kProbeItems = 16;
 
 template <bool MAT>
 __global__ __launch_bounds__(kProbeBlock) void k_pht_probe(

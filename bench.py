@@ -220,6 +220,24 @@ def pattern_ceiling(ksym):
     return None
 
 
+def random_read_ceiling(table_bytes):
+    """Calibrated random 32-byte reads per second (profiles/ceilings_mi355x.md)
+    for the smallest calibrated table at least table_bytes large (the largest
+    calibrated one beyond it)."""
+    path = os.path.join(ROOT, "profiles", "ceilings_mi355x.md")
+    if not os.path.exists(path):
+        return None
+    rows = []
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 4 and cells[0].endswith(" MB"):
+            rows.append((int(cells[0].split()[0]) << 20, float(cells[2]) * 1e9))
+    if not rows:
+        return None
+    fit = [r for r in sorted(rows) if r[0] >= table_bytes]
+    return (fit[0] if fit else sorted(rows)[-1])
+
+
 def roofline(kbytes, kms, ksym, wl, **extra):
     achieved = kbytes / (kms * 1e-3) / 1e9
     d = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -340,6 +358,7 @@ def bench_join32(args, tb):
     s_in = s_in_range(R, S) if args.workload == "c3" else nS
     pair_b = 8 * matches if mat else 0
     nopart = algo in ("PHT", "NPO_st", "CHT")
+    probe_pattern = None
     if nopart:
         # one shared table, no partitioning: the probe kernel is the step's
         # longest (its time is the library's probe interval, HIP events)
@@ -349,6 +368,20 @@ def bench_join32(args, tb):
         kernels = {"probe": (probe_ms, 8 * nS + 8 * matches + pair_b, psym)}
         # + R read once and written once into the table
         b_alg = 16 * nR + 8 * nS + 8 * matches + pair_b
+        # the probe's own bound: random reads into a table far larger than L2
+        # (PHT: one 32-byte bucket per S tuple in nextpow2(|R|/2) x 32 B; CHT:
+        # a bitmap entry and a dense slot per S tuple, the dense array |R| x 8 B)
+        nb = 1 << max(0, (max(1, nR // 2) - 1).bit_length())
+        table = nb * 32 if algo != "CHT" else nR * 8
+        per_probe = 2 if algo == "CHT" else 1
+        rc = random_read_ceiling(table)
+        probe_pattern = None
+        if rc:
+            rate = per_probe * nS / (probe_ms * 1e-3)
+            probe_pattern = {"unit": "random reads/s", "achieved": round(rate / 1e9, 2),
+                             "ceiling": round(rc[1] / 1e9, 2), "table_MB": rc[0] >> 20,
+                             "frac": round(rate / rc[1], 4),
+                             "pattern": f"{per_probe} random read(s) per S tuple"}
     else:
         jsym = "k_join_sort" if algo in ("RSM", "PSM", "MWAY") else (
             "k_join_mat" if mat else "k_join")
@@ -382,6 +415,8 @@ def bench_join32(args, tb):
         "step_roofline": step_roofline(b_alg, ms_per_step, args.workload, algo),
         "cpu_baseline": None,
     }
+    if nopart and probe_pattern:
+        line["roofline"]["pattern_ceiling"] = probe_pattern
     if comparison:
         line["config"]["comparison"] = comparison
     del dR, dS, pairs

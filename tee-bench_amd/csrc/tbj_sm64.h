@@ -163,7 +163,9 @@ void sm64_fill_u64(hipStream_t st, unsigned long long* p, uint64_t n, unsigned l
 // Count.  presorted: R chunks are already sorted in Rs; otherwise the chunks
 // of several tasks are already sorted in Rout (sm64_sort_shared), and each
 // task of a single-task chunk sorts it in LDS and, materialising, writes the
-// sorted chunk to Rout (same offsets).
+// sorted chunk to Rout (same offsets); materialising, the task of a small
+// single-task chunk may instead stage its pairs in key order at its S
+// positions of Rout and mark runs[r_off] (k_emit64 copies them).
 void sm64_count(hipStream_t st, uint32_t grid, bool mat, bool presorted, const ulonglong2* Rs,
                 ulonglong2* Rout, const ulonglong2* Sb, const SmTask* tasks,
                 const uint32_t* ntasks_dev, unsigned long long* runs,

@@ -1238,6 +1238,57 @@ __device__ __forceinline__ void count_all(KeyAt key_at, PayAt pay_at, uint32_t h
     }
 }
 
+// Staging of a small chunk's pairs (count kernel, materialising).  The
+// task's pairs are laid out as k_emit64 would lay them out -- S tuples in
+// (key, position) order, each followed by its matches along the chains --
+// in the task's own S positions of the free buffer stage (s_off .. s_off +
+// s_len: the sorted R copies there use R positions only); pairs past s_len
+// are not written.  When all of them fit (tm <= s_len: R keys unique, or
+// nearly), the count kernel sets runs[r_off], which no task of a single-task
+// chunk uses otherwise, to tm + 1, and k_emit64 copies the staged pairs to
+// the chunk's output range instead of reading the R chunk again.  Every
+// thread of the block calls it with its S tuple (ok: one of the task's) and
+// that tuple's matches.  A task of at most one wave of S tuples has them all
+// in wave 0, which ranks them with lane reads alone; a larger one goes
+// through L.cnt (S keys and counts) and one barrier.
+static_assert(kSmallS * (sizeof(uint64_t) + sizeof(uint32_t)) <= sizeof(SortLds::cnt),
+              "the small chunk's S keys and counts fit the count array");
+__device__ __forceinline__ void stage_small(const SmTask& tk, bool ok, uint64_t key, uint64_t pay,
+                                            uint32_t cnt, uint32_t hb, SortLds& L,
+                                            const uint32_t* head, const uint16_t* nxt,
+                                            ulonglong2* __restrict__ stage) {
+    const uint32_t ns = tk.s_len;
+    uint32_t off = 0;  // pairs of the task's S tuples ordered before this one
+    if (ns <= (uint32_t)TBJ_WAVE) {
+        if (wave_id() != 0) return;
+        const uint32_t lane = lane_id();
+        for (uint32_t j = 0; j < ns; j++) {
+            const uint64_t kj =
+                ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(key >> 32), (int)j) << 32) |
+                (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)key, (int)j);
+            const uint32_t cj = (uint32_t)__builtin_amdgcn_readlane((int)cnt, (int)j);
+            if (kj < key || (kj == key && j < lane)) off += cj;
+        }
+    } else {
+        uint64_t* sks = reinterpret_cast<uint64_t*>(L.cnt);
+        uint32_t* scnt = reinterpret_cast<uint32_t*>(sks + kSmallS);
+        if (ok) {
+            sks[threadIdx.x] = key;
+            scnt[threadIdx.x] = cnt;
+        }
+        __syncthreads();
+        if (ok)
+            for (uint32_t j = 0; j < ns; j++) {
+                const uint64_t kj = sks[j];
+                if (kj < key || (kj == key && j < threadIdx.x)) off += scnt[j];
+            }
+    }
+    if (!ok || !cnt) return;
+    ulonglong2* o = stage + tk.s_off;
+    for (uint32_t e = head[hash64(key, hb)]; e && off < ns; e = nxt[e - 1])
+        if (L.a[e - 1].x == key) o[off++] = make_ulonglong2(L.a[e - 1].y, pay);
+}
+
 // Count.  counters[0] += matches, counters[1] += sum of (R.payload +
 // S.payload).  MAT: runs[g] += S tuples matching the key run starting at
 // sorted R position g; chunk_vol[rchunk] += pairs of the task.
@@ -1298,12 +1349,13 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         if (threadIdx.x == 0) tm = 0;
         __syncthreads();
         uint64_t mt = 0;
-        if (small) {  // one group: s_len <= kSmallS <= G * kSm64Block
+        if (small) {  // one group: s_len <= kSmallS <= kSm64Block
             uint32_t c[G];
             count_all<G>(key_at, [&](uint32_t m) { return (uint64_t)L.a[m].y; }, hb, head, nxt,
                          sk, sp, ok, c, cks);
 #pragma unroll
             for (int q = 0; q < G; q++) mt += c[q];
+            if (MAT) stage_small(tk, ok[0], sk[0], sp[0], c[0], hb, L, head, nxt, Rout);
         }
         for (uint32_t j0 = 0; !small;) {
             uint32_t lo[G];
@@ -1337,6 +1389,7 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
                 if (threadIdx.x == 0 && tm) atomicAdd(&chunk_vol[tk.rchunk], tm);
             } else if (threadIdx.x == 0) {
                 chunk_vol[tk.rchunk] = tm;
+                if (small) runs[tk.r_off] = tm <= tk.s_len ? tm + 1 : 0;  // stage_small
             }
         }
         __syncthreads();
@@ -1484,7 +1537,9 @@ __global__ void k_clear_shared_runs(const SmTask* __restrict__ tasks, uint32_t n
 }
 
 // Emit.  runs[g] holds the next free output slot of the run starting at g;
-// the S tuples of a task claim run-length blocks from it.
+// the S tuples of a task claim run-length blocks from it.  A small chunk
+// whose pairs the count kernel staged (runs[r_off] = pairs + 1) is copied
+// from its S positions of Rsorted's buffer (the free buffer when small_ok).
 __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6))) void k_emit64(
     const ulonglong2* __restrict__ Rsorted, const ulonglong2* __restrict__ Sb,
     const SmTask* __restrict__ tasks, const uint32_t* __restrict__ ntasks_dev,
@@ -1504,6 +1559,12 @@ __global__ __launch_bounds__(kSm64Block) __attribute__((amdgpu_waves_per_eu(6)))
         const uint32_t n = tk.r_len;
         const uint32_t hb = hash_bits(n);
         if (small_ok && small_single(tk)) {
+            const unsigned long long staged = runs[tk.r_off];
+            if (staged) {  // laid out by the count kernel (stage_small)
+                const unsigned long long o = chunk_off[tk.rchunk] + threadIdx.x;
+                if (threadIdx.x < staged - 1 && o < cap) pairs[o] = Rsorted[tk.s_off + threadIdx.x];
+                continue;
+            }
             // unsorted R (the count kernel did not sort it): every tuple
             // chained, one S tuple per thread ranked by (key, position), the
             // pairs laid out in that order from the chunk's offset

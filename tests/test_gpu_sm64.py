@@ -127,6 +127,26 @@ def test_small_single_tasks_with_duplicates(tbj, oracle, algo):
     check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
 
 
+@pytest.mark.parametrize("algo", ["PSM", "RSM"])
+def test_small_tasks_staged_and_recounted(tbj, oracle, algo):
+    # small single-task chunks whose pairs are at most their S tuples are laid
+    # out by the count kernel and copied by the emit kernel; the chunk holding
+    # a key with 40 R copies, hit by 24 S tuples, has more pairs than S tuples
+    # and is placed from its R chunk instead -- both in one join
+    rng = np.random.default_rng(43)
+    base = rng.choice(1 << 62, 20000, replace=False).astype(np.uint64) + np.uint64(1)
+    rk = np.concatenate([base, np.repeat(base[7], 40)])
+    rk = rk[rng.permutation(len(rk))]
+    sk = np.concatenate([rng.choice(base, 2000), np.repeat(base[7], 24),
+                         rng.integers(1, 1 << 62, 300, dtype=np.uint64)])
+    sk = sk[rng.permutation(len(sk))]
+    R, S = rel64(rk), rel64(sk)
+    n = oracle.pairs64(R[:, 0], R[:, 1], S[:, 0], S[:, 1])[0]
+    res, got = run64(tbj, algo, R, S, cap=n)
+    assert res.max_r_partition <= 2048
+    check_pairs(oracle, R, S, got, res, ordered=algo == "PSM")
+
+
 def sampled_positions(n):
     """Tuples a sample-sized first pass reads: exactly one 16-tuple group in
     every block of 256 tuples, at a hashed position."""

@@ -12,6 +12,8 @@
 // nextpow2(|R_p|) buckets on key bits above the partition bits, chains
 // through next[].  kModeHist restates histogram_join (radix_join.c:373-440):
 // R_p reordered by bucket, S probes a contiguous bucket range.
+#include <algorithm>
+
 #include "tbj_device.h"
 #include "tbj_kernels.h"
 
@@ -161,6 +163,60 @@ __global__ void k_fill_tasks(TaskSrc ts) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= min(*ts.ntasks_dev, ts.task_cap)) return;
     ts.tasks[t] = task_in_part(ts, task_part(ts, t), t);
+}
+
+// ---------------------------------------------------------------------------
+// Output claims of the sort-merge join (the hash joins claim directly from
+// counters[2]).  Returning atomics on one address retire at ~83 M/s on this
+// chip, so a join claiming once per group of 2048 S tuples spends ~3 ms of a
+// 2^29-pair output waiting on the counter.  In chunk mode (OutBuf::chunk > 0) a workgroup claims whole
+// chunks, one atomic each, and hands out slots from its current chunk; the
+// unused tail of each workgroup's last chunk is a gap that launch_out_fix
+// fills afterwards from the pairs above the total.
+
+// Slot of the off-th pair of a group: o1 + off below split, else o2 + off -
+// split (a group that overflows its workgroup's chunk continues in new ones).
+struct OutMap {
+    unsigned long long o1, o2;
+    uint32_t split, pad;
+};
+
+__device__ __forceinline__ unsigned long long out_slot(const OutMap& m, uint64_t off) {
+    return off < m.split ? m.o1 + off : m.o2 + (off - m.split);
+}
+
+// Slots below cap are the caller's; chunk mode keeps side_cap more in the
+// side buffer; beyond that pairs are dropped (pairs_written reports what the
+// caller's buffer holds).
+__device__ __forceinline__ void out_put(const OutBuf& b, unsigned long long p, uint2 v) {
+    if (p < b.cap) b.pairs[p] = v;
+    else if (p - b.cap < b.side_cap) b.side[p - b.cap] = v;
+}
+
+// One thread of the workgroup: tot slots for its current group.  cur is the
+// workgroup's current chunk (LDS; used == chunk before its first claim).
+__device__ __forceinline__ OutMap out_claim(const OutBuf& b, OutTail& cur, uint32_t tot,
+                                            unsigned long long* ctr) {
+    OutMap m;
+    m.o2 = 0;
+    m.split = 0xFFFFFFFFu;
+    m.pad = 0;
+    if (!b.chunk) {
+        m.o1 = tot ? atomicAdd(ctr, (unsigned long long)tot) : 0;
+        return m;
+    }
+    const uint32_t rem = b.chunk - cur.used;
+    m.o1 = cur.base + cur.used;
+    if (tot <= rem) {
+        cur.used += tot;
+        return m;
+    }
+    const uint32_t need = tot - rem, k = (need + b.chunk - 1) / b.chunk;
+    m.o2 = atomicAdd(ctr, (unsigned long long)k * b.chunk);
+    m.split = rem;
+    cur.base = m.o2 + (unsigned long long)(k - 1) * b.chunk;
+    cur.used = need - (k - 1) * b.chunk;
+    return m;
 }
 
 // ---------------------------------------------------------------------------
@@ -678,11 +734,10 @@ __device__ void lds_sort_u64(const unsigned long long (&v)[ITEMS], uint32_t n,
     }
 }
 
-__global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restrict__ Rb,
+__global__ __launch_bounds__(kJoinBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_join_sort(const uint2* __restrict__ Rb,
                                                           const uint2* __restrict__ Sb, TaskSrc ts,
                                                           uint32_t* __restrict__ task_ctr,
-                                                          uint2* __restrict__ pairs, uint64_t cap,
-                                                          bool mat, bool rel,
+                                                          OutBuf out, bool mat, bool rel,
                                                           unsigned long long* __restrict__ counters) {
     constexpr int RITEMS = kJoinCap / kJoinBlock;
     constexpr int G = kJoinProbeItems;  // S tuples searched in lockstep per thread
@@ -692,10 +747,23 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
     __shared__ uint32_t mm[2], big;
     __shared__ uint32_t task_s;
     __shared__ JoinTask tk_s;
-    __shared__ unsigned long long obase_s;
+    __shared__ OutMap omap_s;
+    __shared__ OutTail cur_s;  // the workgroup's current output chunk
     __shared__ unsigned long long red_m[kJoinBlock / TBJ_WAVE], red_c[kJoinBlock / TBJ_WAVE];
+    // materialising: per-wave match totals of each item and each wave's
+    // output offset for each item, double-buffered by group parity (a wave
+    // may count the next group while others still write this one)
+    constexpr int NW = kJoinBlock / TBJ_WAVE;
+    __shared__ uint32_t wtot[2][NW][G];
+    __shared__ uint32_t woff[2][NW][G];
+    const uint32_t lane = lane_id(), w = wave_id();
+    uint32_t ph = 0;
     const uint32_t ntasks = *ts.ntasks_dev;
     uint64_t matches = 0, cks = 0;
+    if (threadIdx.x == 0) {
+        cur_s.base = 0;
+        cur_s.used = out.chunk;
+    }
     for (;;) {
         if (threadIdx.x == 0) {
             task_s = atomicAdd(task_ctr, 1u);
@@ -730,7 +798,6 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
             for (int k = 0; k < G; k++)
                 sn[k] = Sb[tk.s_off + min(base + kJoinBlock * G + k * kJoinBlock + threadIdx.x, slast)];
             uint32_t lo[G], hi[G];
-            uint32_t c = 0;
             // first sorted entry with key >= S key, searched inside the S
             // key's bucket [cnt[b], cnt[b + 1]) (keys outside [kmin, kmax]
             // and S tuples past the range: n, no match)
@@ -758,40 +825,218 @@ __global__ __launch_bounds__(kJoinBlock) void k_join_sort(const uint2* __restric
                 }
                 if (!more) break;
             }
+            uint32_t c[G], fpay[G];  // matches of each item, the first one's R payload
 #pragma unroll
             for (int k = 0; k < G; k++) {
+                c[k] = 0;
+                fpay[k] = 0;
                 for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
-                    c++;
+                    if (!c[k]) fpay[k] = (uint32_t)srt[e];
+                    c[k]++;
                     cks += (uint64_t)(uint32_t)srt[e] + sv[k].y;
                 }
+                matches += c[k];
             }
-            matches += c;
             if (mat) {
-                uint32_t total;
-                const uint32_t ex = block_exclusive_scan<kJoinBlock>(c, scratch, &total);
-                if (threadIdx.x == 0)
-                    obase_s = total ? atomicAdd(&counters[2], (unsigned long long)total) : 0;
-                __syncthreads();
-                uint64_t o = obase_s + ex;
-                if (c) {
+                // output slots item-major (item q's pairs follow those of
+                // items < q; inside an item, wave order, then lane order), so
+                // that a wave's stores of one item are contiguous; wave 0
+                // lays the items out and claims the group's range
 #pragma unroll
-                    for (int k = 0; k < G; k++) {
-                        for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
-                            if (o < cap)
-                                pairs[o] = rel ? make_uint2(sv[k].x, (uint32_t)srt[e])
-                                               : make_uint2((uint32_t)srt[e], sv[k].y);
-                            o++;
-                        }
-                    }
+                for (int k = 0; k < G; k++) {
+                    const uint32_t incl = wave_inclusive_scan(c[k]);
+                    if (lane == TBJ_WAVE - 1) wtot[ph][w][k] = incl;
                 }
                 __syncthreads();
+                if (w == 0) {
+                    uint32_t allq = 0;
+                    if (lane < (uint32_t)G)
+                        for (int ww = 0; ww < NW; ww++) allq += wtot[ph][ww][lane];
+                    const uint32_t inc = wave_inclusive_scan(allq);
+                    const uint32_t tot = __shfl(inc, TBJ_WAVE - 1, TBJ_WAVE);
+                    if (lane < (uint32_t)G) {
+                        uint32_t run = inc - allq;
+                        for (int ww = 0; ww < NW; ww++) {
+                            const uint32_t v = wtot[ph][ww][lane];
+                            woff[ph][ww][lane] = run;
+                            run += v;
+                        }
+                    }
+                    if (lane == 0) omap_s = out_claim(out, cur_s, tot, &counters[2]);
+                }
+                __syncthreads();
+                const OutMap om = omap_s;
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    // (the scan again: cheaper than keeping it live across the barriers)
+                    const uint32_t incl = wave_inclusive_scan(c[k]);
+                    if (!c[k]) continue;
+                    uint64_t off = woff[ph][w][k] + incl - c[k];  // in the group's claim
+                    if (c[k] == 1) {
+                        out_put(out, out_slot(om, off),
+                                rel ? make_uint2(sv[k].x, fpay[k]) : make_uint2(fpay[k], sv[k].y));
+                        continue;
+                    }
+                    for (uint32_t e = lo[k]; e < n && (uint32_t)(srt[e] >> 32) == sv[k].x; e++) {
+                        out_put(out, out_slot(om, off),
+                                rel ? make_uint2(sv[k].x, (uint32_t)srt[e])
+                                    : make_uint2((uint32_t)srt[e], sv[k].y));
+                        off++;
+                    }
+                }
+                ph ^= 1;
             }
 #pragma unroll
             for (int k = 0; k < G; k++) sv[k] = sn[k];
         }
         __syncthreads();
     }
+    if (out.chunk && threadIdx.x == 0) out.tails[blockIdx.x] = cur_s;
     flush_counters(matches, cks, red_m, red_c, counters);
+}
+
+// ---------------------------------------------------------------------------
+// Gap fill after a chunk-mode join.  The gaps are the unused tails of the
+// workgroups' last chunks; every other claimed slot holds a pair.  With
+// W = min(matches, cap), the pairs in non-gap slots at or above W (below
+// cap + side_cap: the rest were dropped) move, in slot order, into the gap
+// slots below W, in slot order; there are exactly as many of them as gap
+// slots below W when nothing was dropped, and at least as many otherwise
+// (the side buffer holds as many slots as all gaps together).  k_out_plan
+// (one workgroup) sorts the gaps and lays out both lists with their prefix
+// lengths; k_out_move copies.
+
+constexpr int kFixBlock = 1024;
+constexpr uint32_t kFixMaxTails = 2048;
+
+// The plan in device memory: total pairs to move, gap count m, then the
+// destination list (m entries) and the source list (m + 1 entries: the
+// stretch before each gap and the one after the last), each as start slots
+// and exclusive prefix lengths with the total appended.
+struct FixPlan {
+    unsigned long long total;
+    uint32_t m, pad;
+};
+struct FixLists {
+    unsigned long long* dst_start;
+    uint32_t* dst_pre;  // m + 1
+    unsigned long long* src_start;
+    uint32_t* src_pre;  // m + 2
+};
+
+__host__ __device__ inline FixLists fix_lists(void* work, uint32_t ntails) {
+    uint8_t* p = (uint8_t*)work + 64;
+    FixLists l;
+    l.dst_start = (unsigned long long*)p;
+    p += sizeof(unsigned long long) * (ntails + 2);
+    l.src_start = (unsigned long long*)p;
+    p += sizeof(unsigned long long) * (ntails + 2);
+    l.dst_pre = (uint32_t*)p;
+    p += sizeof(uint32_t) * (ntails + 2);
+    l.src_pre = (uint32_t*)p;
+    return l;
+}
+
+__global__ __launch_bounds__(kFixBlock) void k_out_plan(OutBuf out, uint32_t ntails,
+                                                        const unsigned long long* __restrict__ counters,
+                                                        void* __restrict__ work) {
+    __shared__ unsigned long long gs[kFixMaxTails];  // gap starts, sorted
+    __shared__ uint32_t gl[kFixMaxTails];            // their lengths
+    __shared__ uint32_t dl[kFixMaxTails + 1];        // destination lengths, then prefix
+    __shared__ uint32_t sl[kFixMaxTails + 2];        // source lengths, then prefix
+    __shared__ uint32_t scratch[kFixBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t m_s;
+    __shared__ unsigned long long keys[kFixMaxTails];
+    const uint32_t chunk = out.chunk;
+    const unsigned long long W = min((unsigned long long)counters[0], (unsigned long long)out.cap);
+    const unsigned long long E = min((unsigned long long)counters[2],
+                                     (unsigned long long)(out.cap + out.side_cap));
+    if (threadIdx.x == 0) m_s = 0;
+    // a tail's gap start, or ~0 without a gap
+    for (uint32_t b = threadIdx.x; b < ntails; b += kFixBlock) {
+        const OutTail t = out.tails[b];
+        keys[b] = t.used < chunk ? t.base + t.used : ~0ull;
+        dl[b] = chunk - t.used;  // (lengths, until the ranks are taken)
+    }
+    __syncthreads();
+    // rank sort: gap starts are distinct (each in its own chunk); tails
+    // without a gap sort last, by index
+    uint32_t rk[(kFixMaxTails + kFixBlock - 1) / kFixBlock];
+    uint32_t q = 0;
+    for (uint32_t b = threadIdx.x; b < ntails; b += kFixBlock, q++) {
+        const unsigned long long k = keys[b];
+        uint32_t r = 0;
+        for (uint32_t x = 0; x < ntails; x++) {
+            const unsigned long long kx = keys[x];
+            r += kx < k || (kx == k && x < b);
+        }
+        rk[q] = r;
+    }
+    q = 0;
+    for (uint32_t b = threadIdx.x; b < ntails; b += kFixBlock, q++) {
+        if (keys[b] != ~0ull) {
+            gs[rk[q]] = keys[b];
+            gl[rk[q]] = dl[b];
+            atomicAdd(&m_s, 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t m = m_s;
+    for (uint32_t i = threadIdx.x; i <= m; i += kFixBlock) {
+        if (i < m) {  // gap i below W
+            const unsigned long long a = gs[i], e = a + gl[i];
+            dl[i] = a < W ? (uint32_t)(min(e, W) - a) : 0;
+        }
+        // the stretch before gap i (after the last gap when i == m), inside [W, E)
+        const unsigned long long prev = i ? gs[i - 1] + gl[i - 1] : 0;
+        const unsigned long long next = i < m ? gs[i] : E;
+        const unsigned long long lo = max(prev, W), hi = min(next, E);
+        sl[i] = hi > lo ? (uint32_t)(hi - lo) : 0;
+    }
+    __syncthreads();
+    FixLists L = fix_lists(work, ntails);
+    for (uint32_t i = threadIdx.x; i <= m; i += kFixBlock) {
+        if (i < m) L.dst_start[i] = gs[i];
+        const unsigned long long prev = i ? gs[i - 1] + gl[i - 1] : 0;
+        L.src_start[i] = max(prev, W);
+    }
+    const uint32_t dtot = block_scan_lds<kFixBlock>(dl, m, scratch);
+    const uint32_t stot = block_scan_lds<kFixBlock>(sl, m + 1, scratch);
+    for (uint32_t i = threadIdx.x; i < m; i += kFixBlock) L.dst_pre[i] = dl[i];
+    for (uint32_t i = threadIdx.x; i <= m; i += kFixBlock) L.src_pre[i] = sl[i];
+    if (threadIdx.x == 0) {
+        L.dst_pre[m] = dtot;
+        L.src_pre[m + 1] = stot;
+        FixPlan* plan = (FixPlan*)work;
+        plan->total = min(dtot, stot);
+        plan->m = m;
+    }
+}
+
+// Last index i of pre[0, n] with pre[i] <= t (pre non-decreasing, pre[0] = 0 <= t).
+__device__ __forceinline__ uint32_t fix_find(const uint32_t* pre, uint32_t n, uint32_t t) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_out_move(OutBuf out, uint32_t ntails,
+                                                  const void* __restrict__ work) {
+    const FixPlan plan = *(const FixPlan*)work;
+    const FixLists L = fix_lists((void*)work, ntails);
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < plan.total;
+         t += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t tt = (uint32_t)t;
+        const uint32_t i = fix_find(L.dst_pre, plan.m, tt);
+        const uint32_t j = fix_find(L.src_pre, plan.m + 1, tt);
+        const unsigned long long dst = L.dst_start[i] + (tt - L.dst_pre[i]);
+        const unsigned long long src = L.src_start[j] + (tt - L.src_pre[j]);
+        out.pairs[dst] = src < out.cap ? out.pairs[src] : out.side[src - out.cap];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -821,10 +1066,10 @@ void launch_fill_tasks(hipStream_t st, const TaskSrc& ts) {
 template <int MODE>
 static void launch_join_t(hipStream_t st, bool mat, bool rel, uint32_t grid, const uint2* Rb,
                           const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr, uint32_t hshift,
-                          uint2* pairs, uint64_t cap, unsigned long long* counters) {
+                          const OutBuf& out, unsigned long long* counters) {
     if (mat)
         hipLaunchKernelGGL((k_join_mat<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts,
-                           task_ctr, hshift, pairs, cap, rel, counters);
+                           task_ctr, hshift, out.pairs, out.cap, rel, counters);
     else
         hipLaunchKernelGGL((k_join<MODE>), dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts,
                            task_ctr, hshift, counters);
@@ -832,18 +1077,47 @@ static void launch_join_t(hipStream_t st, bool mat, bool rel, uint32_t grid, con
 
 void launch_join(hipStream_t st, int mode, bool materialize, bool rel, uint32_t grid,
                  const uint2* Rb, const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr,
-                 uint32_t hshift, uint2* pairs, uint64_t cap, unsigned long long* counters) {
+                 uint32_t hshift, const OutBuf& out, unsigned long long* counters) {
     if (mode == kModeSort) {
         hipLaunchKernelGGL(k_join_sort, dim3(grid), dim3(kJoinBlock), 0, st, Rb, Sb, ts, task_ctr,
-                           pairs, cap, materialize, rel, counters);
+                           out, materialize, rel, counters);
         return;
     }
     if (mode == kModeChain)
-        launch_join_t<kModeChain>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, pairs,
-                                  cap, counters);
+        launch_join_t<kModeChain>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, out,
+                                  counters);
     else
-        launch_join_t<kModeHist>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, pairs,
-                                 cap, counters);
+        launch_join_t<kModeHist>(st, materialize, rel, grid, Rb, Sb, ts, task_ctr, hshift, out,
+                                 counters);
+}
+
+// The hash joins claim once per 4096 S tuples: RHO's moves its pairs at
+// streaming bandwidth anyway, and RHT's spills with the chunk logic compiled
+// in (C2: 3.04 against 2.43 ms).  The sort-merge join, claiming once per
+// 2048 S tuples, is bound by the counter (C2: 3.36 ms, 2.21 with the claims
+// taken out) and claims chunks of up to 8192 slots -- about a quarter of a
+// workgroup's share of S, so that gaps stay a few percent of the output and
+// small joins keep a small side buffer.
+uint32_t out_chunk(int mode, uint64_t nS, uint32_t grid) {
+    if (mode != kModeSort || !grid || grid > kFixMaxTails) return 0;
+    uint64_t c = nS / (4ull * grid);
+    uint32_t chunk = 256;
+    while (chunk < 8192 && 2ull * chunk <= c) chunk *= 2;
+    return chunk;
+}
+
+size_t out_fix_bytes(uint32_t ntails) {
+    return 64 + (sizeof(unsigned long long) * 2 + sizeof(uint32_t) * 2) * ((size_t)ntails + 2);
+}
+
+void launch_out_fix(hipStream_t st, const OutBuf& out, uint32_t ntails,
+                    const unsigned long long* counters, void* work) {
+    if (!out.chunk || !ntails || ntails > kFixMaxTails) return;
+    hipLaunchKernelGGL(k_out_plan, dim3(1), dim3(kFixBlock), 0, st, out, ntails, counters, work);
+    // the moves are at most the gaps below W: ntails * chunk slots
+    const uint64_t most = (uint64_t)ntails * out.chunk;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((most + 255) / 256, 2048);
+    hipLaunchKernelGGL(k_out_move, dim3(grid), dim3(256), 0, st, out, ntails, work);
 }
 
 template <typename K>

@@ -188,12 +188,41 @@ void launch_count_tasks(hipStream_t st, const uint2* bounds, uint32_t nparts, ui
 // Lists the first ts.task_cap tasks (the grid covers task_cap; the count
 // itself stays on the device).
 void launch_fill_tasks(hipStream_t st, const TaskSrc& ts);
+// Where a materialising join writes (tbj_join.hip).  chunk == 0: every
+// group of S tuples claims its slots from counters[2] (dense output).
+// chunk > 0: a workgroup claims whole chunks of `chunk` slots and hands out
+// slots from its current one; tails[w] records workgroup w's last chunk
+// (grid entries), whose unused tail is a gap until launch_out_fix fills it,
+// and slots at or above cap go to side (side_cap = grid * chunk slots, as
+// many as the gaps can hold).
+struct OutTail {
+    unsigned long long base;
+    uint32_t used, pad;
+};
+struct OutBuf {
+    uint2* pairs;
+    uint64_t cap;
+    uint2* side;
+    uint64_t side_cap;
+    OutTail* tails;
+    uint32_t chunk, pad;
+};
+// The chunk of a join in `mode` over nS S tuples on grid workgroups (0: one
+// claim per group of S tuples).
+uint32_t out_chunk(int mode, uint64_t nS, uint32_t grid);
 // Persistent join over the tasks of ts (grid: resident workgroups; those that
 // find no task left exit).  rel: materialise (key, R.payload) tuples instead
 // of (R.payload, S.payload) pairs.
 void launch_join(hipStream_t st, int mode, bool materialize, bool rel, uint32_t grid,
                  const uint2* Rb, const uint2* Sb, const TaskSrc& ts, uint32_t* task_ctr,
-                 uint32_t hshift, uint2* pairs, uint64_t cap, unsigned long long* counters);
+                 uint32_t hshift, const OutBuf& out, unsigned long long* counters);
+// After a chunk-mode join: moves the pairs in slots at or above
+// W = min(counters[0], cap) (matches; pairs and side buffer) into the gaps
+// below W, so that pairs[0, W) holds every written pair.  ntails: the join's
+// grid; work: out_fix_bytes(ntails) bytes of device memory.
+size_t out_fix_bytes(uint32_t ntails);
+void launch_out_fix(hipStream_t st, const OutBuf& out, uint32_t ntails,
+                    const unsigned long long* counters, void* work);
 int join_occupancy(int mode, bool materialize);
 
 // No-partitioning joins (tbj_nopart.hip)

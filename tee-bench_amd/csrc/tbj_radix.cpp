@@ -522,8 +522,20 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
     const uint32_t hshift = learned ? 0 : plan.B;
     const uint32_t occ = (uint32_t)join_occupancy(J.mode, J.mat);
     const uint32_t grid = (uint32_t)C.num_cus * occ;
-    launch_join(st, J.mode, J.mat, J.rel, grid, Rb, Sb, ts, &sm->task_ctr, hshift, J.pairs,
-                J.mat ? J.cap : 0, sm->counters);
+    // output claims in chunks where the join would wait on one counter (not
+    // from the fixed no-allocation arena: its joins are small)
+    OutBuf out{J.pairs, J.mat ? J.cap : 0, nullptr, 0, nullptr, 0, 0};
+    void* fix_work = nullptr;
+    out.chunk = J.mat && !W.arena() ? out_chunk(J.mode, nS, grid) : 0;
+    if (out.chunk) {
+        out.side_cap = (uint64_t)grid * out.chunk;
+        TBJ_TRY(W.get("out_side", sizeof(uint2) * out.side_cap, (void**)&out.side));
+        TBJ_TRY(W.get("out_tails", sizeof(OutTail) * grid, (void**)&out.tails));
+        TBJ_TRY(W.get("out_fix", out_fix_bytes(grid), &fix_work));
+    }
+    launch_join(st, J.mode, J.mat, J.rel, grid, Rb, Sb, ts, &sm->task_ctr, hshift, out,
+                sm->counters);
+    if (out.chunk) launch_out_fix(st, out, grid, sm->counters, fix_work);
     TBJ_HIP(hipGetLastError());
     TBJ_TRY(pt.mark(kPhJoin));
     TBJ_HIP(hipEventRecord(C.ev[2], st));
@@ -537,7 +549,7 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
     *overflow_out = h.overflow != 0;
     res->matches = h.counters[0];
     res->checksum = h.counters[1];
-    res->pairs_written = J.mat ? std::min<uint64_t>(h.counters[2], J.cap) : 0;
+    res->pairs_written = J.mat ? std::min<uint64_t>(h.counters[0], J.cap) : 0;
     float ms = 0;
     TBJ_HIP(hipEventElapsedTime(&ms, C.ev[0], C.ev[1]));
     res->ms_partition = ms;

@@ -41,3 +41,56 @@ def test_psm_uses_range_partitions(tbj, oracle):
     S = tbj.gen_zipf(2, nS, nR, 1.0)
     res = assert_join(tbj, oracle, "PSM", R, S)
     assert res.partitioner_used == tbj.PART_LEARNED
+
+
+# The sort-merge join claims its output in chunks per workgroup (one atomic
+# per chunk instead of one per group of S tuples) and afterwards moves the
+# pairs above the total into the unused tails of the workgroups' last chunks;
+# slots past the caller's capacity go to a side buffer meanwhile.  These cases
+# hold the result to the oracle's pairs when the output is sized exactly to
+# the matches, when it is truncated, and when one group's pairs span several
+# chunks.
+
+def assert_pairs_equal(oracle, R, S, got, res, n):
+    _, ra, sa = oracle.pairs(R[:, 0], R[:, 1], S[:, 0], S[:, 1])
+    ga, gb = oracle.sort_pairs(got[:, 0], got[:, 1])
+    assert res.matches == n and res.pairs_written == n
+    assert np.array_equal(ga, ra) and np.array_equal(gb, sa)
+
+
+@pytest.mark.parametrize("algo", ["RSM", "PSM", "MWAY"])
+def test_chunked_output_sized_to_the_matches(tbj, oracle, algo):
+    nR, nS = 1 << 18, 1 << 21  # hundreds of workgroups claim chunks of 512
+    R = tbj.gen_pk(5, nR)
+    S = tbj.gen_fk(6, nS, nR)
+    res, got = gu.run(tbj, algo, R, S, cap=nS)
+    assert_pairs_equal(oracle, R, S, got, res, nS)
+
+
+@pytest.mark.parametrize("algo", ["RSM", "PSM"])
+def test_chunked_output_truncated(tbj, oracle, algo):
+    nR, nS = 1 << 17, 1 << 20
+    R = tbj.gen_pk(7, nR)
+    S = tbj.gen_fk(8, nS, nR)
+    cap = nS // 3 + 17
+    res, got = gu.run(tbj, algo, R, S, cap=cap)
+    assert res.matches == nS and res.pairs_written == cap
+    assert np.array_equal(R[got[:, 0], 0], S[got[:, 1], 0])  # every pair joins
+    # no pair twice: S payloads are row indexes and each S row matches once
+    assert len(np.unique(got[:, 1])) == cap
+
+
+@pytest.mark.parametrize("algo", ["RSM", "PSM"])
+def test_chunked_output_hot_key_spans_chunks(tbj, oracle, algo):
+    # one R key with 3000 copies probed by 400 S tuples: 1.2 M pairs from a
+    # few groups, each far above one chunk, beside ordinary unique keys
+    rng = np.random.default_rng(9)
+    base = rng.choice(1 << 31, 1 << 17, replace=False).astype(np.uint64) + np.uint64(1)
+    rk = np.concatenate([base, np.repeat(base[5], 3000)])
+    rk = rk[rng.permutation(len(rk))]
+    sk = np.concatenate([rng.choice(base, 1 << 19), np.repeat(base[5], 400)])
+    sk = sk[rng.permutation(len(sk))]
+    R, S = rel(rk), rel(sk)
+    n = oracle.count(R[:, 0], S[:, 0])
+    res, got = gu.run(tbj, algo, R, S, cap=n)
+    assert_pairs_equal(oracle, R, S, got, res, n)

@@ -496,14 +496,16 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
 // row's sampled counts per first-pass bin at cnt1[pbase[rel] + row * F1 +
 // b1] (the first-pass regions are laid out row by row, so that a tile's
 // claims touch consecutive words); fine (zeroed, 2 * F1 * F2 words)
-// accumulates (rel, b1, b2) counts from every kFineEvery-th chunk of a row (they
-// only apportion each exact first-pass partition size among its second-pass
-// bins, so a quarter of the sample serves, and the other chunks skip the
-// 128 KB LDS histogram).  A chunk's (b1, b2) counts live in LDS as 16-bit
-// halves of 32-bit words: a chunk holds at most kSampleTiles * kTile / 16 <
-// 2^16 sampled tuples.
-constexpr uint32_t kFineEvery = 4;
-constexpr uint32_t kFineLds = 1u << 16;  // F1 * F2 at most
+// accumulates (rel, b1, b2) counts.  Those only apportion each exact
+// first-pass partition size among its second-pass bins, so a quarter of the
+// sample serves: a chunk counts the (b1, b2) bins of one quarter of the b1
+// values, quarter (row + chunk) mod 4, so that every b1 takes its counts
+// from a like share of the chunks and the LDS histogram is a quarter of
+// F1 * F2 bins (32 KB: several workgroups per CU).  The counts live in LDS
+// as 16-bit halves of 32-bit words: a chunk holds at most kSampleTiles *
+// kTile / 16 < 2^16 sampled tuples.
+constexpr uint32_t kFineParts = 4;
+constexpr uint32_t kFineLds = (1u << 16) / kFineParts;  // (F1 / parts) * F2 at most
 constexpr int kSampleBlock = 512;
 
 __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
@@ -525,9 +527,11 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint32_t t0 = min(r0 + chunk * kSampleTiles, r1), t1 = min(t0 + kSampleTiles, r1);
     if (t0 == t1) return;  // whole workgroup
     const uint32_t nf = F1 * F2;
-    const bool do_fine = chunk % kFineEvery == 0;  // chunk 0 of every row
-    if (do_fine)
-        for (uint32_t i = threadIdx.x; i < nf / 2 + 1 && i < kFineLds / 2; i += kSampleBlock) h[i] = 0;
+    // this chunk's quarter of the b1 values (F1 / nq of them)
+    const uint32_t nq = F1 < kFineParts ? F1 : kFineParts;
+    const uint32_t fq = F1 / nq, qlo = ((rowg + chunk) % nq) * fq;
+    const uint32_t nfq = fq * F2;
+    for (uint32_t i = threadIdx.x; i < nfq / 2 + 1 && i < kFineLds / 2; i += kSampleBlock) h[i] = 0;
     for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock) c1[i] = 0;
     __syncthreads();
     // one evaluation of the partition function per key gives both passes'
@@ -536,25 +540,34 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint32_t s1 = b1f.bs.pass_shift, m1 = b1f.bs.pass_mask;
     const uint32_t s2 = bs2.pass_shift, m2 = bs2.pass_mask;
     const uint32_t lo = t0 * kTile, hi = min(t1 * kTile, n);
-    // blocks of 16 groups = 256 tuples; one thread per block
+    // blocks of 16 groups = 256 tuples, one sampled group (128 B) each.  Eight
+    // lanes share a block's group, two tuples a lane, so that a load
+    // instruction of a wave touches eight lines rather than 64; a wave takes
+    // 64 blocks per round and a lane gathers its 16 keys from eight of them.
     const uint32_t blo = lo / 256, bhi = (hi + 255) / 256;
-    for (uint32_t blk = blo + threadIdx.x; blk < bhi; blk += kSampleBlock) {
-        const uint32_t first = blk * 256 + sampled_group(blk) * 16;
+    const uint32_t lane = threadIdx.x % TBJ_WAVE, wv = threadIdx.x / TBJ_WAVE;
+    constexpr uint32_t kRound = kSampleBlock / TBJ_WAVE * 64;  // blocks per workgroup round
+    for (uint32_t wb = blo + wv * 64; wb < bhi; wb += kRound) {
         uint32_t key[16], full[16];
         bool ok[16];
 #pragma unroll
-        for (uint32_t k = 0; k < 16; k++) {
-            const uint32_t i = first + k;
-            ok[k] = i >= lo && i < hi;
-            key[k] = in[ok[k] ? i : lo].x;
+        for (uint32_t q = 0; q < 8; q++) {
+            const uint32_t blk = wb + q * 8 + lane / 8;
+            const uint32_t i = blk * 256 + sampled_group(blk) * 16 + (lane % 8) * 2;
+#pragma unroll
+            for (uint32_t e = 0; e < 2; e++) {
+                const uint32_t k = 2 * q + e;
+                ok[k] = blk < bhi && i + e >= lo && i + e < hi;
+                key[k] = in[ok[k] ? i + e : lo].x;
+            }
         }
         b1f.template fulls<true>(key, ok, full);
 #pragma unroll
         for (uint32_t k = 0; k < 16; k++) {
             if (full[k] != TBJ_DROP) {
                 const uint32_t b1 = (full[k] >> s1) & m1;
-                if (do_fine) {
-                    const uint32_t f = b1 * F2 + ((full[k] >> s2) & m2);
+                if (b1 - qlo < fq) {
+                    const uint32_t f = (b1 - qlo) * F2 + ((full[k] >> s2) & m2);
                     atomicAdd(&h[f >> 1], 1u << ((f & 1) * 16));
                 }
                 atomicAdd(&c1[b1], 1u);
@@ -565,9 +578,8 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint32_t pbase = rel ? F1 * nrowR : 0;
     for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock)
         if (c1[i]) atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
-    if (!do_fine) return;
-    uint32_t* fr = fine + (uint64_t)rel * nf;
-    for (uint32_t f = threadIdx.x; f < nf; f += kSampleBlock) {
+    uint32_t* fr = fine + (uint64_t)rel * nf + (uint64_t)qlo * F2;
+    for (uint32_t f = threadIdx.x; f < nfq; f += kSampleBlock) {
         const uint32_t v = (h[f >> 1] >> ((f & 1) * 16)) & 0xFFFFu;
         if (v) atomicAdd(&fr[f], v);
     }

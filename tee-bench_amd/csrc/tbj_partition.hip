@@ -286,6 +286,53 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_partials(uint32_t* __restri
     if (threadIdx.x == 0 && grand_total) *grand_total = carry;
 }
 
+// Small arrays (n <= kScanSingleMax): the whole exclusive scan in one
+// workgroup, in rounds of kScanSingleBlock * 8 words carried forward.
+constexpr int kScanSingleBlock = 1024;
+constexpr uint32_t kScanSingleItems = 8;
+constexpr uint32_t kScanSingleRound = kScanSingleBlock * kScanSingleItems;
+constexpr uint64_t kScanSingleMax = 4ull * kScanSingleRound;
+
+__global__ __launch_bounds__(kScanSingleBlock) void k_scan_single(uint32_t* __restrict__ a,
+                                                                  uint64_t n,
+                                                                  uint32_t* __restrict__ grand_total) {
+    __shared__ uint32_t scratch[kScanSingleBlock / TBJ_WAVE + 1];
+    __shared__ uint32_t buf[kScanSingleRound + kScanSingleRound / 32];  // padded
+    uint32_t carry = 0;
+    for (uint64_t base = 0; base < n; base += kScanSingleRound) {
+#pragma unroll
+        for (uint32_t k = 0; k < kScanSingleItems; k++) {
+            const uint32_t j = k * kScanSingleBlock + threadIdx.x;
+            buf[j + (j >> 5)] = base + j < n ? a[base + j] : 0;
+        }
+        __syncthreads();
+        uint32_t v[kScanSingleItems], s = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kScanSingleItems; k++) {
+            const uint32_t j = threadIdx.x * kScanSingleItems + k;
+            v[k] = buf[j + (j >> 5)];
+            s += v[k];
+        }
+        uint32_t total;
+        uint32_t run = carry + block_exclusive_scan<kScanSingleBlock>(s, scratch, &total);
+#pragma unroll
+        for (uint32_t k = 0; k < kScanSingleItems; k++) {
+            const uint32_t j = threadIdx.x * kScanSingleItems + k;
+            buf[j + (j >> 5)] = run;
+            run += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t k = 0; k < kScanSingleItems; k++) {
+            const uint32_t j = k * kScanSingleBlock + threadIdx.x;
+            if (base + j < n) a[base + j] = buf[j + (j >> 5)];
+        }
+        carry += total;
+        __syncthreads();  // buf is refilled by the next round
+    }
+    if (threadIdx.x == 0 && grand_total) *grand_total = carry;
+}
+
 __global__ __launch_bounds__(kScanBlock) void k_scan_down(uint32_t* __restrict__ a, uint64_t n,
                                                           const uint32_t* __restrict__ partial) {
     __shared__ uint32_t scratch[kScanBlock / TBJ_WAVE + 1];
@@ -605,24 +652,27 @@ __global__ void k_row_caps(const uint32_t* __restrict__ cnt, uint32_t n,
 }
 
 // Exact size of each first-pass partition p = rel * F1 + b1 (the sum of its
-// rows' filled regions): one thread per partition; *total (zeroed) receives
-// the sum over all partitions.
+// rows' filled regions): one wave per partition, its lanes over the rows;
+// *total (zeroed) receives the sum over all partitions.
 __global__ void k_part_sizes(const uint32_t* __restrict__ starts,
                              const uint32_t* __restrict__ cursors, uint32_t F1, uint32_t nrowR,
                              uint32_t nrowS, uint32_t* __restrict__ npart,
                              uint32_t* __restrict__ total) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= 2 * F1) return;
+    const uint32_t p = (blockIdx.x * blockDim.x + threadIdx.x) / TBJ_WAVE;
+    if (p >= 2 * F1) return;  // whole waves
     const uint32_t rel = p / F1, b1 = p % F1;
     const uint32_t nrow = rel ? nrowS : nrowR;
     const uint32_t r0 = (rel ? F1 * nrowR : 0) + b1;
-    uint32_t tot = 0;
-    for (uint32_t k = 0; k < nrow; k++) {
+    uint64_t tot = 0;
+    for (uint32_t k = lane_id(); k < nrow; k += TBJ_WAVE) {
         const uint32_t r = r0 + k * F1;
         tot += min(cursors[r], starts[r + 1] - starts[r]);
     }
-    npart[p] = tot;
-    if (tot) atomicAdd(total, tot);
+    tot = wave_reduce_u64(tot);
+    if (lane_id() == 0) {
+        npart[p] = (uint32_t)tot;
+        if (tot) atomicAdd(total, (uint32_t)tot);
+    }
 }
 
 // Estimated largest first-pass R partition (16 times its sampled count over
@@ -1225,6 +1275,11 @@ void launch_scan(hipStream_t st, uint32_t* a, uint64_t n, uint32_t* partial,
                  uint32_t* grand_total) {
     const uint32_t nb = (uint32_t)((n + kScanChunk - 1) / kScanChunk);
     if (nb == 0) return;
+    if (n <= kScanSingleMax) {  // one launch instead of three
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanSingleBlock), 0, st, a, n,
+                           grand_total);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kScanBlock), 0, st, a, n, partial);
     hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanBlock), 0, st, partial, nb,
                        grand_total);
@@ -1339,8 +1394,9 @@ void launch_row_caps(hipStream_t st, const uint32_t* cnt, uint32_t n, uint32_t* 
 void launch_part_sizes(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,
                        uint32_t F1, uint32_t nrowR, uint32_t nrowS, uint32_t* npart,
                        uint32_t* total) {
-    hipLaunchKernelGGL(k_part_sizes, dim3((2 * F1 + 255) / 256), dim3(256), 0, st, starts,
-                       cursors, F1, nrowR, nrowS, npart, total);
+    const uint64_t threads = 2ull * F1 * TBJ_WAVE;
+    hipLaunchKernelGGL(k_part_sizes, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st,
+                       starts, cursors, F1, nrowR, nrowS, npart, total);
 }
 
 void launch_est_part_max(hipStream_t st, const uint32_t* cnt1, uint32_t F1, uint32_t nrowR,

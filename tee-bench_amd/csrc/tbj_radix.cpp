@@ -338,7 +338,11 @@ int partition_sampled(Call& k) {
     const double e2 = 2.0 * (double)ntot;
     const double nsx = (double)F2 * (2.0 * F1 + 7.0);
     const double cap2 = 1.1 * e2 + 48.0 * sqrt(nsx * e2) + 640.0 * nsx + 1;
-    if (cap1 >= 4294967295.0 || cap2 >= 4294967295.0) return partition_exact(k);
+    if (cap1 >= 4294967295.0 || cap2 >= 4294967295.0) {
+        // (run_once left the clearing of sm to this path's first launch)
+        TBJ_HIP(hipMemsetAsync(sm, 0, sizeof(Small), st));
+        return partition_exact(k);
+    }
 
     uint2 *C1 = nullptr, *C2 = nullptr;
     TileDesc *tiles1 = nullptr, *tiles2 = nullptr;
@@ -362,12 +366,16 @@ int partition_sampled(Call& k) {
     TBJ_TRY(W.get("first2", sizeof(uint32_t) * (np1 + 1), (void**)&first2));
     TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(std::max<uint64_t>(np1, nsub8) + 1),
                   (void**)&partial));
+    // the call's device scalars (sm) are cleared here with the counters and
+    // cursors, in the call's first launch (before the learned fit, which
+    // writes sm->lin)
+    static_assert(sizeof(Small) % sizeof(uint32_t) == 0, "Small clears as words");
+    launch_zero_multi(st, ZeroList{{(uint32_t*)sm, fine, cnt1, cursors1, cursors2},
+                                   {sizeof(Small) / sizeof(uint32_t), nsub, np1, np1, nsub8}, 5});
     BinSpec s1r, s1s, s2;
     TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, false));
     const uint32_t nR1 = F1 * nrowR;  // R's first-pass regions (S's follow)
     launch_make_tiles1(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
-    launch_zero_multi(st, ZeroList{{fine, cnt1, cursors1, cursors2},
-                                   {nsub, np1, np1, nsub8}, 4});
     Context& C = k.C;
     if (k.learned) {
         // R's exact key range (what S drops against) is read on the side
@@ -458,13 +466,17 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
 
     Small* sm = nullptr;
     TBJ_TRY(W.get("small", sizeof(Small), (void**)&sm));
-    TBJ_HIP(hipEventRecord(C.ev[0], st));
-    TBJ_TRY(pt.begin());
-    TBJ_HIP(hipMemsetAsync(sm, 0, sizeof(Small), st));
-
     Call k{C, W, J, opt, res, st, pt, sm, {}, 0, nullptr, nullptr, nullptr, false, J.R, J.S, 1};
     k.plan = plan_passes((uint32_t)choose_bits(nR, opt), opt.passes);
     const PassPlan& plan = k.plan;
+    const uint64_t ntot = (uint64_t)nR + nS;
+    const bool sampled = allow_sampled && sampled_fits(plan) &&
+                         (opt.sampled_regions > 0 || (opt.sampled_regions == 0 && ntot >= kSampledMin));
+    TBJ_HIP(hipEventRecord(C.ev[0], st));
+    TBJ_TRY(pt.begin());
+    // (the sample-sized path clears sm in its first launch)
+    if (!(plan.passes && sampled)) TBJ_HIP(hipMemsetAsync(sm, 0, sizeof(Small), st));
+
     k.learned = opt.partitioner == TBJ_PART_LEARNED && plan.passes > 0;
     k.nparts = plan.passes ? plan.F1 * plan.F2 : 1;
     res->radix_bits = (int)plan.B;
@@ -476,9 +488,6 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
     TBJ_TRY(W.get("tpartial", sizeof(uint32_t) * scan_partials((uint64_t)nb + 1),
                   (void**)&k.tpartial));
 
-    const uint64_t ntot = (uint64_t)nR + nS;
-    const bool sampled = allow_sampled && sampled_fits(plan) &&
-                         (opt.sampled_regions > 0 || (opt.sampled_regions == 0 && ntot >= kSampledMin));
     if (plan.passes == 0) {
         launch_part_bounds(st, nullptr, nullptr, 1, 1, 0, nR, nS, k.bounds);
     } else if (sampled) {

@@ -110,10 +110,13 @@ void Workspace::release_all() {
 
 int Context::setup(int cus) {
     num_cus = cus;
+    // call and phase timing: read after the call's stream synchronisation,
+    // so no system-scope fence (cache writeback) at each record, which
+    // stalls the work behind it
     for (auto& e : ev)
-        if (!e) TBJ_HIP(hipEventCreate(&e));
+        if (!e) TBJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     for (auto& e : pev)
-        if (!e) TBJ_HIP(hipEventCreate(&e));
+        if (!e) TBJ_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
     if (!side_stream) TBJ_HIP(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
     if (!fork_ev) TBJ_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
     if (!join_ev) TBJ_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));

@@ -116,6 +116,10 @@ struct ZeroList {
     uint32_t k;
 };
 void launch_zero_multi(hipStream_t st, const ZeroList& z);
+// launch_zero_multi and launch_make_tiles1 in one launch (z must not hold
+// *ntiles_dev).
+void launch_start_tiles1(hipStream_t st, const ZeroList& z, TileDesc* tiles, SegDesc* segs,
+                         uint32_t nR, uint32_t nS, uint32_t nbins, uint32_t* ntiles_dev);
 // Sample-sized passes (no histogram read).  k_sample_rows: per first-pass
 // tile row the sampled counts of each first-pass bin (cnt1[pbase(rel) + row *
 // F1 + b1], pbase(S) = F1 * nrowR) and per relation the sampled (b1, b2)

@@ -1055,11 +1055,12 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_cdf(const uint32_t* __restric
 
 // First-pass work descriptors: segment 0 = R, segment 1 = S, tiles of R
 // first.  Tile offsets are relative to the owning relation's base pointer.
-__global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict__ segs,
-                              uint32_t nR, uint32_t nS, uint32_t nbins,
-                              uint32_t* __restrict__ ntiles_dev) {
+// Descriptor of first-pass tile i (R's tiles, then S's); i == 0 also writes
+// the two segments and the tile count.
+__device__ __forceinline__ void make_tile1(uint32_t i, TileDesc* __restrict__ tiles,
+                                           SegDesc* __restrict__ segs, uint32_t nR, uint32_t nS,
+                                           uint32_t nbins, uint32_t* __restrict__ ntiles_dev) {
     const uint32_t tR = (nR + kTile - 1) / kTile, tS = (nS + kTile - 1) / kTile;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < tR) {
         TileDesc d;
         d.in_off = i * kTile;
@@ -1096,6 +1097,26 @@ __global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict_
         segs[1] = b;
         *ntiles_dev = tR + tS;
     }
+}
+
+__global__ void k_make_tiles1(TileDesc* __restrict__ tiles, SegDesc* __restrict__ segs,
+                              uint32_t nR, uint32_t nS, uint32_t nbins,
+                              uint32_t* __restrict__ ntiles_dev) {
+    make_tile1(blockIdx.x * blockDim.x + threadIdx.x, tiles, segs, nR, nS, nbins, ntiles_dev);
+}
+
+// The first launch of a sample-sized call: the arrays of z cleared and the
+// first pass's tiles described, grid-stride (z must not hold *ntiles_dev).
+__global__ void k_start_tiles1(ZeroList z, TileDesc* __restrict__ tiles,
+                               SegDesc* __restrict__ segs, uint32_t nR, uint32_t nS,
+                               uint32_t nbins, uint32_t* __restrict__ ntiles_dev) {
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (uint32_t a = 0; a < z.k; a++)
+        for (uint64_t i = t0; i < z.n[a]; i += step) z.p[a][i] = 0;
+    const uint64_t nt = (uint64_t)(nR + kTile - 1) / kTile + (nS + kTile - 1) / kTile;
+    for (uint64_t i = t0; i < (nt ? nt : 1); i += step)
+        make_tile1((uint32_t)i, tiles, segs, nR, nS, nbins, ntiles_dev);
 }
 
 // ---------------------------------------------------------------------------
@@ -1350,6 +1371,15 @@ __global__ void k_zero_multi(ZeroList z) {
     for (uint32_t a = 0; a < z.k; a++)
         for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < z.n[a]; i += step)
             z.p[a][i] = 0;
+}
+
+void launch_start_tiles1(hipStream_t st, const ZeroList& z, TileDesc* tiles, SegDesc* segs,
+                         uint32_t nR, uint32_t nS, uint32_t nbins, uint32_t* ntiles_dev) {
+    uint64_t most = (uint64_t)(nR + kTile - 1) / kTile + (nS + kTile - 1) / kTile;
+    for (uint32_t a = 0; a < z.k; a++) most = z.n[a] > most ? z.n[a] : most;
+    const uint64_t grid = std::min<uint64_t>((most + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_start_tiles1, dim3((uint32_t)(grid ? grid : 1)), dim3(256), 0, st, z,
+                       tiles, segs, nR, nS, nbins, ntiles_dev);
 }
 
 void launch_make_tiles1(hipStream_t st, TileDesc* tiles, SegDesc* segs, uint32_t nR, uint32_t nS,

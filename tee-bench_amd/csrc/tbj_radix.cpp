@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cstddef>
 
 #include "tbj_kernels.h"
 #include "tbj_runtime.h"
@@ -366,16 +367,20 @@ int partition_sampled(Call& k) {
     TBJ_TRY(W.get("first2", sizeof(uint32_t) * (np1 + 1), (void**)&first2));
     TBJ_TRY(W.get("partial", sizeof(uint32_t) * scan_partials(std::max<uint64_t>(np1, nsub8) + 1),
                   (void**)&partial));
-    // the call's device scalars (sm) are cleared here with the counters and
-    // cursors, in the call's first launch (before the learned fit, which
-    // writes sm->lin)
-    static_assert(sizeof(Small) % sizeof(uint32_t) == 0, "Small clears as words");
-    launch_zero_multi(st, ZeroList{{(uint32_t*)sm, fine, cnt1, cursors1, cursors2},
-                                   {sizeof(Small) / sizeof(uint32_t), nsub, np1, np1, nsub8}, 5});
+    // the call's first launch: its device scalars (sm, but ntiles1, which
+    // the tile descriptors set) cleared with the counters and cursors, and
+    // the first pass's tiles described -- before the learned fit, which
+    // writes sm->lin
+    static_assert(offsetof(Small, ntiles1) == 0 && sizeof(Small) % sizeof(uint32_t) == 0,
+                  "Small clears as words after ntiles1");
+    launch_start_tiles1(st,
+                        ZeroList{{(uint32_t*)sm + 1, fine, cnt1, cursors1, cursors2},
+                                 {sizeof(Small) / sizeof(uint32_t) - 1, nsub, np1, np1, nsub8},
+                                 5},
+                        tiles1, segs1, nR, nS, F1, &sm->ntiles1);
     BinSpec s1r, s1s, s2;
     TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, false));
     const uint32_t nR1 = F1 * nrowR;  // R's first-pass regions (S's follow)
-    launch_make_tiles1(st, tiles1, segs1, nR, nS, F1, &sm->ntiles1);
     Context& C = k.C;
     if (k.learned) {
         // R's exact key range (what S drops against) is read on the side

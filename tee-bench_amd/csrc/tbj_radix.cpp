@@ -543,9 +543,16 @@ int run_once(Context& C, const Job& J, const tbj_options& opt, bool allow_sample
     out.chunk = J.mat && !W.arena() ? out_chunk(J.mode, nS, grid) : 0;
     if (out.chunk) {
         out.side_cap = (uint64_t)grid * out.chunk;
-        TBJ_TRY(W.get("out_side", sizeof(uint2) * out.side_cap, (void**)&out.side));
-        TBJ_TRY(W.get("out_tails", sizeof(OutTail) * grid, (void**)&out.tails));
-        TBJ_TRY(W.get("out_fix", out_fix_bytes(grid), &fix_work));
+        // under a workspace cap too tight for the side buffer the join
+        // claims once per group instead
+        if (W.get("out_side", sizeof(uint2) * out.side_cap, (void**)&out.side) != TBJ_OK ||
+            W.get("out_tails", sizeof(OutTail) * grid, (void**)&out.tails) != TBJ_OK ||
+            W.get("out_fix", out_fix_bytes(grid), &fix_work) != TBJ_OK) {
+            out.chunk = 0;
+            out.side = nullptr;
+            out.side_cap = 0;
+            out.tails = nullptr;
+        }
     }
     launch_join(st, J.mode, J.mat, J.rel, grid, Rb, Sb, ts, &sm->task_ctr, hshift, out,
                 sm->counters);

@@ -94,3 +94,36 @@ def test_chunked_output_hot_key_spans_chunks(tbj, oracle, algo):
     n = oracle.count(R[:, 0], S[:, 0])
     res, got = gu.run(tbj, algo, R, S, cap=n)
     assert_pairs_equal(oracle, R, S, got, res, n)
+
+
+def test_sort_join_under_workspace_caps(tbj, oracle):
+    # the side buffer of the chunked claims is the join's last request, so
+    # at the smallest workspace cap that still runs the join it is refused
+    # and the join claims once per group: every run that succeeds, down to
+    # that cap, must give the oracle's pairs
+    import torch
+    nR, nS = 1 << 16, 1 << 19
+    R = tbj.gen_pk(13, nR)
+    S = tbj.gen_fk(14, nS, nR)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def attempt(cap):
+        tbj.release_stream(stream)
+        try:
+            res, got = gu.run(tbj, "RSM", R, S, cap=nS, workspace_cap=cap)
+        except tbj.TbjError as e:
+            assert e.status == tbj.ERR_OOM
+            return False
+        assert_pairs_equal(oracle, R, S, got, res, nS)
+        return True
+
+    lo, hi = 1 << 20, 1 << 30  # MiB .. GiB: fails at lo, runs at hi
+    assert attempt(hi) and not attempt(lo)
+    while hi - lo > (1 << 16):
+        mid = (lo + hi) // 2
+        if attempt(mid):
+            hi = mid
+        else:
+            lo = mid
+    assert attempt(hi)
+    tbj.release_stream(stream)

@@ -68,7 +68,10 @@ struct ClaimSpec {
     uint32_t cb0, cmul, cstride;
     uint32_t xmul;  // r += xmul * the XCD eighth the tile runs in
     const uint32_t* agg_flag = nullptr;  // last pass: aggregate ranks only while *agg_flag
+    uint2* tile_range = nullptr;  // non-null: each tile's {min, max} key, by tile - tile_base
 };
+// minmax[0..1] = {least minimum, greatest maximum} of n tile ranges.
+void launch_range_reduce(hipStream_t st, const uint2* tile_range, uint32_t n, uint32_t* minmax);
 // offs: the per-tile offsets from launch_tile_offsets (claims == NULL), or
 // sample-sized regions claimed per (tile, bin) (claims).  last_pass: the
 // plan's last pass, whose segments are narrow key ranges where one hot key can

@@ -250,6 +250,33 @@ __global__ __launch_bounds__(256) void k_minmax(const uint2* __restrict__ in, ui
     }
 }
 
+// minmax[0] = least of the n tiles' minima, minmax[1] = greatest of their
+// maxima (one workgroup).
+__global__ __launch_bounds__(1024) void k_range_reduce(const uint2* __restrict__ r, uint32_t n,
+                                                       uint32_t* __restrict__ minmax) {
+    __shared__ uint2 red[1024 / TBJ_WAVE];
+    uint32_t mn = 0xFFFFFFFFu, mx = 0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
+        const uint2 v = r[i];
+        mn = min(mn, v.x);
+        mx = max(mx, v.y);
+    }
+    for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
+    }
+    if (lane_id() == 0) red[wave_id()] = make_uint2(mn, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / TBJ_WAVE; w++) {
+            mn = min(mn, red[w].x);
+            mx = max(mx, red[w].y);
+        }
+        minmax[0] = mn;
+        minmax[1] = mx;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Device-wide exclusive scan of a u32 array (reduce, scan partials, downsweep).
 
@@ -397,6 +424,7 @@ struct Claims {
     uint32_t cb0, cmul, cstride;
     uint32_t xmul;  // r += xmul * the tile's XCD eighth (block index mod 8)
     const uint32_t* agg_flag;  // AGG only while *agg_flag (NULL: always)
+    uint2* tile_range;  // non-null: {min, max} key of each tile, by tile - tile_base
 };
 
 // CLAIM = false: offs row t = the output offset of each bin's run of tile t.
@@ -422,6 +450,7 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
     __shared__ uint2 stage[kPartBlock * IT];
     __shared__ uint16_t sbin[kPartBlock * IT];
     __shared__ uint32_t scratch[kPartBlock / TBJ_WAVE + 1];
+    __shared__ uint2 wrange[CLAIM ? kPartBlock / TBJ_WAVE : 1];
     extern __shared__ uint8_t dyn_lds[];
     LdsModel& lm = *reinterpret_cast<LdsModel*>(dyn_lds);
     uint32_t* loff = reinterpret_cast<uint32_t*>(
@@ -474,6 +503,20 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
                 ok[k] = k * kPartBlock + threadIdx.x < td.len;
             }
             bin.template bins<GMODEL>(keys, ok, b);
+            if (CLAIM && cl.tile_range) {  // the tile's key range, per wave
+                uint32_t mn = 0xFFFFFFFFu, mx = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < IT; k++)
+                    if (ok[k]) {
+                        mn = min(mn, keys[k]);
+                        mx = max(mx, keys[k]);
+                    }
+                for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
+                    mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
+                    mx = max(mx, (uint32_t)__shfl_xor(mx, d, TBJ_WAVE));
+                }
+                if (lane_id() == 0) wrange[wave_id()] = make_uint2(mn, mx);
+            }
         }
         // the aggregated ranks cost a ballot per item: a sample-sized pass
         // takes them only when its sample saw a hot bin
@@ -487,6 +530,14 @@ __global__ __launch_bounds__(kPartBlock) __attribute__((amdgpu_waves_per_eu(IT >
                 if (b[k] != TBJ_DROP) rank[k] = atomicAdd(&loff[b[k]], 1u);
         }
         __syncthreads();
+        if (CLAIM && cl.tile_range && threadIdx.x == 0) {
+            uint2 r = wrange[0];
+            for (int w = 1; w < kPartBlock / TBJ_WAVE; w++) {
+                r.x = min(r.x, wrange[w].x);
+                r.y = max(r.y, wrange[w].y);
+            }
+            cl.tile_range[t - tile_base] = r;
+        }
         // claims: one atomic per non-empty bin, issued now, awaited after the
         // scan and the staging
         uint32_t cnt[kScatterBinsPerThread], base[kScatterBinsPerThread];
@@ -1259,6 +1310,7 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
         cl.cstride = claims->cstride;
         cl.xmul = claims->xmul;
         cl.agg_flag = claims->agg_flag;
+        cl.tile_range = claims->tile_range;
     }
     decltype(&k_scatter<false, false, false>) k;
     if (claim)
@@ -1275,6 +1327,10 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
                     : (last_pass ? k_scatter<false, true, false> : k_scatter<false, false, false>);
     hipLaunchKernelGGL(k, dim3(ntiles), dim3(kPartBlock), lds, st, in, out, tiles, ntiles_dev,
                        tile_base, tile_base + ntiles, offs, cl, bs, nbins);
+}
+
+void launch_range_reduce(hipStream_t st, const uint2* tile_range, uint32_t n, uint32_t* minmax) {
+    hipLaunchKernelGGL(k_range_reduce, dim3(1), dim3(1024), 0, st, tile_range, n, minmax);
 }
 
 void launch_tile_offsets(hipStream_t st, uint32_t max_tiles, const uint16_t* counts,

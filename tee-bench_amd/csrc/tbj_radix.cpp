@@ -381,14 +381,9 @@ int partition_sampled(Call& k) {
     BinSpec s1r, s1s, s2;
     TBJ_TRY(make_specs(k, &s1r, &s1s, &s2, false));
     const uint32_t nR1 = F1 * nrowR;  // R's first-pass regions (S's follow)
-    Context& C = k.C;
+    uint2* tile_range = nullptr;  // learned: R's key range, per tile of R's scatter
     if (k.learned) {
-        // R's exact key range (what S drops against) is read on the side
-        // stream beside R's sample, the judge and R's scatter
-        TBJ_HIP(hipEventRecord(C.fork_ev, st));
-        TBJ_HIP(hipStreamWaitEvent(C.side_stream, C.fork_ev, 0));
-        launch_minmax(C.side_stream, k.J.R, nR, sm->minmax);
-        TBJ_HIP(hipEventRecord(C.join_ev, C.side_stream));
+        TBJ_TRY(W.get("tile_range", sizeof(uint2) * std::max<uint32_t>(1, tR), (void**)&tile_range));
         // R first: its sample, then the judge on R's estimated partitions
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
                            fine, 0);
@@ -397,7 +392,6 @@ int partition_sampled(Call& k) {
         TBJ_TRY(judge_learned(k, &fb));
         launch_fill_u32(st, sm->stats, 2, 0);
         if (fb) {
-            TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
             k.learned = false;
             k.res->fallback = 1;
             s1r = s1s = radix_spec(plan, 1);
@@ -420,14 +414,17 @@ int partition_sampled(Call& k) {
         launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
                        &c1s);
     } else {
-        // R's regions and scatter; then S's sample, regions and scatter,
-        // which drop S keys outside R's range
+        // R's regions and scatter, which also takes R's exact key range (what
+        // S drops against: its reads cost nothing there, where a pass of its
+        // own beside R's sample took that sample from ~40 to ~230 us); then
+        // S's sample, regions and scatter
         TBJ_TRY(k.pt.mark(kPhHist1));
         launch_row_caps(st, cnt1, nR1, starts1);
         launch_scan(st, starts1, (uint64_t)nR1 + 1, partial, &sm->region_total);
         ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
+        c1r.tile_range = tile_range;
         launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
-        TBJ_HIP(hipStreamWaitEvent(st, C.join_ev, 0));
+        launch_range_reduce(st, tile_range, tR, sm->minmax);
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
                            fine, 1);
         launch_row_caps(st, cnt1 + nR1, np1 - nR1, starts1 + nR1);

@@ -143,9 +143,9 @@ struct Call {
 // The learned model, fitted on a strided key sample of R, and the partition
 // functions of both passes.  exact_range: the model spans R's exact key range
 // (a pass over R, beside the sample's sort); otherwise it spans the sample's
-// range, and R's exact range -- which decides what S drops -- is read into
-// sm->minmax by the sample-sized flow (beside R's sample and scatter).  Either
-// way the drop test reads sm->minmax.
+// range, and R's exact range -- which decides what S drops -- is written to
+// sm->minmax by the sample-sized flow after R's scatter (from the tiles'
+// ranges).  Either way the drop test reads sm->minmax.
 int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_range) {
     const PassPlan& plan = k.plan;
     if (!k.learned) {
@@ -165,10 +165,11 @@ int make_specs(Call& k, BinSpec* s1r, BinSpec* s1s, BinSpec* s2, bool exact_rang
     TBJ_TRY(k.W.get("sample", sizeof(uint32_t) * m, (void**)&sample));
     // 64 samples per segment: knot noise of about 1/8 of a segment
     const uint32_t nseg = std::max<uint32_t>(1, std::min<uint32_t>(kLdsKnots, m / 64));
-    // {UINT_MAX, 0}: the start of both ways of measuring R's range
-    launch_fill_u32(k.st, k.sm->minmax, 1, 0xFFFFFFFFu);
-    launch_fill_u32(k.st, k.sm->minmax + 1, 1, 0);
     if (exact_range) {
+        // {UINT_MAX, 0}: the start of the atomic min / max over R (the
+        // sample-sized flow writes the range whole after R's scatter)
+        launch_fill_u32(k.st, k.sm->minmax, 1, 0xFFFFFFFFu);
+        launch_fill_u32(k.st, k.sm->minmax + 1, 1, 0);
         // the sample and its sort (one workgroup) run on the side stream while
         // the exact key range reads all of R
         TBJ_HIP(hipEventRecord(C.fork_ev, k.st));

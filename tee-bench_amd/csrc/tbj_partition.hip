@@ -256,10 +256,18 @@ __global__ __launch_bounds__(1024) void k_range_reduce(const uint2* __restrict__
                                                        uint32_t* __restrict__ minmax) {
     __shared__ uint2 red[1024 / TBJ_WAVE];
     uint32_t mn = 0xFFFFFFFFu, mx = 0;
-    for (uint32_t i = threadIdx.x; i < n; i += 1024) {
-        const uint2 v = r[i];
-        mn = min(mn, v.x);
-        mx = max(mx, v.y);
+    for (uint32_t i0 = 0; i0 < n; i0 += 1024 * 8) {  // eight loads in flight
+        uint2 v[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t i = i0 + k * 1024 + threadIdx.x;
+            v[k] = i < n ? r[i] : make_uint2(0xFFFFFFFFu, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            mn = min(mn, v[k].x);
+            mx = max(mx, v[k].y);
+        }
     }
     for (int d = TBJ_WAVE / 2; d > 0; d >>= 1) {
         mn = min(mn, (uint32_t)__shfl_xor(mn, d, TBJ_WAVE));
@@ -832,18 +840,21 @@ __global__ void k_region_tile_descs(const uint32_t* __restrict__ starts,
                                     uint32_t nrowR, uint32_t nrowS,
                                     TileDesc* __restrict__ tiles, uint32_t tile,
                                     uint32_t seg_is_part) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    // one wave per region, its lanes over the region's tiles (a hot key's
+    // region holds hundreds)
+    const uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / TBJ_WAVE;
+    if (i >= n) return;  // whole waves
     const uint32_t st = starts[i], len = min(cursors[i], starts[i + 1] - st);
     uint32_t part;
-    uint32_t t = first[region_order(i, F1, nrowR, nrowS, &part)];
-    for (uint32_t k = 0; k < len; k += tile, t++) {
+    const uint32_t t0 = first[region_order(i, F1, nrowR, nrowS, &part)];
+    for (uint32_t q = lane_id(); (uint64_t)q * tile < len; q += TBJ_WAVE) {
+        const uint32_t k = q * tile;
         TileDesc d;
         d.in_off = st + k;
         d.len = min(tile, len - k);
         d.seg = seg_is_part ? part : i;
         d.aux = part;
-        tiles[t] = d;
+        tiles[t0 + q] = d;
     }
 }
 
@@ -1511,8 +1522,10 @@ void launch_region_tile_descs(hipStream_t st, const uint32_t* starts, const uint
                               const uint32_t* first, uint32_t n, uint32_t F1, uint32_t nrowR,
                               uint32_t nrowS, TileDesc* tiles, uint32_t tile, bool seg_is_part) {
     if (!n) return;
-    hipLaunchKernelGGL(k_region_tile_descs, dim3((n + 255) / 256), dim3(256), 0, st, starts,
-                       cursors, first, n, F1, nrowR, nrowS, tiles, tile, (uint32_t)seg_is_part);
+    const uint64_t threads = (uint64_t)n * TBJ_WAVE;
+    hipLaunchKernelGGL(k_region_tile_descs, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
+                       st, starts, cursors, first, n, F1, nrowR, nrowS, tiles, tile,
+                       (uint32_t)seg_is_part);
 }
 
 void launch_bounds_x8(hipStream_t st, const uint32_t* starts, const uint32_t* cursors,

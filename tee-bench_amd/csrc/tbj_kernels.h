@@ -67,7 +67,8 @@ struct ClaimSpec {
     uint32_t* overflow;
     uint32_t cb0, cmul, cstride;
     uint32_t xmul;  // r += xmul * the XCD eighth the tile runs in
-    const uint32_t* agg_flag = nullptr;  // last pass: aggregate ranks only while *agg_flag
+    const uint32_t* agg_flag = nullptr;  // aggregate ranks only while *agg_flag (last pass:
+                                         // NULL means always)
     uint2* tile_range = nullptr;  // non-null: each tile's {min, max} key, by tile - tile_base
 };
 // minmax[0..1] = {least minimum, greatest maximum} of n tile ranges.
@@ -76,8 +77,10 @@ void launch_range_reduce(hipStream_t st, const uint2* tile_range, uint32_t n, ui
 // sample-sized regions claimed per (tile, bin) (claims).  last_pass: the
 // plan's last pass, whose segments are narrow key ranges where one hot key can
 // fill most of a wave: lanes sharing a bin then take their ranks from one LDS
-// atomic (in a first pass of two, whose tiles mix the whole key range, the
-// plain per-tuple atomics are cheaper).
+// atomic.  A first pass of two, whose tiles mix the whole key range, takes
+// the plain per-tuple atomics, or the aggregated ranks while
+// *claims->agg_flag (a range partition of skewed keys can put most of a tile
+// in one bin).
 void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out,
                     const TileDesc* tiles, const uint32_t* ntiles_dev, uint32_t tile_base,
                     const uint32_t* offs, const BinSpec& bs, uint32_t nbins, bool last_pass,
@@ -129,11 +132,14 @@ void launch_start_tiles1(hipStream_t st, const ZeroList& z, TileDesc* tiles, Seg
 // counts (fine, 2 * F1 * F2 words, zeroed; F1 * F2 <= 65536).
 // which: 0 = R's rows, 1 = S's rows, 2 = both.  The second-pass bin comes
 // from the same evaluation of the partition function as the first-pass bin
-// (bs2 contributes its pass shift and mask only).
+// (bs2 contributes its pass shift and mask only).  *hot1 (if non-null) is
+// set when a sample chunk finds over 1/8 of its sampled tuples in one
+// first-pass bin.
 void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
                         uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
                         const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
-                        uint32_t* cnt1, uint32_t* fine, int which = 2);
+                        uint32_t* cnt1, uint32_t* fine, int which = 2,
+                        uint32_t* hot1 = nullptr);
 // a[i] += *base for i < n.
 void launch_add_base(hipStream_t st, uint32_t* a, uint32_t n, const uint32_t* base);
 // caps[i] from 16 * cnt[i] for i < n, caps[n] = 0 (scan to region starts).

@@ -617,7 +617,8 @@ constexpr int kSampleBlock = 512;
 __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     const uint2* __restrict__ R, uint32_t nR, const uint2* __restrict__ S, uint32_t nS,
     uint32_t nrowR, uint32_t nrowS, uint32_t row0, BinSpec bsr, BinSpec bss, BinSpec bs2,
-    uint32_t F1, uint32_t F2, uint32_t* __restrict__ cnt1, uint32_t* __restrict__ fine) {
+    uint32_t F1, uint32_t F2, uint32_t* __restrict__ cnt1, uint32_t* __restrict__ fine,
+    uint32_t* __restrict__ hot1) {
     __shared__ uint32_t h[kFineLds / 2];
     __shared__ uint32_t c1[kMaxBins];
     const uint32_t rowg = row0 + blockIdx.x / kChunksPerRow, chunk = blockIdx.x % kChunksPerRow;
@@ -682,8 +683,13 @@ __global__ __launch_bounds__(kSampleBlock) void k_sample_rows(
     }
     __syncthreads();
     const uint32_t pbase = rel ? F1 * nrowR : 0;
+    // a bin with over 1/8 of the chunk's sampled positions (hi - lo) / 16
+    const uint32_t hot_min = max(8u, (hi - lo) / 128);
     for (uint32_t i = threadIdx.x; i < F1; i += kSampleBlock)
-        if (c1[i]) atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
+        if (c1[i]) {
+            atomicAdd(&cnt1[pbase + row * F1 + i], c1[i]);
+            if (hot1 && c1[i] > hot_min) atomicOr(hot1, 1u);
+        }
     uint32_t* fr = fine + (uint64_t)rel * nf + (uint64_t)qlo * F2;
     for (uint32_t f = threadIdx.x; f < nfq; f += kSampleBlock) {
         const uint32_t v = (h[f >> 1] >> ((f & 1) * 16)) & 0xFFFFu;
@@ -1324,9 +1330,10 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
         cl.tile_range = claims->tile_range;
     }
     decltype(&k_scatter<false, false, false>) k;
+    const bool agg = last_pass || (claim && claims->agg_flag);
     if (claim)
-        k = learned ? (last_pass ? k_scatter<true, true, true> : k_scatter<true, false, true>)
-                    : (last_pass ? k_scatter<false, true, true> : k_scatter<false, false, true>);
+        k = learned ? (agg ? k_scatter<true, true, true> : k_scatter<true, false, true>)
+                    : (agg ? k_scatter<false, true, true> : k_scatter<false, false, true>);
     else if (tile == kTile / 2 && !learned)  // other tile shapes: radix passes without claims
         k = last_pass ? k_scatter<false, true, false, kPartItems / 2>
                       : k_scatter<false, false, false, kPartItems / 2>;
@@ -1467,12 +1474,12 @@ void launch_zero_tail(hipStream_t st, uint32_t* hist, uint64_t n, const uint32_t
 void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
                         uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
                         const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,
-                        uint32_t* cnt1, uint32_t* fine, int which) {
+                        uint32_t* cnt1, uint32_t* fine, int which, uint32_t* hot1) {
     const uint32_t row0 = which == 1 ? nrowR : 0;
     const uint32_t rows = which == 0 ? nrowR : which == 1 ? nrowS : nrowR + nrowS;
     if (rows == 0) return;
     hipLaunchKernelGGL(k_sample_rows, dim3(rows * kChunksPerRow), dim3(kSampleBlock), 0, st, R, nR,
-                       S, nS, nrowR, nrowS, row0, bsr, bss, bs2, F1, F2, cnt1, fine);
+                       S, nS, nrowR, nrowS, row0, bsr, bss, bs2, F1, F2, cnt1, fine, hot1);
 }
 
 __global__ void k_add_base(uint32_t* __restrict__ a, uint32_t n, const uint32_t* __restrict__ base) {

@@ -39,6 +39,7 @@ struct Small {  // device scalars, one allocation
     uint32_t total1, total2, overflow, region_total;
     uint32_t ngroups2, est_max;
     uint32_t lin;  // the learned fit gave the linear model
+    uint32_t hot1;  // a sample chunk saw a hot first-pass bin
     uint32_t hot2;  // the sample saw a hot second-pass bin
     unsigned long long counters[4];
 };
@@ -387,7 +388,7 @@ int partition_sampled(Call& k) {
         TBJ_TRY(W.get("tile_range", sizeof(uint2) * std::max<uint32_t>(1, tR), (void**)&tile_range));
         // R first: its sample, then the judge on R's estimated partitions
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
-                           fine, 0);
+                           fine, 0, &sm->hot1);
         launch_est_part_max(st, cnt1, F1, nrowR, sm->stats);
         bool fb = false;
         TBJ_TRY(judge_learned(k, &fb));
@@ -397,20 +398,20 @@ int partition_sampled(Call& k) {
             k.res->fallback = 1;
             s1r = s1s = radix_spec(plan, 1);
             s2 = radix_spec(plan, 2);
-            launch_zero_multi(st, ZeroList{{fine, cnt1}, {nsub, np1}, 2});
+            launch_zero_multi(st, ZeroList{{fine, cnt1, &sm->hot1}, {nsub, np1, 1}, 3});
         }
     }
     if (!k.learned) {
         // ---- sample both relations, size the first pass's regions
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
-                           fine, 2);
+                           fine, 2, &sm->hot1);
         TBJ_TRY(k.pt.mark(kPhHist1));
         launch_row_caps(st, cnt1, np1, starts1);
         launch_scan(st, starts1, (uint64_t)np1 + 1, partial, &sm->region_total);
         TBJ_TRY(k.pt.mark(kPhScan1));
         // ---- pass 1: claims per (tile, bin) in the tile row's regions
-        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
-        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0};
+        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0, &sm->hot1};
+        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0, &sm->hot1};
         launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
         launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
                        &c1s);
@@ -422,17 +423,17 @@ int partition_sampled(Call& k) {
         TBJ_TRY(k.pt.mark(kPhHist1));
         launch_row_caps(st, cnt1, nR1, starts1);
         launch_scan(st, starts1, (uint64_t)nR1 + 1, partial, &sm->region_total);
-        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0};
+        ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0, &sm->hot1};
         c1r.tile_range = tile_range;
         launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
         launch_range_reduce(st, tile_range, tR, sm->minmax);
         launch_sample_rows(st, k.J.R, nR, k.J.S, nS, nrowR, nrowS, s1r, s1s, s2, F1, F2, cnt1,
-                           fine, 1);
+                           fine, 1, &sm->hot1);
         launch_row_caps(st, cnt1 + nR1, np1 - nR1, starts1 + nR1);
         launch_scan(st, starts1 + nR1, (uint64_t)(np1 - nR1) + 1, partial, nullptr);
         launch_add_base(st, starts1 + nR1, np1 - nR1 + 1, &sm->region_total);
         TBJ_TRY(k.pt.mark(kPhScan1));
-        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0};
+        ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0, &sm->hot1};
         launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
                        &c1s);
     }

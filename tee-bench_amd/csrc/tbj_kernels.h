@@ -133,8 +133,8 @@ void launch_start_tiles1(hipStream_t st, const ZeroList& z, TileDesc* tiles, Seg
 // which: 0 = R's rows, 1 = S's rows, 2 = both.  The second-pass bin comes
 // from the same evaluation of the partition function as the first-pass bin
 // (bs2 contributes its pass shift and mask only).  *hot1 (if non-null) is
-// set when a sample chunk finds over 1/8 of its sampled tuples in one
-// first-pass bin.
+// set when a sample chunk finds one first-pass bin holding over 1/8 of its
+// sampled positions (dropped keys count as positions, not in any bin).
 void launch_sample_rows(hipStream_t st, const uint2* R, uint32_t nR, const uint2* S,
                         uint32_t nS, uint32_t nrowR, uint32_t nrowS, const BinSpec& bsr,
                         const BinSpec& bss, const BinSpec& bs2, uint32_t F1, uint32_t F2,

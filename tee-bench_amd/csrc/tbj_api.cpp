@@ -120,6 +120,7 @@ int Context::setup(int cus) {
     if (!side_stream) TBJ_HIP(hipStreamCreateWithFlags(&side_stream, hipStreamNonBlocking));
     if (!fork_ev) TBJ_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
     if (!join_ev) TBJ_HIP(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
+    if (!host_ev) TBJ_HIP(hipEventCreateWithFlags(&host_ev, hipEventDisableTiming));
     return TBJ_OK;
 }
 
@@ -132,6 +133,8 @@ void Context::destroy() {
     if (side_stream) (void)hipStreamDestroy(side_stream), side_stream = nullptr;
     if (fork_ev) (void)hipEventDestroy(fork_ev), fork_ev = nullptr;
     if (join_ev) (void)hipEventDestroy(join_ev), join_ev = nullptr;
+    if (host_ev) (void)hipEventDestroy(host_ev), host_ev = nullptr;
+    if (pinned) (void)hipHostFree(pinned), pinned = nullptr;
 }
 
 int PhaseTimer::begin() {

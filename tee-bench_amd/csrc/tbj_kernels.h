@@ -71,6 +71,11 @@ struct ClaimSpec {
                                          // NULL means always)
     uint2* tile_range = nullptr;  // non-null: each tile's {min, max} key, by tile - tile_base
 };
+// out[r * F2 / 2 + j] = in[r * F2 + j] + in[r * F2 + j + F2 / 2] for rows r
+// of F2 counts: the counts of bins j and j + F2 / 2 (bins of one more key
+// bit) merged into bin j.
+void launch_fold_bins(hipStream_t st, const uint32_t* in, uint32_t rows, uint32_t F2,
+                      uint32_t* out);
 // minmax[0..1] = {least minimum, greatest maximum} of n tile ranges.
 void launch_range_reduce(hipStream_t st, const uint2* tile_range, uint32_t n, uint32_t* minmax);
 // offs: the per-tile offsets from launch_tile_offsets (claims == NULL), or

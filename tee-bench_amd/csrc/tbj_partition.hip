@@ -1347,6 +1347,22 @@ void launch_scatter(hipStream_t st, uint32_t ntiles, const uint2* in, uint2* out
                        tile_base, tile_base + ntiles, offs, cl, bs, nbins);
 }
 
+__global__ void k_fold_bins(const uint32_t* __restrict__ in, uint32_t rows, uint32_t F2,
+                            uint32_t* __restrict__ out) {
+    const uint32_t h = F2 / 2;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)rows * h) return;
+    const uint32_t r = (uint32_t)(i / h), j = (uint32_t)(i % h);
+    out[i] = in[(uint64_t)r * F2 + j] + in[(uint64_t)r * F2 + j + h];
+}
+
+void launch_fold_bins(hipStream_t st, const uint32_t* in, uint32_t rows, uint32_t F2,
+                      uint32_t* out) {
+    const uint64_t n = (uint64_t)rows * (F2 / 2);
+    if (n) hipLaunchKernelGGL(k_fold_bins, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, st,
+                              in, rows, F2, out);
+}
+
 void launch_range_reduce(hipStream_t st, const uint2* tile_range, uint32_t n, uint32_t* minmax) {
     hipLaunchKernelGGL(k_range_reduce, dim3(1), dim3(1024), 0, st, tile_range, n, minmax);
 }

@@ -323,13 +323,14 @@ int partition_sampled(Call& k) {
     Small* sm = k.sm;
     const uint32_t nR = k.J.nR, nS = k.J.nS;
     const uint64_t ntot = (uint64_t)nR + nS;
-    const uint32_t F1 = plan.F1, F2 = plan.F2;
+    const uint32_t F1 = plan.F1;
+    uint32_t F2 = plan.F2;  // a dense R may narrow the second pass (below)
     const uint32_t tR = (nR + kTile - 1) / kTile, tS = (nS + kTile - 1) / kTile;
     const uint32_t T1 = tR + tS;
     const uint32_t nrowR = 8 * rows_per_eighth(tR), nrowS = 8 * rows_per_eighth(tS);
     const uint32_t np1 = F1 * (nrowR + nrowS);  // first-pass regions
-    const uint32_t nsub = 2 * k.nparts;           // second-pass regions
-    const uint32_t nsub8 = 8 * nsub;              // their per-XCD sub-regions
+    uint32_t nsub = 2 * k.nparts;                 // second-pass regions
+    uint32_t nsub8 = 8 * nsub;                    // their per-XCD sub-regions
     // Region sizes are only known on the device; the buffers take the largest
     // sum the capacity rule allows (sum of 16 * counts <= |R| + |S| plus the
     // sampled blocks' rounding; sum of sqrt <= sqrt(n * sum), Cauchy-Schwarz;
@@ -412,9 +413,56 @@ int partition_sampled(Call& k) {
         // ---- pass 1: claims per (tile, bin) in the tile row's regions
         ClaimSpec c1r{starts1, cursors1, &sm->overflow, 0, F1, 1, 0, &sm->hot1};
         ClaimSpec c1s{starts1, cursors1, &sm->overflow, nR1, F1, 1, 0, &sm->hot1};
+        // Dense R: when R's keys span exactly |R| values (as primary keys
+        // 1..|R| do), unique keys fill every partition of the plan one bit
+        // narrower with exactly |R| / 2^(B-1) tuples, and if that fits one
+        // build the narrower second pass writes runs twice as long and the
+        // join builds half as often.  R's scatter takes R's range on the way
+        // (per tile, then one reduction); the host reads it while S's
+        // scatter runs.  Only plans whose first pass the narrowing keeps
+        // (B even: B and B - 1 split with the same first-pass bits).
+        const bool try_narrow = k.opt.radix_bits <= 0 && k.key_shift == 0 && !W.arena() &&
+                                plan.B % 2 == 0 && plan.b2 >= 2 &&
+                                ((uint64_t)nR >> (plan.B - 1)) <= (uint64_t)kJoinCap;
+        uint2* trange = nullptr;
+        if (try_narrow && !k.C.pinned) {
+            void* h = nullptr;
+            TBJ_HIP(hipHostMalloc(&h, 256, hipHostMallocDefault));
+            k.C.pinned = static_cast<uint32_t*>(h);
+        }
+        if (try_narrow) {
+            TBJ_TRY(W.get("tile_range", sizeof(uint2) * std::max<uint32_t>(1, tR),
+                          (void**)&trange));
+            c1r.tile_range = trange;
+        }
         launch_scatter(st, tR, k.J.R, C1, tiles1, &sm->ntiles1, 0, nullptr, s1r, F1, false, &c1r);
+        if (try_narrow) {
+            launch_range_reduce(st, trange, tR, sm->minmax);
+            TBJ_HIP(hipMemcpyAsync(k.C.pinned, sm->minmax, 2 * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, st));
+            TBJ_HIP(hipEventRecord(k.C.host_ev, st));
+        }
         launch_scatter(st, tS, k.J.S, C1, tiles1, &sm->ntiles1, tR, nullptr, s1s, F1, false,
                        &c1s);
+        if (try_narrow) {
+            TBJ_HIP(hipEventSynchronize(k.C.host_ev));
+            const uint32_t mn = k.C.pinned[0], mx = k.C.pinned[1];
+            if (nR && mx >= mn && (uint64_t)mx - mn + 1 == nR) {
+                // the sample's (b1, b2) counts folded to the narrower b2
+                uint32_t* fine_n = nullptr;
+                TBJ_TRY(W.get("fine_narrow", sizeof(uint32_t) * (nsub / 2), (void**)&fine_n));
+                launch_fold_bins(st, fine, 2 * F1, F2, fine_n);
+                fine = fine_n;
+                k.plan = plan_passes(plan.B - 1, 2);
+                s2 = radix_spec(k.plan, 2);
+                F2 = k.plan.F2;
+                k.nparts = F1 * F2;
+                nsub = 2 * k.nparts;
+                nsub8 = 8 * nsub;
+                k.res->radix_bits = (int)k.plan.B;
+                k.res->num_partitions = k.nparts;
+            }
+        }
     } else {
         // R's regions and scatter, which also takes R's exact key range (what
         // S drops against: its reads cost nothing there, where a pass of its

@@ -75,6 +75,11 @@ struct Context {
     // through the two events)
     hipStream_t side_stream = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // a few device scalars the host reads in the middle of a call while the
+    // stream runs on (pinned, allocated by the first call that needs it);
+    // host_ev keeps the system-scope release so that the copy is visible
+    uint32_t* pinned = nullptr;
+    hipEvent_t host_ev = nullptr;
     Workspace ws;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     hipEvent_t pev[24] = {};

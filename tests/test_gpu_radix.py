@@ -259,6 +259,28 @@ def test_sampled_learned_drops_outside_range(tbj, oracle):
     assert res.redone_exact == 0
 
 
+@pytest.mark.parametrize("algo", ["RHO", "RHT", "RSM"])
+def test_dense_keys_narrow_the_second_pass(tbj, oracle, algo):
+    # R's keys 1..|R| span exactly |R| values: the plan one bit narrower (9
+    # instead of 10 bits here) holds exactly |R| / 2^9 = 4096 R tuples, one
+    # build, per partition
+    nR, nS = 1 << 21, 1 << 23
+    R = tbj.gen_pk(11111, nR)
+    S = tbj.gen_fk(22222, nS, nR)
+    res = assert_join(tbj, oracle, algo, R, S, sampled_regions=True)
+    assert (res.passes, res.radix_bits, res.num_partitions) == (2, 9, 1 << 9)
+    assert res.max_r_partition == 4096 and res.redone_exact == 0
+    # key 1 moved to |R| + 2 keeps the plan (the range [2, |R| + 2] spans
+    # |R| + 1 values)
+    R2 = R.copy()
+    R2[R2[:, 0] == 1, 0] = nR + 2
+    res = assert_join(tbj, oracle, algo, R2, S, sampled_regions=True)
+    assert (res.passes, res.radix_bits) == (2, 10)
+    # asked-for bits are kept
+    res = assert_join(tbj, oracle, algo, R, S, sampled_regions=True, radix_bits=10)
+    assert res.radix_bits == 10
+
+
 def test_sample_sized_region_overflow_redoes_exact(tbj, oracle):
     # tuples the sample sees all carry key 1; the rest carry key 2, whose
     # regions the sample believes empty

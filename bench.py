@@ -193,11 +193,12 @@ def measured_step_bytes(workload_name, algo="RHO"):
     return None
 
 
-def pattern_ceiling(ksym):
+def pattern_ceiling(ksym, bins=256):
     """The committed calibration's rate (GB/s) for the access pattern of a
     kernel (profiles/ceilings_mi355x.md, from bench.py --ceilings): a
-    scatter's 128-byte runs into 256 regions per XCD, or a plain copy for a
-    kernel that streams its input and output.  None if not calibrated."""
+    scatter's runs of 4096 / bins tuples into `bins` regions per XCD, or a
+    plain copy for a kernel that streams its input and output.  None if not
+    calibrated."""
     path = os.path.join(ROOT, "profiles", "ceilings_mi355x.md")
     if not os.path.exists(path):
         return None
@@ -205,8 +206,10 @@ def pattern_ceiling(ksym):
     if ksym == "k_scatter":
         for line in text.splitlines():
             cells = [c.strip() for c in line.strip("|").split("|")]
-            if len(cells) == 5 and cells[:3] == ["256", "16", "128"]:
-                return {"GB/s": float(cells[4]) * 1e3, "pattern": "runs of 128 B, 256 bins/XCD"}
+            run = 4096 // bins
+            if len(cells) == 5 and cells[:3] == [str(bins), str(run), str(8 * run)]:
+                return {"GB/s": float(cells[4]) * 1e3,
+                        "pattern": f"runs of {8 * run} B, {bins} bins/XCD"}
     if ksym in ("k_join_mat", "k_join", "k_join_sort"):
         # the faster of torch's copy and one 32-KB run per tile (a streamed copy)
         best = None
@@ -238,12 +241,12 @@ def random_read_ceiling(table_bytes):
     return (fit[0] if fit else sorted(rows)[-1])
 
 
-def roofline(kbytes, kms, ksym, wl, **extra):
+def roofline(kbytes, kms, ksym, wl, bins=256, **extra):
     achieved = kbytes / (kms * 1e-3) / 1e9
     d = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(ksym, wl),
          "kernel": ksym, "kernel_ms": round(kms, 4), "algorithmic_bytes": kbytes}
-    pc = pattern_ceiling(ksym)
+    pc = pattern_ceiling(ksym, bins)
     if pc:
         # the same launch against the measured rate of its access pattern
         d["pattern_ceiling"] = dict(pc, frac=round(achieved / pc["GB/s"], 4))
@@ -411,7 +414,10 @@ def bench_join32(args, tb):
                    "s_dropped": results[-1].s_dropped,
                    "phase_ms": {k: round(v, 4) for k, v in phases.items()},
                    "gen_seconds": round(gen_s, 1)},
-        "roofline": roofline(kbytes, kms, ksym, args.workload),
+        # the pass-2 scatter's bins: the plan's bits less the first pass's
+        "roofline": roofline(kbytes, kms, ksym, args.workload,
+                             bins=1 << (results[-1].radix_bits - (results[-1].radix_bits + 1) // 2)
+                             if results[-1].passes == 2 else 256),
         "step_roofline": step_roofline(b_alg, ms_per_step, args.workload, algo),
         "cpu_baseline": None,
     }
